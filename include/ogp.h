@@ -1,0 +1,177 @@
+/* OpenGPOP-MI355X: C ABI of the propagation and conjunction engine.
+ *
+ * One context = one device (device 0) + one HIP stream + the loaded datasets. A context is not
+ * thread-safe; separate contexts are independent. Every call is synchronous at return, like
+ * the reference (which synchronises after nearly every launch). Errors never exit or throw
+ * across this boundary: every entry point returns OGP_OK or an error code, and
+ * ogp_last_error(ctx) returns the message (ogp_last_error(NULL) the last context-less one).
+ *
+ * Units and layouts follow the reference: metres, m/s, seconds, UTC calendar epoch, and
+ * array-of-structures xyz triples per object (runner.cu:27-33). Column-major 3x3 matrices.
+ */
+#ifndef OGP_H
+#define OGP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    OGP_OK = 0,
+    OGP_ERR_ARGUMENT = 1, /* invalid sizes, set ids, step sizes, degree above file maximum */
+    OGP_ERR_DATA = 2,     /* missing or malformed dataset file */
+    OGP_ERR_DEVICE = 3,   /* HIP runtime error (includes out-of-memory) */
+    OGP_ERR_STATE = 4,    /* call out of sequence (e.g. step before init) */
+    OGP_ERR_NONFINITE = 5 /* non-finite input state */
+};
+
+/* Force kinds. OGP_FORCE_GRAVITY is the product force model; the analytic kinds reproduce the
+ * reference's test forces (unit_tests/include/gpu_test_helpers.cuh) with the reference mock's
+ * time mapping (unit_tests/include/mock_force_model.cuh:69-73, :99). */
+enum {
+    OGP_FORCE_GRAVITY = 0,
+    OGP_FORCE_NONE = 1,
+    OGP_FORCE_CONST = 2,
+    OGP_FORCE_POLY7 = 3,
+    OGP_FORCE_POLY8 = 4,
+    OGP_FORCE_POLY9 = 5,
+    OGP_FORCE_EXP = 6,
+    OGP_FORCE_DAMPED = 7
+};
+
+/* Precision of the gravity harmonics. */
+enum { OGP_PRECISION_FP64 = 0, OGP_PRECISION_SPLIT = 1 };
+
+/* Reference behaviours kept by default (bit set = reference behaviour). */
+enum {
+    OGP_QUIRK_RK8_ROW7 = 1,       /* RK8 a[6][0] = -234/20 (gpu_constants.cuh:31) */
+    OGP_QUIRK_BACKWARD_TIMES = 2, /* backward RK8 stages at (s + c_k) h (force_model.cu:69-71) */
+    OGP_QUIRKS_REFERENCE = 3
+};
+
+typedef struct ogp_config {
+    const char* data_dir;  /* folder with JGM3.gfc, EOP, nutation and leap-second files */
+    int degree;            /* gravity degree/order; < 0 = the file's max_degree */
+    int precision;         /* OGP_PRECISION_* */
+    int split_degree;      /* split mode: degrees <= split_degree stay FP64 (default 2) */
+    int n_corrections;     /* GJ8 / startup corrector iterations; reference: 7 */
+    int quirks;            /* OGP_QUIRK_* bits */
+    int force;             /* OGP_FORCE_* */
+    int year, month, day, hour, minute; /* epoch, UTC (TimeData, Parameters.hpp:19-26) */
+    double second;
+} ogp_config;
+
+typedef struct ogp_context ogp_context;
+
+/* Fills cfg with the reference defaults (degree 70, FP64, 7 corrections, reference quirks,
+ * gravity, epoch 2025-01-01T00:00:00). */
+void ogp_default_config(ogp_config* cfg);
+
+/* Replaces ForceModel::PreInitialize (force_model.cu:23-45): device selection and dataset
+ * loading (FMTimePoints::Initialize time_points.cu:73-124, GravitySource::Initialize
+ * gravity_source.cu:61-135). */
+int ogp_create(const ogp_config* cfg, ogp_context** out);
+void ogp_destroy(ogp_context* ctx);
+const char* ogp_last_error(const ogp_context* ctx);
+/* Size of the library's device allocations (bytes). */
+int64_t ogp_device_bytes(const ogp_context* ctx);
+
+/* ---- force model boundary --------------------------------------------------------------- */
+
+/* GravitySource::ComputeAcceleration with an identity frame (the reference's gravity tests,
+ * fm_gravity_source_test.cu:43-47): ITRS positions -> ITRS accelerations, host arrays. */
+int ogp_gravity_itrs(ogp_context* ctx, const double* r, int64_t n, double* a);
+
+/* ForceModel::ComputePropagationAcceleration (force_model.cu:91-109): GCRS positions at
+ * propagation node `node` of a run with step `step_s` (node time (node - 4) * step_s relative
+ * to the epoch) -> GCRS accelerations. out_stride = 3 * reference outStride (doubles between
+ * consecutive objects' outputs). Host arrays. */
+int ogp_accel(ogp_context* ctx, const double* r, const double* v, int64_t n, double step_s,
+    int64_t node, double* a, int64_t out_stride);
+
+/* FMTimePoints::CalculateParameters + GetParametersDev (time_points.cu:126-207): the
+ * ITRS->GCRS matrix (column-major, 9 per node) and MJD(UTC) of `len` nodes at offsets
+ * offsets_s (seconds) from (epoch - before * step_s). */
+int ogp_time_nodes(ogp_context* ctx, double step_s, int before, const double* offsets_s,
+    int64_t len, double* M, double* mjd);
+
+/* ---- propagation ----------------------------------------------------------------------- */
+
+/* The runner's propagate() (exec/runner.cu:11-70) for n objects with per-object step sizes
+ * and step counts (uneven spans, mixed regimes): GpuStartup::Startup followed by
+ * GpuPropagator::Step up to steps[i] for each object. Host arrays in/out. */
+int ogp_propagate(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    const double* step_s, const int64_t* steps, double* r_out, double* v_out);
+
+/* Same on device-resident SoA arrays (x[n], y[n], z[n] blocks of the given pointers);
+ * nothing crosses PCIe. */
+int ogp_propagate_device(ogp_context* ctx, const double* d_r0_soa, const double* d_v0_soa,
+    int64_t n, const double* step_s, const int64_t* steps, double* d_r_soa, double* d_v_soa);
+
+/* Timing of the last propagate call, from HIP events on the context stream. */
+typedef struct ogp_timing {
+    double total_ms;       /* all device work of the call (nodes + startup + steps) */
+    double nodes_ms;       /* time-node (frame) kernels */
+    double startup_ms;     /* RK8 + startup corrector */
+    double steps_ms;       /* GJ8 chunk kernels */
+    double h2d_ms, d2h_ms; /* host copies (host-array entry points only) */
+    double step_launches;  /* number of GJ8 chunk launches */
+    double object_steps;   /* sum over objects of GJ8 + startup-copy steps */
+    double gj_object_steps; /* object-steps executed by the GJ8 chunk kernels */
+} ogp_timing;
+int ogp_last_timing(const ogp_context* ctx, ogp_timing* t);
+
+/* Stepwise GpuPropagator (gpu_propagator.cuh:50-58): init with n objects and one step size;
+ * each ogp_prop_step performs one GpuPropagator::Step (startup on the first call, startup
+ * nodes for steps 1-4, GJ8 from step 5) and optionally returns r, v and the acceleration
+ * column the reference hands to the conjunction screen (gpu_propagator.cu:68-72). */
+int ogp_prop_init(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
+    int64_t max_steps);
+int ogp_prop_step(ogp_context* ctx, double* r_out, double* v_out, double* a_out);
+
+/* GpuStartup::Startup (gpu_startup.cu:55-68) for n objects: node matrices, s, S and the
+ * acceleration matrix, each [n][9][3] (host). */
+int ogp_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
+    double* pos, double* vel, double* s, double* S, double* acc);
+
+/* Known-answer hooks that run the device's own predictor and corrector arithmetic on given
+ * inputs (integrator_prediction_test.cu, integrator_test.cu:207-245). n objects, host arrays
+ * [n][27] / [n][3]. */
+int ogp_kat_predict(ogp_context* ctx, const double* acc27, const double* s3, double* S3,
+    int64_t n, double h, double* r3, double* v3);
+int ogp_kat_finalize_correction(ogp_context* ctx, double* r3, double* v3, const double* acc27,
+    const double* s3, const double* S3, int64_t n, double h);
+
+/* ---- conjunction (ClosestApproach, closest_approach.cuh:36-88) --------------------------- */
+
+/* ClosestApproach::Initialize: set ids must start at 0, be non-decreasing with steps of +1
+ * (partition_parameters.cuh:94-135); at least two sets. */
+int ogp_conj_init(ogp_context* ctx, const double* r, const double* v, const uint32_t* set_ids,
+    int64_t n, double step_s);
+/* ClosestApproach::Propagate: continues from the current state; may be called repeatedly. */
+int ogp_conj_propagate(ogp_context* ctx, int64_t n_steps);
+/* ClosestApproach::FindClosest. threshold_m <= 0 refines every cross-set pair (reference
+ * behaviour); threshold_m > 0 screens and refines only pairs whose sampled minimum distance is
+ * below the threshold. Returns the number of result records in *count. */
+int ogp_conj_find(ogp_context* ctx, double threshold_m, int64_t* count);
+/* Copies the records of the last ogp_conj_find, sorted by pair index: object indices (a < b),
+ * sampled minimum step, refined miss distance, TCA (s from the epoch) and positions at TCA. */
+int ogp_conj_results(ogp_context* ctx, uint32_t* obj_a, uint32_t* obj_b, int32_t* step,
+    double* distance, double* time, double* pos_a, double* pos_b);
+/* Sampled screening minimum of every cross-set pair (pair index order). */
+int ogp_conj_screen(ogp_context* ctx, double* min_dist, int32_t* min_step);
+int ogp_conj_num_pairs(const ogp_context* ctx, int64_t* num_pairs);
+/* Copies the stored trajectory [steps][n][9] = r, v, a per step (host). */
+int ogp_conj_trajectory(ogp_context* ctx, double* out, int64_t* steps);
+typedef struct ogp_conj_timing {
+    double propagate_ms, screen_ms, refine_ms;
+    double pair_steps, pairs, candidates;
+} ogp_conj_timing;
+int ogp_conj_last_timing(const ogp_context* ctx, ogp_conj_timing* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1193 @@
+// C ABI of the engine (include/ogp.h): context, datasets on the device, propagation and
+// conjunction orchestration. Host code; all device work is issued on the context's stream.
+#include "../../include/ogp.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "conjunction.hpp"
+#include "datasets.hpp"
+#include "kernels.hpp"
+
+using namespace ogp;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string& m)
+        : std::runtime_error(m)
+        , code(c)
+    {
+    }
+};
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw ApiError(OGP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+void arg_check(bool ok, const std::string& msg)
+{
+    if (!ok)
+        throw ApiError(OGP_ERR_ARGUMENT, msg);
+}
+
+// Device allocation owned by the context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int64_t* counter = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) {
+            (void)hipFree(p);
+            if (counter)
+                *counter -= (int64_t)bytes;
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+    // grows (never shrinks) to at least `need` bytes; contents are not preserved
+    void reserve(size_t need, int64_t* ctr)
+    {
+        if (need <= bytes && p)
+            return;
+        release();
+        counter = ctr;
+        size_t b = std::max<size_t>(need, 256);
+        HIPC(hipMalloc(&p, b));
+        bytes = b;
+        if (counter)
+            *counter += (int64_t)b;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Time-node tables of one step-size class (FMTimePoints, time_points.cu:126-207).
+struct NodeTable {
+    DevBuf gen_M, prop_M, prop_mjd, gen_mjd, offsets;
+    int64_t n_prop = 0;
+};
+
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { HIPC(hipEventCreate(&e)); }
+    ~Event()
+    {
+        if (e)
+            (void)hipEventDestroy(e);
+    }
+};
+
+} // namespace
+
+struct ogp_context {
+    ogp_config cfg {};
+    std::string data_dir;
+    std::string error;
+    hipStream_t stream = nullptr;
+    int64_t dev_bytes = 0;
+
+    GravityModel grav;
+    FrameData frames;
+    DevBuf d_rec, d_recf, d_diag, d_eop, d_leap_mjd, d_leap_sec, d_nut[4];
+    GravityDev gdev {};
+    FrameDev fdev {};
+    int kind = kGravity;  // kernel variant of the force
+    int rk8_variant = 0;  // 0 = reference row 7
+    int block = 64;       // objects per block of the propagation kernels
+    int64_t chunk = 64;   // GJ8 steps per launch
+
+    // node tables keyed by (h, t0)
+    std::map<std::pair<double, double>, std::unique_ptr<NodeTable>> nodes;
+
+    // propagation state (internal, class-sorted order)
+    DevBuf st_r, st_v, st_s, st_S, st_hist, st_ns, perm, io_a, io_b, io_c;
+    DevBuf sc[7];
+    ogp_timing timing {};
+
+    // stepwise propagator
+    int64_t sw_n = 0, sw_step = 0, sw_max = 0;
+    double sw_h = 0;
+    DevBuf sw_traj;
+
+    // conjunction
+    int64_t cj_n = 0, cj_steps = 0, cj_cap_steps = 0;
+    double cj_h = 0;
+    std::vector<uint32_t> cj_sets;
+    std::vector<int64_t> cj_set_off;
+    DevBuf cj_traj, cj_r, cj_v, cj_cnt, cj_a, cj_b, cj_k, cj_d, cj_out, cj_mind, cj_mink;
+    std::vector<uint32_t> res_a, res_b;
+    std::vector<int32_t> res_k;
+    std::vector<double> res_out;
+    ogp_conj_timing cj_timing {};
+
+    ~ogp_context()
+    {
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    void sync() { HIPC(hipStreamSynchronize(stream)); }
+    void check_launch() { HIPC(hipGetLastError()); }
+
+    ForceCtx force_ctx(const NodeTable* nt, double h) const
+    {
+        ForceCtx f;
+        f.grav = gdev;
+        f.gen_M = nt ? nt->gen_M.as<double>() : nullptr;
+        f.prop_M = nt ? nt->prop_M.as<double>() : nullptr;
+        f.h = h;
+        return f;
+    }
+
+    // Node offsets of the 80 RK8 stage nodes (time_points.cu:35-46 with beforeEpoch = 4);
+    // without OGP_QUIRK_BACKWARD_TIMES the backward stages use their true times.
+    std::vector<double> gen_offsets(double h) const
+    {
+        static const double c[10] = { 0.0, 4.0 / 27.0, 2.0 / 9.0, 1.0 / 3.0, 1.0 / 2.0,
+            2.0 / 3.0, 1.0 / 6.0, 1.0, 5.0 / 6.0, 1.0 };
+        std::vector<double> t(80);
+        for (int idx = 0; idx < 80; ++idx) {
+            const int blk = idx / 10, k = idx % 10, s = blk - 4;
+            if ((cfg.quirks & OGP_QUIRK_BACKWARD_TIMES) || s >= 0)
+                t[idx] = ((double)blk + c[k]) * h;
+            else
+                t[idx] = ((double)(s + 1) - c[k]) * h + 4 * h;
+        }
+        return t;
+    }
+
+    // Ensures the node table for step h (time origin t0 seconds after the epoch) covers
+    // propagation nodes 0 .. 4 + max_steps.
+    NodeTable* node_table(double h, int64_t max_steps, double t0 = 0.0)
+    {
+        if (cfg.force != OGP_FORCE_GRAVITY)
+            return nullptr;
+        auto key = std::make_pair(h, t0);
+        auto& slot = nodes[key];
+        const int64_t need = 4 + std::max<int64_t>(max_steps, 4) + 1;
+        if (slot && slot->n_prop >= need)
+            return slot.get();
+        if (!slot)
+            slot = std::make_unique<NodeTable>();
+        NodeTable& nt = *slot;
+        const int ymdhm[5] = { cfg.year, cfg.month, cfg.day, cfg.hour, cfg.minute };
+        const double shifted = (cfg.second - (double)4 * h) + t0;
+        std::vector<double> off = gen_offsets(h);
+        const int64_t len = std::max<int64_t>(need, 80);
+        nt.offsets.reserve(sizeof(double) * len, &dev_bytes);
+        nt.gen_M.reserve(sizeof(double) * 9 * 80, &dev_bytes);
+        nt.gen_mjd.reserve(sizeof(double) * 80, &dev_bytes);
+        HIPC(hipMemcpyAsync(nt.offsets.p, off.data(), sizeof(double) * 80, hipMemcpyHostToDevice,
+            stream));
+        launch_time_nodes(stream, fdev, ymdhm, shifted, nt.offsets.as<double>(), 80,
+            nt.gen_M.as<double>(), nt.gen_mjd.as<double>());
+        check_launch();
+        sync(); // offsets buffer is reused below
+        off.resize(need);
+        for (int64_t i = 0; i < need; ++i)
+            off[i] = (double)i * h;
+        nt.prop_M.reserve(sizeof(double) * 9 * need, &dev_bytes);
+        nt.prop_mjd.reserve(sizeof(double) * need, &dev_bytes);
+        HIPC(hipMemcpyAsync(nt.offsets.p, off.data(), sizeof(double) * need, hipMemcpyHostToDevice,
+            stream));
+        launch_time_nodes(stream, fdev, ymdhm, shifted, nt.offsets.as<double>(), need,
+            nt.prop_M.as<double>(), nt.prop_mjd.as<double>());
+        check_launch();
+        sync();
+        nt.n_prop = need;
+        return &nt;
+    }
+
+    PropState prop_state(int64_t N)
+    {
+        PropState ps;
+        ps.r = st_r.as<double>();
+        ps.v = st_v.as<double>();
+        ps.s = st_s.as<double>();
+        ps.S = st_S.as<double>();
+        ps.hist = st_hist.as<double>();
+        ps.nsteps = st_ns.as<int64_t>();
+        ps.N = N;
+        return ps;
+    }
+
+    void reserve_state(int64_t N)
+    {
+        const size_t v3 = sizeof(double) * 3 * N;
+        st_r.reserve(v3, &dev_bytes);
+        st_v.reserve(v3, &dev_bytes);
+        st_s.reserve(v3, &dev_bytes);
+        st_S.reserve(v3, &dev_bytes);
+        st_hist.reserve(9 * v3, &dev_bytes);
+        st_ns.reserve(sizeof(int64_t) * N, &dev_bytes);
+        perm.reserve(sizeof(int64_t) * N, &dev_bytes);
+        const int slots[7] = { 9, 9, 9, 9, 9, 10, 10 };
+        for (int q = 0; q < 7; ++q)
+            sc[q].reserve(slots[q] * v3, &dev_bytes);
+    }
+
+    StartupScratch scratch()
+    {
+        StartupScratch s;
+        s.node_r = sc[0].as<double>();
+        s.node_v = sc[1].as<double>();
+        s.acc = sc[2].as<double>();
+        s.sm = sc[3].as<double>();
+        s.Sm = sc[4].as<double>();
+        s.Kx = sc[5].as<double>();
+        s.Kv = sc[6].as<double>();
+        return s;
+    }
+
+    struct Group {
+        double h;
+        int64_t first, count, max_steps;
+    };
+
+    // Startup and GJ8 steps for all groups of the internal state (class-sorted order).
+    // traj (optional) receives steps 1..traj_steps of the single group.
+    void run_groups(const std::vector<Group>& groups, int64_t N, double t0, double* traj,
+        int64_t traj_steps)
+    {
+        Event e0, e1, e2, e3;
+        const PropState ps = prop_state(N);
+        // nodes
+        std::vector<NodeTable*> tabs;
+        HIPC(hipEventRecord(e0.e, stream));
+        for (const Group& g : groups)
+            tabs.push_back(node_table(g.h, g.max_steps, t0));
+        HIPC(hipEventRecord(e1.e, stream));
+        for (size_t q = 0; q < groups.size(); ++q) {
+            const Group& g = groups[q];
+            launch_startup(stream, kind, force_ctx(tabs[q], g.h), rk8_variant, ps, scratch(),
+                g.first, g.count, cfg.n_corrections, traj, traj_steps, block);
+            check_launch();
+        }
+        HIPC(hipEventRecord(e2.e, stream));
+        double launches = 0, gj_steps = 0;
+        for (size_t q = 0; q < groups.size(); ++q) {
+            const Group& g = groups[q];
+            for (int64_t k0 = 5; k0 <= g.max_steps; k0 += chunk) {
+                const int64_t k1 = std::min<int64_t>(k0 + chunk, g.max_steps + 1);
+                launch_chunk(stream, kind, force_ctx(tabs[q], g.h), ps, g.first, g.count, k0, k1,
+                    cfg.n_corrections, traj, 1, traj_steps, block);
+                check_launch();
+                launches += 1;
+            }
+        }
+        HIPC(hipEventRecord(e3.e, stream));
+        sync();
+        float ms0 = 0, ms1 = 0, ms2 = 0;
+        HIPC(hipEventElapsedTime(&ms0, e0.e, e1.e));
+        HIPC(hipEventElapsedTime(&ms1, e1.e, e2.e));
+        HIPC(hipEventElapsedTime(&ms2, e2.e, e3.e));
+        timing.nodes_ms = ms0;
+        timing.startup_ms = ms1;
+        timing.steps_ms = ms2;
+        timing.total_ms = (double)ms0 + ms1 + ms2;
+        timing.step_launches = launches;
+        (void)gj_steps;
+    }
+};
+
+namespace {
+
+int fail(ogp_context* ctx, int code, const std::string& msg)
+{
+    g_error = msg;
+    if (ctx)
+        ctx->error = msg;
+    return code;
+}
+
+template <class F> int guarded(ogp_context* ctx, F&& fn)
+{
+    try {
+        fn();
+        return OGP_OK;
+    } catch (const ApiError& e) {
+        return fail(ctx, e.code, e.what());
+    } catch (const std::invalid_argument& e) {
+        return fail(ctx, OGP_ERR_ARGUMENT, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, OGP_ERR_DEVICE, "host out of memory");
+    } catch (const std::exception& e) {
+        return fail(ctx, OGP_ERR_DATA, e.what());
+    }
+}
+
+template <typename T> void upload(ogp_context* c, DevBuf& b, const std::vector<T>& v)
+{
+    b.reserve(sizeof(T) * std::max<size_t>(v.size(), 1), &c->dev_bytes);
+    if (!v.empty())
+        HIPC(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
+}
+
+void check_finite(const double* p, int64_t n, const char* what)
+{
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i]))
+            throw ApiError(OGP_ERR_NONFINITE, std::string("non-finite value in ") + what
+                    + " at index " + std::to_string(i));
+}
+
+} // namespace
+
+// =========================================================================================
+extern "C" {
+
+void ogp_default_config(ogp_config* cfg)
+{
+    if (!cfg)
+        return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->data_dir = "data";
+    cfg->degree = -1;
+    cfg->precision = OGP_PRECISION_FP64;
+    cfg->split_degree = 2;
+    cfg->n_corrections = 7;
+    cfg->quirks = OGP_QUIRKS_REFERENCE;
+    cfg->force = OGP_FORCE_GRAVITY;
+    cfg->year = 2025;
+    cfg->month = 1;
+    cfg->day = 1;
+}
+
+int ogp_create(const ogp_config* cfg, ogp_context** out)
+{
+    if (!cfg || !out)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null config or output pointer");
+    *out = nullptr;
+    auto ctx = std::make_unique<ogp_context>();
+    int rc = guarded(ctx.get(), [&] {
+        ogp_context& c = *ctx;
+        c.cfg = *cfg;
+        c.data_dir = cfg->data_dir ? cfg->data_dir : "data";
+        c.cfg.data_dir = c.data_dir.c_str();
+        arg_check(cfg->n_corrections >= 1, "n_corrections must be at least 1");
+        arg_check(cfg->force >= OGP_FORCE_GRAVITY && cfg->force <= OGP_FORCE_DAMPED,
+            "unknown force kind");
+        arg_check(cfg->precision == OGP_PRECISION_FP64 || cfg->precision == OGP_PRECISION_SPLIT,
+            "unknown precision");
+        arg_check(cfg->month >= 1 && cfg->month <= 12 && cfg->day >= 1 && cfg->day <= 31,
+            "invalid epoch date");
+        int ndev = 0;
+        HIPC(hipGetDeviceCount(&ndev));
+        if (ndev < 1)
+            throw ApiError(OGP_ERR_DEVICE, "no HIP device");
+        HIPC(hipSetDevice(0));
+        HIPC(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        c.rk8_variant = (cfg->quirks & OGP_QUIRK_RK8_ROW7) ? 0 : 1;
+        if (cfg->force == OGP_FORCE_GRAVITY) {
+            c.kind = cfg->precision == OGP_PRECISION_SPLIT ? kGravitySplit : kGravity;
+            c.grav = load_gravity(c.data_dir + "/JGM3.gfc", cfg->degree);
+            c.frames = load_frames(c.data_dir);
+            upload(&c, c.d_rec, c.grav.records);
+            upload(&c, c.d_recf, c.grav.records_f);
+            upload(&c, c.d_diag, c.grav.diag_ratio);
+            c.gdev.rec = c.d_rec.as<double>();
+            c.gdev.recf = c.d_recf.as<float>();
+            c.gdev.diag = c.d_diag.as<double>();
+            c.gdev.L = c.grav.L;
+            c.gdev.nsplit = cfg->precision == OGP_PRECISION_SPLIT ? std::max(cfg->split_degree, 1)
+                                                                  : c.grav.L;
+            c.gdev.gm = c.grav.gm;
+            c.gdev.radius = c.grav.radius;
+            std::vector<double> eop;
+            for (const EopRecord& r : c.frames.eop) {
+                eop.push_back(r.mjd);
+                eop.push_back(r.xp);
+                eop.push_back(r.yp);
+                eop.push_back(r.ut1_utc);
+            }
+            upload(&c, c.d_eop, eop);
+            upload(&c, c.d_leap_mjd, c.frames.leap_mjd);
+            upload(&c, c.d_leap_sec, c.frames.leap_sec);
+            const std::vector<NutationTerm>* tabs[4] = { &c.frames.lon0, &c.frames.lon1,
+                &c.frames.obl0, &c.frames.obl1 };
+            for (int q = 0; q < 4; ++q) {
+                std::vector<double> packed;
+                for (const NutationTerm& t : *tabs[q]) {
+                    packed.push_back(t.sin_uas);
+                    packed.push_back(t.cos_uas);
+                    for (int k = 0; k < 14; ++k)
+                        packed.push_back((double)t.mult[k]);
+                }
+                upload(&c, c.d_nut[q], packed);
+                c.fdev.nut[q] = c.d_nut[q].as<double>();
+                c.fdev.n_nut[q] = (int)tabs[q]->size();
+            }
+            c.fdev.eop = c.d_eop.as<double>();
+            c.fdev.n_eop = (int64_t)c.frames.eop.size();
+            c.fdev.leap_mjd = c.d_leap_mjd.as<double>();
+            c.fdev.leap_sec = c.d_leap_sec.as<double>();
+            c.fdev.n_leap = (int)c.frames.leap_mjd.size();
+        } else {
+            c.kind = cfg->force;
+        }
+        c.sync();
+    });
+    if (rc != OGP_OK)
+        return rc;
+    *out = ctx.release();
+    return OGP_OK;
+}
+
+void ogp_destroy(ogp_context* ctx) { delete ctx; }
+
+const char* ogp_last_error(const ogp_context* ctx)
+{
+    return ctx ? ctx->error.c_str() : g_error.c_str();
+}
+
+int64_t ogp_device_bytes(const ogp_context* ctx) { return ctx ? ctx->dev_bytes : 0; }
+
+int ogp_gravity_itrs(ogp_context* ctx, const double* r, int64_t n, double* a)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n >= 0 && (n == 0 || (r && a)), "invalid gravity arguments");
+        if (ctx->cfg.force != OGP_FORCE_GRAVITY)
+            throw ApiError(OGP_ERR_STATE, "context was created without the gravity field");
+        if (n == 0)
+            return;
+        check_finite(r, 3 * n, "positions");
+        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->io_b.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        HIPC(hipMemcpyAsync(ctx->io_a.p, r, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_gravity_aos(ctx->stream, ctx->kind == kGravitySplit, ctx->gdev, nullptr,
+            ctx->io_a.as<double>(), n, ctx->io_b.as<double>(), 3, 64);
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(a, ctx->io_b.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+        ctx->sync();
+    });
+}
+
+int ogp_accel(ogp_context* ctx, const double* r, const double* v, int64_t n, double step_s,
+    int64_t node, double* a, int64_t out_stride)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    (void)v;
+    return guarded(ctx, [&] {
+        arg_check(n >= 0 && step_s > 0 && node >= 0 && out_stride >= 3, "invalid accel arguments");
+        if (ctx->cfg.force != OGP_FORCE_GRAVITY)
+            throw ApiError(OGP_ERR_STATE, "context was created without the gravity field");
+        if (n == 0)
+            return;
+        check_finite(r, 3 * n, "positions");
+        NodeTable* nt = ctx->node_table(step_s, node);
+        const int64_t out_len = out_stride * (n - 1) + 3;
+        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->io_b.reserve(sizeof(double) * out_len, &ctx->dev_bytes);
+        HIPC(hipMemcpyAsync(ctx->io_a.p, r, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->io_b.p, a, sizeof(double) * out_len, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_gravity_aos(ctx->stream, ctx->kind == kGravitySplit, ctx->gdev,
+            nt->prop_M.as<double>() + 9 * node, ctx->io_a.as<double>(), n, ctx->io_b.as<double>(),
+            out_stride, 64);
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(a, ctx->io_b.p, sizeof(double) * out_len, hipMemcpyDeviceToHost,
+            ctx->stream));
+        ctx->sync();
+    });
+}
+
+int ogp_time_nodes(ogp_context* ctx, double step_s, int before, const double* offsets_s,
+    int64_t len, double* M, double* mjd)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(len >= 0 && (len == 0 || (offsets_s && M && mjd)), "invalid node arguments");
+        if (ctx->cfg.force != OGP_FORCE_GRAVITY)
+            throw ApiError(OGP_ERR_STATE, "context was created without the frame datasets");
+        if (len == 0)
+            return;
+        const ogp_config& c = ctx->cfg;
+        const int ymdhm[5] = { c.year, c.month, c.day, c.hour, c.minute };
+        const double shifted = c.second - (double)before * step_s;
+        ctx->io_a.reserve(sizeof(double) * len, &ctx->dev_bytes);
+        ctx->io_b.reserve(sizeof(double) * 9 * len, &ctx->dev_bytes);
+        ctx->io_c.reserve(sizeof(double) * len, &ctx->dev_bytes);
+        HIPC(hipMemcpyAsync(ctx->io_a.p, offsets_s, sizeof(double) * len, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_time_nodes(ctx->stream, ctx->fdev, ymdhm, shifted, ctx->io_a.as<double>(), len,
+            ctx->io_b.as<double>(), ctx->io_c.as<double>());
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(M, ctx->io_b.p, sizeof(double) * 9 * len, hipMemcpyDeviceToHost,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(mjd, ctx->io_c.p, sizeof(double) * len, hipMemcpyDeviceToHost,
+            ctx->stream));
+        ctx->sync();
+    });
+}
+
+} // extern "C"
+
+namespace {
+
+// Groups objects by step size (one time-node table and one launch sequence per class) and
+// sorts each class by descending step count so that the lanes of a wave end together.
+std::vector<ogp_context::Group> plan_groups(const double* step_s, const int64_t* steps, int64_t n,
+    std::vector<int64_t>& perm)
+{
+    perm.resize(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+        if (step_s[a] != step_s[b])
+            return step_s[a] < step_s[b];
+        return steps[a] > steps[b];
+    });
+    std::vector<ogp_context::Group> groups;
+    for (int64_t p = 0; p < n; ++p) {
+        const int64_t i = perm[p];
+        if (groups.empty() || groups.back().h != step_s[i])
+            groups.push_back({ step_s[i], p, 0, 0 });
+        groups.back().count++;
+        groups.back().max_steps = std::max(groups.back().max_steps, steps[i]);
+    }
+    return groups;
+}
+
+void validate_prop(const double* step_s, const int64_t* steps, int64_t n)
+{
+    arg_check(n >= 0, "negative object count");
+    for (int64_t i = 0; i < n; ++i) {
+        arg_check(std::isfinite(step_s[i]) && step_s[i] > 0,
+            "step size must be positive and finite (object " + std::to_string(i) + ")");
+        arg_check(steps[i] >= 0, "negative step count (object " + std::to_string(i) + ")");
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int ogp_propagate(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    const double* step_s, const int64_t* steps, double* r_out, double* v_out)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n == 0 || (r0 && v0 && step_s && steps && r_out && v_out), "null array");
+        validate_prop(step_s, steps, n);
+        if (n == 0)
+            return;
+        check_finite(r0, 3 * n, "positions");
+        check_finite(v0, 3 * n, "velocities");
+        std::vector<int64_t> perm;
+        auto groups = plan_groups(step_s, steps, n, perm);
+        std::vector<int64_t> ns(n);
+        for (int64_t p = 0; p < n; ++p)
+            ns[p] = steps[perm[p]];
+        ctx->reserve_state(n);
+        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->io_b.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        Event h0, h1, d0, d1;
+        HIPC(hipEventRecord(h0.e, ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->perm.p, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->io_a.p, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->io_b.p, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_permute(ctx->stream, 0, ctx->io_a.as<double>(), ctx->perm.as<int64_t>(),
+            ctx->st_r.as<double>(), n);
+        launch_permute(ctx->stream, 0, ctx->io_b.as<double>(), ctx->perm.as<int64_t>(),
+            ctx->st_v.as<double>(), n);
+        ctx->check_launch();
+        HIPC(hipEventRecord(h1.e, ctx->stream));
+        ctx->run_groups(groups, n, 0.0, nullptr, 0);
+        HIPC(hipEventRecord(d0.e, ctx->stream));
+        launch_permute(ctx->stream, 1, ctx->st_r.as<double>(), ctx->perm.as<int64_t>(),
+            ctx->io_a.as<double>(), n);
+        launch_permute(ctx->stream, 1, ctx->st_v.as<double>(), ctx->perm.as<int64_t>(),
+            ctx->io_b.as<double>(), n);
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(r_out, ctx->io_a.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(v_out, ctx->io_b.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+        HIPC(hipEventRecord(d1.e, ctx->stream));
+        ctx->sync();
+        float a = 0, b = 0;
+        HIPC(hipEventElapsedTime(&a, h0.e, h1.e));
+        HIPC(hipEventElapsedTime(&b, d0.e, d1.e));
+        ctx->timing.h2d_ms = a;
+        ctx->timing.d2h_ms = b;
+        double os = 0, gj = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            os += (double)steps[i];
+            gj += (double)std::max<int64_t>(steps[i] - 4, 0);
+        }
+        ctx->timing.object_steps = os;
+        ctx->timing.gj_object_steps = gj;
+    });
+}
+
+int ogp_propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v0, int64_t n,
+    const double* step_s, const int64_t* steps, double* d_r, double* d_v)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n == 0 || (d_r0 && d_v0 && step_s && steps && d_r && d_v), "null array");
+        validate_prop(step_s, steps, n);
+        if (n == 0)
+            return;
+        std::vector<int64_t> perm;
+        auto groups = plan_groups(step_s, steps, n, perm);
+        std::vector<int64_t> ns(n);
+        for (int64_t p = 0; p < n; ++p)
+            ns[p] = steps[perm[p]];
+        ctx->reserve_state(n);
+        HIPC(hipMemcpyAsync(ctx->perm.p, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_permute(ctx->stream, 2, d_r0, ctx->perm.as<int64_t>(), ctx->st_r.as<double>(), n);
+        launch_permute(ctx->stream, 2, d_v0, ctx->perm.as<int64_t>(), ctx->st_v.as<double>(), n);
+        ctx->check_launch();
+        ctx->run_groups(groups, n, 0.0, nullptr, 0);
+        launch_permute(ctx->stream, 3, ctx->st_r.as<double>(), ctx->perm.as<int64_t>(), d_r, n);
+        launch_permute(ctx->stream, 3, ctx->st_v.as<double>(), ctx->perm.as<int64_t>(), d_v, n);
+        ctx->check_launch();
+        ctx->sync();
+        double os = 0, gj = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            os += (double)steps[i];
+            gj += (double)std::max<int64_t>(steps[i] - 4, 0);
+        }
+        ctx->timing.object_steps = os;
+        ctx->timing.gj_object_steps = gj;
+        ctx->timing.h2d_ms = ctx->timing.d2h_ms = 0;
+    });
+}
+
+int ogp_last_timing(const ogp_context* ctx, ogp_timing* t)
+{
+    if (!ctx || !t)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null argument");
+    *t = ctx->timing;
+    return OGP_OK;
+}
+
+// ---- stepwise propagator --------------------------------------------------------------
+int ogp_prop_init(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
+    int64_t max_steps)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n > 0 && r0 && v0, "GpuPropagator needs at least one object");
+        arg_check(std::isfinite(step_s) && step_s > 0, "step size must be positive");
+        arg_check(max_steps >= 1, "max_steps must be at least 1");
+        check_finite(r0, 3 * n, "positions");
+        check_finite(v0, 3 * n, "velocities");
+        ctx->reserve_state(n);
+        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        HIPC(hipMemcpyAsync(ctx->io_a.p, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->st_r.as<double>(), n);
+        ctx->sync();
+        HIPC(hipMemcpyAsync(ctx->io_a.p, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->st_v.as<double>(), n);
+        std::vector<int64_t> ns(n, max_steps);
+        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        ctx->sw_traj.reserve(sizeof(double) * 9 * 4 * n, &ctx->dev_bytes);
+        ctx->check_launch();
+        ctx->sync();
+        ctx->sw_n = n;
+        ctx->sw_step = 0;
+        ctx->sw_max = max_steps;
+        ctx->sw_h = step_s;
+    });
+}
+
+int ogp_prop_step(ogp_context* ctx, double* r_out, double* v_out, double* a_out)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        if (ctx->sw_n <= 0)
+            throw ApiError(OGP_ERR_STATE, "ogp_prop_step before ogp_prop_init");
+        const int64_t n = ctx->sw_n;
+        const int64_t k = ctx->sw_step + 1;
+        if (k > ctx->sw_max)
+            throw ApiError(OGP_ERR_STATE, "step beyond max_steps of ogp_prop_init");
+        NodeTable* nt = ctx->node_table(ctx->sw_h, ctx->sw_max);
+        const ForceCtx f = ctx->force_ctx(nt, ctx->sw_h);
+        const PropState ps = ctx->prop_state(n);
+        double* traj = ctx->sw_traj.as<double>();
+        if (k == 1) {
+            launch_startup(ctx->stream, ctx->kind, f, ctx->rk8_variant, ps, ctx->scratch(), 0, n,
+                ctx->cfg.n_corrections, traj, 4, ctx->block);
+            ctx->check_launch();
+        }
+        const double* row;
+        if (k <= 4) {
+            row = traj + (k - 1) * 9 * n;
+        } else {
+            launch_chunk(ctx->stream, ctx->kind, f, ps, 0, n, k, k + 1, ctx->cfg.n_corrections,
+                traj, k, 1, ctx->block);
+            ctx->check_launch();
+            row = traj;
+        }
+        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        double* outs[3] = { r_out, v_out, a_out };
+        for (int q = 0; q < 3; ++q) {
+            if (!outs[q])
+                continue;
+            launch_soa_to_aos(ctx->stream, row + 3 * q * n, ctx->io_a.as<double>(), n);
+            ctx->check_launch();
+            HIPC(hipMemcpyAsync(outs[q], ctx->io_a.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                ctx->stream));
+            ctx->sync();
+        }
+        ctx->sync();
+        ctx->sw_step = k;
+    });
+}
+
+int ogp_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
+    double* pos, double* vel, double* s, double* S, double* acc)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n > 0 && r0 && v0 && step_s > 0, "invalid startup arguments");
+        check_finite(r0, 3 * n, "positions");
+        check_finite(v0, 3 * n, "velocities");
+        ctx->reserve_state(n);
+        ctx->io_a.reserve(sizeof(double) * 27 * n, &ctx->dev_bytes);
+        HIPC(hipMemcpyAsync(ctx->io_a.p, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->st_r.as<double>(), n);
+        ctx->sync();
+        HIPC(hipMemcpyAsync(ctx->io_a.p, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->st_v.as<double>(), n);
+        std::vector<int64_t> ns(n, 4);
+        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        NodeTable* nt = ctx->node_table(step_s, 4);
+        launch_startup(ctx->stream, ctx->kind, ctx->force_ctx(nt, step_s), ctx->rk8_variant,
+            ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0,
+            ctx->block);
+        ctx->check_launch();
+        ctx->sync();
+        // scratch [slot][c][obj] -> caller [obj][slot][c]
+        double* outs[5] = { pos, vel, acc, s, S };
+        std::vector<double> host(27 * n);
+        for (int q = 0; q < 5; ++q) {
+            if (!outs[q])
+                continue;
+            HIPC(hipMemcpy(host.data(), ctx->sc[q].p, sizeof(double) * 27 * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i)
+                for (int sl = 0; sl < 9; ++sl)
+                    for (int c = 0; c < 3; ++c)
+                        outs[q][27 * i + 3 * sl + c] = host[(sl * 3 + c) * n + i];
+        }
+    });
+}
+
+int ogp_kat_predict(ogp_context* ctx, const double* acc27, const double* s3, double* S3,
+    int64_t n, double h, double* r3, double* v3)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n > 0, "n must be positive");
+        DevBuf a, s, S, r, v;
+        a.reserve(sizeof(double) * 27 * n, nullptr);
+        s.reserve(sizeof(double) * 3 * n, nullptr);
+        S.reserve(sizeof(double) * 3 * n, nullptr);
+        r.reserve(sizeof(double) * 3 * n, nullptr);
+        v.reserve(sizeof(double) * 3 * n, nullptr);
+        HIPC(hipMemcpy(a.p, acc27, sizeof(double) * 27 * n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s.p, s3, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(S.p, S3, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        launch_kat_predict(ctx->stream, a.as<double>(), s.as<double>(), S.as<double>(), n, h,
+            r.as<double>(), v.as<double>());
+        ctx->check_launch();
+        ctx->sync();
+        HIPC(hipMemcpy(S3, S.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(r3, r.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(v3, v.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int ogp_kat_finalize_correction(ogp_context* ctx, double* r3, double* v3, const double* acc27,
+    const double* s3, const double* S3, int64_t n, double h)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n > 0, "n must be positive");
+        DevBuf a, s, S, r, v;
+        a.reserve(sizeof(double) * 27 * n, nullptr);
+        s.reserve(sizeof(double) * 3 * n, nullptr);
+        S.reserve(sizeof(double) * 3 * n, nullptr);
+        r.reserve(sizeof(double) * 3 * n, nullptr);
+        v.reserve(sizeof(double) * 3 * n, nullptr);
+        HIPC(hipMemcpy(a.p, acc27, sizeof(double) * 27 * n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s.p, s3, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(S.p, S3, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(r.p, r3, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(v.p, v3, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        launch_kat_finalize(ctx->stream, r.as<double>(), v.as<double>(), a.as<double>(),
+            s.as<double>(), S.as<double>(), n, h);
+        ctx->check_launch();
+        ctx->sync();
+        HIPC(hipMemcpy(r3, r.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(v3, v.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    });
+}
+
+// ---- conjunction ------------------------------------------------------------------------
+int ogp_conj_init(ogp_context* ctx, const double* r, const double* v, const uint32_t* set_ids,
+    int64_t n, double step_s)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        // closest_approach.cu:50-81 and partition_parameters.cuh:94-135
+        if (ctx->cj_n != 0)
+            throw ApiError(OGP_ERR_STATE, "ClosestApproach already initialised");
+        arg_check(n > 0 && r && v && set_ids, "positions array is empty");
+        arg_check(std::isfinite(step_s) && step_s > 0, "step size must be positive");
+        arg_check(set_ids[0] == 0, "set ids must start at 0");
+        arg_check(set_ids[0] != set_ids[n - 1], "invalid setIndices: at least 2 sets are required");
+        std::vector<int64_t> off { 0 };
+        for (int64_t i = 1; i < n; ++i) {
+            arg_check(set_ids[i] == set_ids[i - 1] || set_ids[i] == set_ids[i - 1] + 1,
+                "set ids must be non-decreasing in steps of 1");
+            if (set_ids[i] != set_ids[i - 1])
+                off.push_back(i);
+        }
+        check_finite(r, 3 * n, "positions");
+        check_finite(v, 3 * n, "velocities");
+        ctx->cj_sets.assign(set_ids, set_ids + n);
+        ctx->cj_set_off = off;
+        ctx->cj_r.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->cj_v.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        HIPC(hipMemcpyAsync(ctx->io_a.p, r, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->cj_r.as<double>(), n);
+        ctx->sync();
+        HIPC(hipMemcpyAsync(ctx->io_a.p, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->cj_v.as<double>(), n);
+        ctx->check_launch();
+        ctx->sync();
+        ctx->cj_n = n;
+        ctx->cj_h = step_s;
+        ctx->cj_steps = 0;
+    });
+}
+
+int ogp_conj_propagate(ogp_context* ctx, int64_t n_steps)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        if (ctx->cj_n <= 0)
+            throw ApiError(OGP_ERR_STATE, "ClosestApproach::Propagate before Initialize");
+        arg_check(n_steps > 0, "numSteps must be positive");
+        const int64_t n = ctx->cj_n;
+        const int64_t total = ctx->cj_steps + n_steps;
+        // grow the trajectory store, keeping the samples already taken
+        if (total > ctx->cj_cap_steps) {
+            const int64_t cap = std::max<int64_t>(total, ctx->cj_cap_steps * 2);
+            DevBuf nb;
+            nb.reserve(sizeof(double) * 9 * n * cap, &ctx->dev_bytes);
+            if (ctx->cj_steps > 0)
+                HIPC(hipMemcpyAsync(nb.p, ctx->cj_traj.p, sizeof(double) * 9 * n * ctx->cj_steps,
+                    hipMemcpyDeviceToDevice, ctx->stream));
+            ctx->sync();
+            std::swap(ctx->cj_traj.p, nb.p);
+            std::swap(ctx->cj_traj.bytes, nb.bytes);
+            std::swap(ctx->cj_traj.counter, nb.counter);
+            ctx->cj_cap_steps = cap;
+        }
+        // every phase restarts the startup from the current state (a new GpuPropagator per
+        // Propagate call, closest_approach.cu:134-135); its time origin continues from the
+        // previous phase
+        ctx->reserve_state(n);
+        HIPC(hipMemcpyAsync(ctx->st_r.p, ctx->cj_r.p, sizeof(double) * 3 * n,
+            hipMemcpyDeviceToDevice, ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->st_v.p, ctx->cj_v.p, sizeof(double) * 3 * n,
+            hipMemcpyDeviceToDevice, ctx->stream));
+        std::vector<int64_t> ns(n, n_steps);
+        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        ctx->sync();
+        const double t0 = (double)ctx->cj_steps * ctx->cj_h;
+        std::vector<ogp_context::Group> g { { ctx->cj_h, 0, n, n_steps } };
+        double* traj = ctx->cj_traj.as<double>() + 9 * n * ctx->cj_steps;
+        ctx->run_groups(g, n, t0, traj, n_steps);
+        HIPC(hipMemcpyAsync(ctx->cj_r.p, ctx->st_r.p, sizeof(double) * 3 * n,
+            hipMemcpyDeviceToDevice, ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->cj_v.p, ctx->st_v.p, sizeof(double) * 3 * n,
+            hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->sync();
+        ctx->cj_steps = total;
+        ctx->cj_timing.propagate_ms = ctx->timing.total_ms;
+    });
+}
+
+int ogp_conj_num_pairs(const ogp_context* ctx, int64_t* num_pairs)
+{
+    if (!ctx || !num_pairs)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null argument");
+    int64_t np = 0;
+    const auto& off = ctx->cj_set_off;
+    for (size_t s = 0; s + 1 < off.size(); ++s)
+        np += (off[s + 1] - off[s]) * (ctx->cj_n - off[s + 1]);
+    *num_pairs = np;
+    return OGP_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// Screens every set rectangle; all_pairs writes the per-pair minima, threshold collects
+// candidates. Returns the number of candidates.
+int64_t conj_screen(ogp_context* ctx, double threshold, bool all_pairs)
+{
+    const int64_t n = ctx->cj_n, nsteps = ctx->cj_steps;
+    int64_t np = 0;
+    ogp_conj_num_pairs(ctx, &np);
+    const auto& off = ctx->cj_set_off;
+    if (all_pairs) {
+        ctx->cj_mind.reserve(sizeof(double) * std::max<int64_t>(np, 1), &ctx->dev_bytes);
+        ctx->cj_mink.reserve(sizeof(int32_t) * std::max<int64_t>(np, 1), &ctx->dev_bytes);
+    }
+    int64_t cap = std::max<int64_t>(1 << 16, ctx->cj_a.bytes / sizeof(uint32_t));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        ctx->cj_cnt.reserve(sizeof(uint32_t), &ctx->dev_bytes);
+        ctx->cj_a.reserve(sizeof(uint32_t) * cap, &ctx->dev_bytes);
+        ctx->cj_b.reserve(sizeof(uint32_t) * cap, &ctx->dev_bytes);
+        ctx->cj_k.reserve(sizeof(int32_t) * cap, &ctx->dev_bytes);
+        ctx->cj_d.reserve(sizeof(double) * cap, &ctx->dev_bytes);
+        const uint32_t zero = 0;
+        HIPC(hipMemcpyAsync(ctx->cj_cnt.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+        int64_t pair0 = 0;
+        for (size_t s = 0; s + 1 < off.size(); ++s) {
+            const int64_t a0 = off[s], na = off[s + 1] - off[s];
+            const int64_t b0 = off[s + 1], nb = n - off[s + 1];
+            launch_screen(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps, a0, na, b0, nb, pair0,
+                all_pairs ? 0.0 : threshold, all_pairs ? ctx->cj_mind.as<double>() : nullptr,
+                all_pairs ? ctx->cj_mink.as<int32_t>() : nullptr, ctx->cj_cnt.as<uint32_t>(), cap,
+                ctx->cj_a.as<uint32_t>(), ctx->cj_b.as<uint32_t>(), ctx->cj_k.as<int32_t>(),
+                ctx->cj_d.as<double>());
+            ctx->check_launch();
+            pair0 += na * nb;
+        }
+        uint32_t cnt = 0;
+        HIPC(hipMemcpyAsync(&cnt, ctx->cj_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        if (all_pairs)
+            return np;
+        if ((int64_t)cnt <= cap)
+            return cnt;
+        cap = (int64_t)cnt; // too many candidates for the buffer: grow once and rescreen
+    }
+    throw ApiError(OGP_ERR_DEVICE, "candidate buffer overflow");
+}
+
+} // namespace
+
+extern "C" {
+
+int ogp_conj_screen(ogp_context* ctx, double* min_dist, int32_t* min_step)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        if (ctx->cj_steps <= 0)
+            throw ApiError(OGP_ERR_STATE, "screen before Propagate");
+        const int64_t np = conj_screen(ctx, 0.0, true);
+        HIPC(hipMemcpy(min_dist, ctx->cj_mind.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(min_step, ctx->cj_mink.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+    });
+}
+
+int ogp_conj_find(ogp_context* ctx, double threshold_m, int64_t* count)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        if (ctx->cj_steps <= 0)
+            throw ApiError(OGP_ERR_STATE, "FindClosest before Propagate");
+        arg_check(count != nullptr, "null count");
+        const int64_t n = ctx->cj_n;
+        Event e0, e1, e2;
+        HIPC(hipEventRecord(e0.e, ctx->stream));
+        int64_t nc;
+        if (threshold_m > 0) {
+            nc = conj_screen(ctx, threshold_m, false);
+        } else {
+            // every cross-set pair becomes a candidate (reference: all pairs refined)
+            const int64_t np = conj_screen(ctx, 0.0, true);
+            std::vector<double> d(np);
+            std::vector<int32_t> k(np);
+            HIPC(hipMemcpy(d.data(), ctx->cj_mind.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(k.data(), ctx->cj_mink.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost));
+            std::vector<uint32_t> a(np), b(np);
+            int64_t p = 0;
+            const auto& off = ctx->cj_set_off;
+            for (size_t s = 0; s + 1 < off.size(); ++s)
+                for (int64_t i = off[s]; i < off[s + 1]; ++i)
+                    for (int64_t j = off[s + 1]; j < n; ++j, ++p) {
+                        a[p] = (uint32_t)i;
+                        b[p] = (uint32_t)j;
+                    }
+            ctx->cj_a.reserve(sizeof(uint32_t) * np, &ctx->dev_bytes);
+            ctx->cj_b.reserve(sizeof(uint32_t) * np, &ctx->dev_bytes);
+            ctx->cj_k.reserve(sizeof(int32_t) * np, &ctx->dev_bytes);
+            ctx->cj_d.reserve(sizeof(double) * np, &ctx->dev_bytes);
+            HIPC(hipMemcpy(ctx->cj_a.p, a.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(ctx->cj_b.p, b.data(), sizeof(uint32_t) * np, hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(ctx->cj_k.p, k.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(ctx->cj_d.p, d.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+            nc = np;
+        }
+        HIPC(hipEventRecord(e1.e, ctx->stream));
+        ctx->cj_out.reserve(sizeof(double) * 8 * std::max<int64_t>(nc, 1), &ctx->dev_bytes);
+        launch_refine(ctx->stream, ctx->cj_traj.as<double>(), n, ctx->cj_steps, ctx->cj_h, 0.0, nc,
+            ctx->cj_a.as<uint32_t>(), ctx->cj_b.as<uint32_t>(), ctx->cj_k.as<int32_t>(),
+            ctx->cj_d.as<double>(), ctx->cj_out.as<double>());
+        ctx->check_launch();
+        HIPC(hipEventRecord(e2.e, ctx->stream));
+        ctx->sync();
+        ctx->res_a.resize(nc);
+        ctx->res_b.resize(nc);
+        ctx->res_k.resize(nc);
+        ctx->res_out.resize(8 * nc);
+        if (nc > 0) {
+            HIPC(hipMemcpy(ctx->res_a.data(), ctx->cj_a.p, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(ctx->res_b.data(), ctx->cj_b.p, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(ctx->res_k.data(), ctx->cj_k.p, sizeof(int32_t) * nc, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(ctx->res_out.data(), ctx->cj_out.p, sizeof(double) * 8 * nc,
+                hipMemcpyDeviceToHost));
+        }
+        // order by pair (a, b): the candidate list of the threshold screen is filled by
+        // atomics and has no fixed order
+        std::vector<int64_t> ord(nc);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) {
+            return ctx->res_a[x] != ctx->res_a[y] ? ctx->res_a[x] < ctx->res_a[y]
+                                                  : ctx->res_b[x] < ctx->res_b[y];
+        });
+        auto permute = [&](auto& vec, int w) {
+            auto cp = vec;
+            for (int64_t q = 0; q < nc; ++q)
+                for (int j = 0; j < w; ++j)
+                    vec[q * w + j] = cp[ord[q] * w + j];
+        };
+        permute(ctx->res_a, 1);
+        permute(ctx->res_b, 1);
+        permute(ctx->res_k, 1);
+        permute(ctx->res_out, 8);
+        float s1 = 0, s2 = 0;
+        HIPC(hipEventElapsedTime(&s1, e0.e, e1.e));
+        HIPC(hipEventElapsedTime(&s2, e1.e, e2.e));
+        ctx->cj_timing.screen_ms = s1;
+        ctx->cj_timing.refine_ms = s2;
+        int64_t np = 0;
+        ogp_conj_num_pairs(ctx, &np);
+        ctx->cj_timing.pairs = (double)np;
+        ctx->cj_timing.pair_steps = (double)np * (double)ctx->cj_steps;
+        ctx->cj_timing.candidates = (double)nc;
+        *count = nc;
+    });
+}
+
+int ogp_conj_results(ogp_context* ctx, uint32_t* a, uint32_t* b, int32_t* step, double* distance,
+    double* time, double* pos_a, double* pos_b)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    const size_t nc = ctx->res_a.size();
+    for (size_t q = 0; q < nc; ++q) {
+        if (a)
+            a[q] = ctx->res_a[q];
+        if (b)
+            b[q] = ctx->res_b[q];
+        if (step)
+            step[q] = ctx->res_k[q];
+        const double* o = &ctx->res_out[8 * q];
+        if (distance)
+            distance[q] = o[0];
+        if (time)
+            time[q] = o[1];
+        for (int c = 0; c < 3; ++c) {
+            if (pos_a)
+                pos_a[3 * q + c] = o[2 + c];
+            if (pos_b)
+                pos_b[3 * q + c] = o[5 + c];
+        }
+    }
+    return OGP_OK;
+}
+
+int ogp_conj_trajectory(ogp_context* ctx, double* out, int64_t* steps)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        if (steps)
+            *steps = ctx->cj_steps;
+        if (!out || ctx->cj_steps == 0)
+            return;
+        const int64_t n = ctx->cj_n, ns = ctx->cj_steps;
+        std::vector<double> host(9 * n * ns);
+        HIPC(hipMemcpy(host.data(), ctx->cj_traj.p, sizeof(double) * host.size(), hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < ns; ++k)
+            for (int c = 0; c < 9; ++c)
+                for (int64_t i = 0; i < n; ++i)
+                    out[(k * n + i) * 9 + c] = host[(k * 9 + c) * n + i];
+    });
+}
+
+int ogp_conj_last_timing(const ogp_context* ctx, ogp_conj_timing* t)
+{
+    if (!ctx || !t)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null argument");
+    *t = ctx->cj_timing;
+    return OGP_OK;
+}
+
+} // extern "C"

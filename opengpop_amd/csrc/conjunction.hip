@@ -1,0 +1,289 @@
+// Conjunction screening and refinement for gfx950.
+//
+// Screening (replaces UpdateClosestApproachKernel, closest_approach/src/device.cu:56-91): the
+// reference updates a per-pair running minimum in HBM after every step (492 B of state per
+// pair). Here the trajectory of every object is stored once ([step][component][object]) and a
+// pair tile (64 x 64 objects) walks the whole time axis with its running minima in registers;
+// object positions are staged through LDS 8 steps at a time. Per pair the state is the best
+// squared distance and its step.
+//
+// Decision rule, identical to the reference's "dist < closest" on sqrt'ed distances with the
+// first occurrence kept: a sample is taken iff d2 < best_d2 and sqrt(d2) < sqrt(best_d2).
+// The first test is implied by the second (sqrt is monotone) and filters out almost all
+// samples, so the square root runs only when a pair improves. The squared distance is formed
+// as fma(dz, dz, fma(dy, dy, dx * dx)) exactly as in the host oracle, so identical
+// trajectories give identical bits.
+//
+// Refinement (CalculateClosestApproachesKernel, device.cu:93-173; interpolation.cuh:27-152):
+// one thread per candidate pair; 8th-degree Hermite interpolation through the samples at
+// steps k-1, k, k+1 and a secant root of (rA - rB).(vA - vB).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "conjunction.hpp"
+
+namespace ogp {
+
+namespace {
+constexpr int TA = 64, TB = 64, PA = 4, PB = 4, STAGE = 8, THREADS = 256;
+}
+
+__global__ void __launch_bounds__(THREADS) screen_kernel(const double* __restrict__ traj,
+    int64_t N, int64_t nsteps, int64_t a0, int64_t na, int64_t b0, int64_t nb, int64_t pair0,
+    int64_t row_len, double threshold, double* __restrict__ min_dist, int32_t* __restrict__ min_step,
+    uint32_t* __restrict__ cand_count, int64_t cand_cap, uint32_t* __restrict__ cand_a,
+    uint32_t* __restrict__ cand_b, int32_t* __restrict__ cand_k, double* __restrict__ cand_d)
+{
+    __shared__ double sA[STAGE][3][TA];
+    __shared__ double sB[STAGE][3][TB];
+    const int tid = threadIdx.x;
+    const int tx = tid % (TB / PB); // 16 columns of B sub-tiles
+    const int ty = tid / (TB / PB); // 16 rows of A sub-tiles
+    const int64_t ta0 = a0 + (int64_t)blockIdx.y * TA;
+    const int64_t tb0 = b0 + (int64_t)blockIdx.x * TB;
+    double best[PA][PB];
+    int32_t bk[PA][PB];
+#pragma unroll
+    for (int i = 0; i < PA; ++i)
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+            best[i][j] = __builtin_inf();
+            bk[i][j] = -1;
+        }
+    for (int64_t k0 = 0; k0 < nsteps; k0 += STAGE) {
+        // stage positions of STAGE steps for both tiles: STAGE * 3 * (TA + TB) doubles
+        for (int e = tid; e < STAGE * 3 * (TA + TB); e += THREADS) {
+            const int which = e / (STAGE * 3 * TA); // 0 = A, 1 = B (TA == TB)
+            const int rem = e % (STAGE * 3 * TA);
+            const int st = rem / (3 * TA);
+            const int c = (rem / TA) % 3;
+            const int o = rem % TA;
+            const int64_t k = k0 + st;
+            const int64_t obj = (which == 0 ? ta0 : tb0) + o;
+            const int64_t lim = which == 0 ? a0 + na : b0 + nb;
+            double val = 0.0;
+            if (k < nsteps && obj < lim)
+                val = traj[(k * 9 + c) * N + obj];
+            if (which == 0)
+                sA[st][c][o] = val;
+            else
+                sB[st][c][o] = val;
+        }
+        __syncthreads();
+        const int smax = (int)((nsteps - k0) < STAGE ? (nsteps - k0) : STAGE);
+        for (int st = 0; st < smax; ++st) {
+            double ax[PA], ay[PA], az[PA], bx[PB], by[PB], bz[PB];
+#pragma unroll
+            for (int i = 0; i < PA; ++i) {
+                ax[i] = sA[st][0][ty * PA + i];
+                ay[i] = sA[st][1][ty * PA + i];
+                az[i] = sA[st][2][ty * PA + i];
+            }
+#pragma unroll
+            for (int j = 0; j < PB; ++j) {
+                bx[j] = sB[st][0][tx * PB + j];
+                by[j] = sB[st][1][tx * PB + j];
+                bz[j] = sB[st][2][tx * PB + j];
+            }
+            const int32_t k = (int32_t)(k0 + st);
+#pragma unroll
+            for (int i = 0; i < PA; ++i)
+#pragma unroll
+                for (int j = 0; j < PB; ++j) {
+                    const double dx = ax[i] - bx[j];
+                    const double dy = ay[i] - by[j];
+                    const double dz = az[i] - bz[j];
+                    const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                    if (d2 < best[i][j]) {
+                        if (bk[i][j] < 0 || __builtin_sqrt(d2) < __builtin_sqrt(best[i][j])) {
+                            best[i][j] = d2;
+                            bk[i][j] = k;
+                        }
+                    }
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < PA; ++i)
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+            const int64_t ia = ta0 + ty * PA + i;
+            const int64_t jb = tb0 + tx * PB + j;
+            if (ia >= a0 + na || jb >= b0 + nb)
+                continue;
+            // pairs that never took a sample keep the reference's initial DBL_MAX
+            const double d = bk[i][j] < 0 ? 1.7976931348623157e308 : __builtin_sqrt(best[i][j]);
+            const int64_t p = pair0 + (ia - a0) * row_len + (jb - b0);
+            if (min_dist) {
+                min_dist[p] = d;
+                min_step[p] = bk[i][j];
+            }
+            if (threshold > 0.0 && d < threshold) {
+                const uint32_t slot = atomicAdd(cand_count, 1u);
+                if (slot < cand_cap) {
+                    cand_a[slot] = (uint32_t)ia;
+                    cand_b[slot] = (uint32_t)jb;
+                    cand_k[slot] = bk[i][j];
+                    cand_d[slot] = d;
+                }
+            }
+        }
+}
+
+void launch_screen(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, int64_t a0,
+    int64_t na, int64_t b0, int64_t nb, int64_t pair0, double threshold, double* min_dist,
+    int32_t* min_step, uint32_t* cand_count, int64_t cand_cap, uint32_t* cand_a,
+    uint32_t* cand_b, int32_t* cand_k, double* cand_d)
+{
+    if (na <= 0 || nb <= 0 || nsteps <= 0)
+        return;
+    const dim3 grid((unsigned)((nb + TB - 1) / TB), (unsigned)((na + TA - 1) / TA));
+    hipLaunchKernelGGL(screen_kernel, grid, dim3(THREADS), 0, st, traj, N, nsteps, a0, na, b0,
+        nb, pair0, nb, threshold, min_dist, min_step, cand_count, cand_cap, cand_a, cand_b,
+        cand_k, cand_d);
+}
+
+// ------------------------------------------------------------------------------------------
+// Refinement
+// ------------------------------------------------------------------------------------------
+namespace {
+
+// Newton-form Hermite interpolation through (p, v, a) at 0, dt, 2 dt
+// (interpolation.cuh:27-98).
+struct Hermite {
+    double c[9];
+    double dt;
+    __device__ Hermite(const double* q, double dt_)
+        : dt(dt_)
+    {
+        const double t[9] = { 0.0, 0.0, 0.0, dt, dt, dt, 2.0 * dt, 2.0 * dt, 2.0 * dt };
+        for (int i = 0; i < 9; ++i)
+            c[i] = q[3 * (i / 3)];
+        for (int j = 1; j < 9; ++j)
+            for (int i = 8; i >= j; --i) {
+                if (t[i] == t[i - j]) {
+                    if (j == 1)
+                        c[i] = q[3 * (i / 3) + 1];
+                    else if (j == 2)
+                        c[i] = q[3 * (i / 3) + 2] / 2.0;
+                } else {
+                    c[i] = (c[i] - c[i - 1]) / (t[i] - t[i - j]);
+                }
+            }
+    }
+    __device__ double val(double t) const
+    {
+        const double z[8] = { 0.0, 0.0, 0.0, dt, dt, dt, 2.0 * dt, 2.0 * dt };
+        double res = c[0], term = 1.0;
+        for (int i = 1; i < 9; ++i) {
+            term *= (t - z[i - 1]);
+            res += c[i] * term;
+        }
+        return res;
+    }
+    __device__ double der(double t) const
+    {
+        const double z[8] = { 0.0, 0.0, 0.0, dt, dt, dt, 2.0 * dt, 2.0 * dt };
+        double res = 0.0, Nn = 1.0, dN = 0.0;
+        for (int i = 0; i < 9; ++i) {
+            res += c[i] * dN;
+            if (i < 8) {
+                dN = dN * (t - z[i]) + Nn;
+                Nn *= (t - z[i]);
+            }
+        }
+        return res;
+    }
+};
+
+__device__ double approach(const Hermite* A, const Hermite* B, double t)
+{
+    double f = 0.0;
+    for (int c = 0; c < 3; ++c)
+        f += (A[c].val(t) - B[c].val(t)) * (A[c].der(t) - B[c].der(t));
+    return f;
+}
+
+} // namespace
+
+__global__ void refine_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps, double h,
+    double t0, int64_t n, const uint32_t* __restrict__ ia, const uint32_t* __restrict__ ib,
+    const int32_t* __restrict__ ik, const double* __restrict__ id, double* __restrict__ out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n)
+        return;
+    const int64_t a = ia[q], b = ib[q];
+    const int64_t k = ik[q];
+    double* o = out + 8 * q; // distance, time, posA[3], posB[3]
+    if (k <= 0 || k >= nsteps - 1) {
+        // The reference reports time 0 (first sample) or total * h (last sample) and leaves
+        // the positions unset; the sampled positions are returned here.
+        o[0] = id[q];
+        o[1] = t0 + (k <= 0 ? 0.0 : (double)nsteps * h);
+        for (int c = 0; c < 3; ++c) {
+            o[2 + c] = traj[((k < 0 ? 0 : k) * 9 + c) * N + a];
+            o[5 + c] = traj[((k < 0 ? 0 : k) * 9 + c) * N + b];
+        }
+        return;
+    }
+    double qa[3][9], qb[3][9];
+    for (int j = 0; j < 3; ++j)
+        for (int c = 0; c < 3; ++c) {
+            const int64_t base = (k - 1 + j) * 9;
+            qa[c][3 * j + 0] = traj[(base + c) * N + a];
+            qa[c][3 * j + 1] = traj[(base + 3 + c) * N + a];
+            qa[c][3 * j + 2] = traj[(base + 6 + c) * N + a];
+            qb[c][3 * j + 0] = traj[(base + c) * N + b];
+            qb[c][3 * j + 1] = traj[(base + 3 + c) * N + b];
+            qb[c][3 * j + 2] = traj[(base + 6 + c) * N + b];
+        }
+    const Hermite HA[3] = { Hermite(qa[0], h), Hermite(qa[1], h), Hermite(qa[2], h) };
+    const Hermite HB[3] = { Hermite(qb[0], h), Hermite(qb[1], h), Hermite(qb[2], h) };
+    // FindRoot (interpolation.cuh:125-152): secant from (0, h), tol 1e-7, <= 100 iterations
+    double x0 = 0.0, x1 = h;
+    double f0 = approach(HA, HB, x0), f1 = approach(HA, HB, x1);
+    double t = x1;
+    for (int it = 0; it < 100; ++it) {
+        if (fabs(f1 - f0) < 1e-14) {
+            t = x1;
+            break;
+        }
+        const double xn = x1 - f1 * (x1 - x0) / (f1 - f0);
+        const double fn = approach(HA, HB, xn);
+        if (fabs(xn - x1) < 1e-7 || fabs(fn) < 1e-7) {
+            t = xn;
+            break;
+        }
+        x0 = x1;
+        f0 = f1;
+        x1 = xn;
+        f1 = fn;
+        t = x1;
+    }
+    double pa[3], pb[3];
+    for (int c = 0; c < 3; ++c) {
+        pa[c] = HA[c].val(t);
+        pb[c] = HB[c].val(t);
+    }
+    const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
+    o[0] = sqrt(dx * dx + dy * dy + dz * dz);
+    o[1] = t0 + t + (double)k * h;
+    for (int c = 0; c < 3; ++c) {
+        o[2 + c] = pa[c];
+        o[5 + c] = pb[c];
+    }
+}
+
+void launch_refine(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, double h,
+    double t0, int64_t n, const uint32_t* a, const uint32_t* b, const int32_t* k, const double* d,
+    double* out)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(refine_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, traj, N,
+        nsteps, h, t0, n, a, b, k, d, out);
+}
+
+} // namespace ogp

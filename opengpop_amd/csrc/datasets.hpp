@@ -1,0 +1,53 @@
+// Host-side datasets of the engine: ICGEM gravity field, IERS EOP (finals2000A), IERS 2010
+// nutation tables 5.3a/b, NTP leap-second list, and the device-ready gravity records.
+//
+// File formats and parsing rules follow lib/force_model/src/utility/dataset_loaders.cu:30-299
+// (gfc lines "gfc L M C S", EOP fixed columns 8-15/19-27/38-46/59-68 with rows shorter than
+// 68 characters skipped, nutation rows "i A A'' + 14 integer multipliers" in a j = 0 and a
+// j = 1 block, leap-second rows "NTP-seconds TAI-UTC").
+#pragma once
+#include <string>
+#include <vector>
+
+namespace ogp {
+
+struct NutationTerm {
+    double sin_uas, cos_uas; // A_i (or B''_i) and A''_i (or B_i), micro-arcseconds
+    int mult[14];            // l l' F D Om L_Me L_Ve L_E L_Ma L_J L_Sa L_U L_Ne p_A
+};
+
+struct EopRecord {
+    double mjd, xp, yp, ut1_utc; // MJD(UTC), pole (rad), UT1-UTC (s)
+};
+
+struct FrameData {
+    std::vector<NutationTerm> lon0, lon1, obl0, obl1;
+    std::vector<EopRecord> eop;
+    std::vector<double> leap_mjd, leap_sec;
+};
+
+// One record per (n, m) visited by the device's column recursion, column-major (m outer,
+// n = m..L inner). Eight coefficients:
+//   xi, eta : recursion Pbar_n^m = xi * eps * Pbar_{n-1}^m - eta * Pbar_{n-2}^m
+//             (for n = m + 1: xi = sqrt(2m + 3), eta = 0; unused on the diagonal n = m)
+//   c, s    : Cbar_nm, Sbar_nm                       (J and K sums)
+//   kc, ks  : (1 + n + m) Cbar_nm, (1 + n + m) Sbar_nm (Gamma sum)
+//   zc, zs  : zeta_{n,m-1} Cbar_{n,m-1}, zeta_{n,m-1} Sbar_{n,m-1} (H sum carried by column m)
+// Coefficients of degrees n < 2 are zero: the central term is added separately
+// (gravity_source_device.cu:146-152, :219).
+struct GravityModel {
+    int L = 0;         // degree/order used
+    int file_max = 0;  // max_degree of the file
+    double gm = 0, radius = 0;
+    std::vector<double> C, S;       // triangular, index n(n+1)/2 + m
+    std::vector<double> records;    // 8 doubles per record
+    std::vector<float> records_f;   // same in FP32
+    std::vector<double> diag_ratio; // Pbar_mm / Pbar_{m-1,m-1} (m >= 2), [1] = sqrt(3)
+    size_t num_records() const { return records.size() / 8; }
+};
+
+// Throws std::runtime_error with a message naming the file on any failure.
+GravityModel load_gravity(const std::string& path, int degree);
+FrameData load_frames(const std::string& dir);
+
+} // namespace ogp

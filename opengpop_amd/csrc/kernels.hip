@@ -1,0 +1,831 @@
+// Propagation kernels for gfx950: time nodes, gravity, RK8 startup and the GJ8 step chunks.
+#include "kernels.hpp"
+
+#include <math.h>
+
+namespace ogp {
+
+// gpu_constants.cuh:37-98 (Gauss-Jackson and summed-Adams ordinate tables, [row][k]).
+__constant__ double c_gj[10][9] = {
+    { 3250433.0 / 53222400.0, 572741.0 / 5702400.0, -8701681.0 / 39916800.0,
+        4026311.0 / 13305600.0, -917039.0 / 3193344.0, 7370669.0 / 39916800.0,
+        -1025779.0 / 13305600.0, 754331.0 / 39916800.0, -330157.0 / 159667200.0 },
+    { -330157.0 / 159667200.0, 530113.0 / 6652800.0, 518887.0 / 19958400.0, -27631.0 / 623700.0,
+        44773.0 / 1064448.0, -531521.0 / 19958400.0, 109343.0 / 9979200.0, -1261.0 / 475200.0,
+        45911.0 / 159667200.0 },
+    { 45911.0 / 159667200.0, -185839.0 / 39916800.0, 171137.0 / 1900800.0,
+        73643.0 / 39916800.0, -25775.0 / 3193344.0, 77597.0 / 13305600.0,
+        -98911.0 / 39916800.0, 24173.0 / 39916800.0, -3499.0 / 53222400.0 },
+    { -3499.0 / 53222400.0, 4387.0 / 4989600.0, -35039.0 / 4989600.0, 90817.0 / 950400.0,
+        -20561.0 / 3193344.0, 2117.0 / 9979200.0, 2059.0 / 6652800.0, -317.0 / 2851200.0,
+        317.0 / 22809600.0 },
+    { 317.0 / 22809600.0, -2539.0 / 13305600.0, 55067.0 / 39916800.0, -326911.0 / 39916800.0,
+        14797.0 / 152064.0, -326911.0 / 39916800.0, 55067.0 / 39916800.0,
+        -2539.0 / 13305600.0, 317.0 / 22809600.0 },
+    { 317.0 / 22809600.0, -317.0 / 2851200.0, 2059.0 / 6652800.0, 2117.0 / 9979200.0,
+        -20561.0 / 3193344.0, 90817.0 / 950400.0, -35039.0 / 4989600.0, 4387.0 / 4989600.0,
+        -3499.0 / 53222400.0 },
+    { -3499.0 / 53222400.0, 24173.0 / 39916800.0, -98911.0 / 39916800.0,
+        77597.0 / 13305600.0, -25775.0 / 3193344.0, 73643.0 / 39916800.0,
+        171137.0 / 1900800.0, -185839.0 / 39916800.0, 45911.0 / 159667200.0 },
+    { 45911.0 / 159667200.0, -1261.0 / 475200.0, 109343.0 / 9979200.0,
+        -531521.0 / 19958400.0, 44773.0 / 1064448.0, -27631.0 / 623700.0,
+        518887.0 / 19958400.0, 530113.0 / 6652800.0, -330157.0 / 159667200.0 },
+    { -330157.0 / 159667200.0, 754331.0 / 39916800.0, -1025779.0 / 13305600.0,
+        7370669.0 / 39916800.0, -917039.0 / 3193344.0, 4026311.0 / 13305600.0,
+        -8701681.0 / 39916800.0, 572741.0 / 5702400.0, 3250433.0 / 53222400.0 },
+    { 3250433.0 / 53222400.0, -11011481.0 / 19958400.0, 6322573.0 / 2851200.0,
+        -8660609.0 / 1663200.0, 25162927.0 / 3193344.0, -159314453.0 / 19958400.0,
+        18071351.0 / 3326400.0, -24115843.0 / 9979200.0, 103798439.0 / 159667200.0 }
+};
+
+__constant__ double c_sa[10][9] = {
+    { 19087.0 / 89600.0, -427487.0 / 725760.0, 3498217.0 / 3628800.0, -500327.0 / 403200.0,
+        6467.0 / 5670.0, -2616161.0 / 3628800.0, 24019.0 / 80640.0, -263077.0 / 3628800.0,
+        8183.0 / 1036800.0 },
+    { 8183.0 / 1036800.0, 57251.0 / 403200.0, -1106377.0 / 3628800.0, 218483.0 / 725760.0,
+        -69.0 / 280.0, 530177.0 / 3628800.0, -210359.0 / 3628800.0, 5533.0 / 403200.0,
+        -425.0 / 290304.0 },
+    { -425.0 / 290304.0, 76453.0 / 3628800.0, 5143.0 / 57600.0, -660127.0 / 3628800.0,
+        661.0 / 5670.0, -4997.0 / 80640.0, 83927.0 / 3628800.0, -19109.0 / 3628800.0,
+        7.0 / 12800.0 },
+    { 7.0 / 12800.0, -23173.0 / 3628800.0, 29579.0 / 725760.0, 2497.0 / 57600.0,
+        -2563.0 / 22680.0, 172993.0 / 3628800.0, -6463.0 / 403200.0, 2497.0 / 725760.0,
+        -2497.0 / 7257600.0 },
+    { -2497.0 / 7257600.0, 1469.0 / 403200.0, -68119.0 / 3628800.0, 252769.0 / 3628800.0, 0.0,
+        -252769.0 / 3628800.0, 68119.0 / 3628800.0, -1469.0 / 403200.0, 2497.0 / 7257600.0 },
+    { 2497.0 / 7257600.0, -2497.0 / 725760.0, 6463.0 / 403200.0, -172993.0 / 3628800.0,
+        2563.0 / 22680.0, -2497.0 / 57600.0, -29579.0 / 725760.0, 23173.0 / 3628800.0,
+        -7.0 / 12800.0 },
+    { -7.0 / 12800.0, 19109.0 / 3628800.0, -83927.0 / 3628800.0, 4997.0 / 80640.0,
+        -661.0 / 5670.0, 660127.0 / 3628800.0, -5143.0 / 57600.0, -76453.0 / 3628800.0,
+        425.0 / 290304.0 },
+    { 425.0 / 290304.0, -5533.0 / 403200.0, 210359.0 / 3628800.0, -530177.0 / 3628800.0,
+        69.0 / 280.0, -218483.0 / 725760.0, 1106377.0 / 3628800.0, -57251.0 / 403200.0,
+        -8183.0 / 1036800.0 },
+    { -8183.0 / 1036800.0, 263077.0 / 3628800.0, -24019.0 / 80640.0, 2616161.0 / 3628800.0,
+        -6467.0 / 5670.0, 500327.0 / 403200.0, -3498217.0 / 3628800.0, 427487.0 / 725760.0,
+        -19087.0 / 89600.0 },
+    { 25713.0 / 89600.0, -9401029.0 / 3628800.0, 5393233.0 / 518400.0, -9839609.0 / 403200.0,
+        167287.0 / 4536.0, -135352319.0 / 3628800.0, 10219841.0 / 403200.0,
+        -40987771.0 / 3628800.0, 3288521.0 / 1036800.0 }
+};
+
+__constant__ double c_rk8b[10] = { 41.0 / 840.0, 0.0, 0.0, 27.0 / 840.0, 272.0 / 840.0,
+    27.0 / 840.0, 216.0 / 840.0, 0.0, 216.0 / 840.0, 41.0 / 840.0 };
+__constant__ double c_rk8c[10] = { 0.0, 4.0 / 27.0, 2.0 / 9.0, 1.0 / 3.0, 1.0 / 2.0,
+    2.0 / 3.0, 1.0 / 6.0, 1.0, 5.0 / 6.0, 1.0 };
+// RK8 stage matrix (gpu_constants.cuh:25-35). Variant 0 = the reference's row 7 (-234/20,
+// which sums to 17/20 although c7 = 1); variant 1 = -231/20 (row sum 1, 8th-order linear
+// conditions hold).
+__constant__ double c_rk8a[2][9][9] = {
+    { { 4.0 / 27, 0, 0, 0, 0, 0, 0, 0, 0 }, { 1.0 / 18, 3.0 / 18, 0, 0, 0, 0, 0, 0, 0 },
+        { 1.0 / 12, 0, 3.0 / 12, 0, 0, 0, 0, 0, 0 }, { 1.0 / 8, 0, 0, 3.0 / 8, 0, 0, 0, 0, 0 },
+        { 13.0 / 54, 0, -27.0 / 54, 42.0 / 54, 8.0 / 54, 0, 0, 0, 0 },
+        { 389.0 / 4320, 0, -54.0 / 4320, 966.0 / 4320, -824.0 / 4320.0, 243.0 / 4320, 0, 0, 0 },
+        { -234.0 / 20, 0, 81.0 / 20, -1164.0 / 20, 656.0 / 20, -122.0 / 20, 800.0 / 20, 0, 0 },
+        { -127.0 / 288, 0, 18.0 / 288, -678.0 / 288, 456.0 / 288, -9.0 / 288, 576.0 / 288,
+            4.0 / 288, 0 },
+        { 1481.0 / 820, 0, -81.0 / 820, 7104.0 / 820, -3376.0 / 820, 72.0 / 820, -5040.0 / 820,
+            -60.0 / 820, 720.0 / 820 } },
+    { { 4.0 / 27, 0, 0, 0, 0, 0, 0, 0, 0 }, { 1.0 / 18, 3.0 / 18, 0, 0, 0, 0, 0, 0, 0 },
+        { 1.0 / 12, 0, 3.0 / 12, 0, 0, 0, 0, 0, 0 }, { 1.0 / 8, 0, 0, 3.0 / 8, 0, 0, 0, 0, 0 },
+        { 13.0 / 54, 0, -27.0 / 54, 42.0 / 54, 8.0 / 54, 0, 0, 0, 0 },
+        { 389.0 / 4320, 0, -54.0 / 4320, 966.0 / 4320, -824.0 / 4320.0, 243.0 / 4320, 0, 0, 0 },
+        { -231.0 / 20, 0, 81.0 / 20, -1164.0 / 20, 656.0 / 20, -122.0 / 20, 800.0 / 20, 0, 0 },
+        { -127.0 / 288, 0, 18.0 / 288, -678.0 / 288, 456.0 / 288, -9.0 / 288, 576.0 / 288,
+            4.0 / 288, 0 },
+        { 1481.0 / 820, 0, -81.0 / 820, 7104.0 / 820, -3376.0 / 820, 72.0 / 820, -5040.0 / 820,
+            -60.0 / 820, 720.0 / 820 } }
+};
+
+// =========================================================================================
+// Time nodes: ITRS->GCRS matrix per node (time_points.cu:26-71, ref_sys_trans.cu)
+// =========================================================================================
+namespace frames {
+
+constexpr double kPi = 3.14159265358979324;
+constexpr double kArcsPerRad = 3600.0 * 180.0 / kPi;
+constexpr double kMjdJ2000 = 51544.5;
+
+__device__ double calendar_to_mjd(int year, int mon, int day, int h, int min, double sec)
+{
+    if (mon <= 2) {
+        --year;
+        mon += 12;
+    }
+    int B;
+    const long key = 10000L * year + 100L * mon + day;
+    if (key <= 15821004L)
+        B = -2 + ((year + 4716) / 4) - 1179;
+    else if (key >= 15821015L)
+        B = (year / 400) - (year / 100) + (year / 4);
+    else
+        return -1.0;
+    const long mjd = 365L * year - 679004L + B + (int)(30.6001 * (mon + 1)) + day;
+    const double frac = (h + min / 60.0 + sec / 3600.0) / 24.0;
+    return (double)mjd + frac;
+}
+
+struct Eop {
+    double xp, yp, ut1_utc;
+};
+
+// Binary search and linear interpolation, clamped at the table ends (types.cuh:79-130).
+__device__ Eop eop_at(const FrameDev& fd, double t)
+{
+    const double* e = fd.eop;
+    const int64_t n = fd.n_eop;
+    if (t <= e[0])
+        return { e[1], e[2], e[3] };
+    if (t >= e[4 * (n - 1)])
+        return { e[4 * (n - 1) + 1], e[4 * (n - 1) + 2], e[4 * (n - 1) + 3] };
+    int64_t lo = 0, hi = n - 1;
+    while (lo <= hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        const double c = e[4 * mid];
+        if (c == t)
+            return { e[4 * mid + 1], e[4 * mid + 2], e[4 * mid + 3] };
+        if (c < t)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    const double* a = e + 4 * hi;
+    const double* b = e + 4 * lo;
+    const double f = (t - a[0]) / (b[0] - a[0]);
+    return { a[1] + f * (b[1] - a[1]), a[2] + f * (b[2] - a[2]), a[3] + f * (b[3] - a[3]) };
+}
+
+__device__ double leap_at(const FrameDev& fd, double t)
+{
+    for (int i = fd.n_leap; i-- > 0;)
+        if (t >= fd.leap_mjd[i])
+            return fd.leap_sec[i];
+    return fd.leap_sec[0];
+}
+
+// IERS 2010 eq. 5.43 and 5.44 (Delaunay and planetary arguments), mod 2 pi.
+__device__ void fund_args(double t, double* F)
+{
+    const double l[5] = { 134.96340251 * 3600.0, 1717915923.2178, 31.8792, 0.051635, -0.00024470 };
+    const double lp[5] = { 357.52910918 * 3600.0, 129596581.0481, -0.5532, 0.000136, -0.00001149 };
+    const double f[5] = { 93.27209062 * 3600.0, 1739527262.8478, -12.7512, -0.001037, 0.00000417 };
+    const double d[5] = { 297.85019547 * 3600.0, 1602961601.2090, -6.3706, 0.006593, -0.00003169 };
+    const double om[5] = { 125.04455501 * 3600.0, -6962890.5431, 7.4722, 0.007702, -0.00005939 };
+    const double* ls[5] = { l, lp, f, d, om };
+    for (int q = 0; q < 5; ++q) {
+        double v = ls[q][4];
+        for (int i = 3; i >= 0; --i)
+            v = v * t + ls[q][i];
+        F[q] = fmod(v / kArcsPerRad, 2.0 * M_PI);
+    }
+    const double pl[9][3] = { { 4.402608842, 2608.7903141574, 0.0 },
+        { 3.176146697, 1021.3285546211, 0.0 }, { 1.753470314, 628.3075849991, 0.0 },
+        { 6.203480913, 334.0612426700, 0.0 }, { 0.599546497, 52.9690962641, 0.0 },
+        { 0.874016757, 21.3299104960, 0.0 }, { 5.481293872, 7.4781598567, 0.0 },
+        { 5.311886287, 3.8133035638, 0.0 }, { 0.0, 0.02438175, 0.00000538691 } };
+    for (int q = 0; q < 9; ++q)
+        F[5 + q] = fmod((pl[q][2] * t + pl[q][1]) * t + pl[q][0], 2.0 * M_PI);
+}
+
+// IERS 2010 eq. 5.40 (epsilon_A, gamma_bar, phi_bar, psi_bar), radians.
+__device__ void precession(double t, double* o)
+{
+    const double c[4][6] = {
+        { 84381.406, -46.836769, -0.0001831, 0.00200340, -0.000000576, -0.0000000434 },
+        { -0.052928, 10.556378, 0.4932044, -0.00031238, -0.000002788, 0.0000000260 },
+        { 84381.412819, -46.811016, 0.0511268, 0.00053289, -0.000000440, -0.0000000176 },
+        { -0.041775, 5038.481484, 1.5584175, -0.00018522, -0.000026452, -0.0000000148 } };
+    for (int q = 0; q < 4; ++q) {
+        double v = c[q][5];
+        for (int i = 4; i >= 0; --i)
+            v = v * t + c[q][i];
+        o[q] = v / kArcsPerRad;
+    }
+}
+
+// Greenwich apparent sidereal time, equinox method (ref_sys_trans.cu:75-119).
+__device__ double gast(double mjd_ut1, double mjd_tt, double dpsi, double epsA, const double* F)
+{
+    const double T = (mjd_tt - kMjdJ2000) / 36525.0;
+    const double Du = mjd_ut1 - kMjdJ2000;
+    double gmst = 86400.0 * (0.7790572732640 + 0.00273781191135448 * Du + (Du - floor(Du)));
+    const double c[6] = { 0.00096707, 307.47710227, 0.092772113, -0.0000000293, -0.00000199707,
+        -2.453e-9 };
+    double poly = c[5];
+    for (int i = 4; i >= 0; --i)
+        poly = poly * T + c[i];
+    gmst = (gmst + poly) * (M_PI / 43200.0);
+    const double l = F[0], lp = F[1], f = F[2], D = F[3], Om = F[4];
+    double ee = -0.00000087 * T * sin(Om);
+    ee -= 0.00000063 * sin(l - Om);
+    ee -= 0.00000063 * sin(l + Om);
+    ee -= 0.00000126 * sin(lp - Om);
+    ee -= 0.00000141 * sin(lp + Om);
+    ee -= 0.00000172 * sin(3.0 * Om);
+    ee += 0.00000198 * sin(2.0 * f + Om);
+    ee += 0.00000202 * sin(2.0 * f + 3.0 * Om);
+    ee -= 0.00000455 * sin(2.0 * f - 2.0 * D + 2.0 * Om);
+    ee += 0.00001121 * sin(2.0 * f - 2.0 * D + Om);
+    ee += 0.00001175 * sin(2.0 * f - 2.0 * D + 3.0 * Om);
+    ee += 0.00006352 * sin(2.0 * Om);
+    ee += 0.00264096 * sin(Om);
+    ee = ee * ((M_PI / 43200.0) / 15.0) + dpsi * cos(epsA);
+    return fmod(gmst + ee, 2.0 * M_PI);
+}
+
+// Rotation matrices in the SOFA sign convention, column-major (ref_sys_trans.cu:24-67).
+__device__ void rot(int axis, double a, double* p)
+{
+    const double c = cos(a), s = sin(a);
+    for (int i = 0; i < 9; ++i)
+        p[i] = 0.0;
+    const int i1 = (axis + 1) % 3, i2 = (axis + 2) % 3;
+    p[axis * 3 + axis] = 1.0;
+    p[i1 * 3 + i1] = c;
+    p[i2 * 3 + i2] = c;
+    // element (i1, i2) = s, (i2, i1) = -s for x and z; y follows the same cyclic rule
+    p[i2 * 3 + i1] = s;
+    p[i1 * 3 + i2] = -s;
+}
+
+__device__ void mat3(const double* A, const double* B, double* C)
+{
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k)
+                s = fma(A[k * 3 + i], B[j * 3 + k], s);
+            C[j * 3 + i] = s;
+        }
+}
+
+// Partial nutation sum over terms i = lane, lane + 64, ... (largest index first, like the
+// reference's smallest-first order, ref_sys_trans.cu:241-254), then a wave reduction.
+__device__ double series(const double* terms, int n, const double* F, int lane)
+{
+    constexpr double scale = 1.0 / (kArcsPerRad * 1e6);
+    double acc = 0.0;
+    for (int i = n - 1 - lane; i >= 0; i -= 64) {
+        const double* q = terms + 16 * i;
+        double arg = 0.0;
+        for (int k = 13; k >= 0; --k)
+            arg = fma(F[k], q[2 + k], arg);
+        double s, c;
+        sincos(arg, &s, &c);
+        acc = fma(q[0] * scale, s, acc);
+        acc = fma(q[1] * scale, c, acc);
+    }
+    for (int off = 32; off >= 1; off >>= 1)
+        acc += __shfl_xor(acc, off, 64);
+    return acc;
+}
+
+} // namespace frames
+
+__global__ void __launch_bounds__(64) time_nodes_kernel(FrameDev fd, int y, int mo, int d, int hh,
+    int mi, double sec_shift, const double* offsets, int64_t len, double* M, double* mjd_out)
+{
+    using namespace frames;
+    const int64_t node = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (node >= len)
+        return;
+    const double mjd = calendar_to_mjd(y, mo, d, hh, mi, sec_shift + offsets[node]);
+    const Eop e = eop_at(fd, mjd);
+    const double mjd_ut1 = mjd + e.ut1_utc / 86400.0;
+    const double mjd_tt = mjd + (leap_at(fd, mjd) + 32.184) / 86400.0;
+    const double t = (mjd_tt - kMjdJ2000) / 36525.0;
+    double F[14];
+    fund_args(t, F);
+    const double lon0 = series(fd.nut[0], fd.n_nut[0], F, lane);
+    const double lon1 = series(fd.nut[1], fd.n_nut[1], F, lane);
+    const double obl0 = series(fd.nut[2], fd.n_nut[2], F, lane);
+    const double obl1 = series(fd.nut[3], fd.n_nut[3], F, lane);
+    if (lane != 0)
+        return;
+    const double dpsi = lon0 + lon1 * t;
+    const double deps = obl0 + obl1 * t;
+    double pr[4];
+    precession(t, pr);
+    double Q1[9], Q2[9], Q3[9], Q4[9], W1[9], W2[9], W3[9], R[9], A[9], B[9], Q[9], W[9];
+    rot(2, -pr[1], Q1);
+    rot(0, -pr[2], Q2);
+    rot(2, pr[3] + dpsi, Q3);
+    rot(0, pr[0] + deps, Q4);
+    const double sp = -47e-6 * t / kArcsPerRad;
+    rot(2, -sp, W1);
+    rot(1, e.xp, W2);
+    rot(0, e.yp, W3);
+    rot(2, -gast(mjd_ut1, mjd_tt, dpsi, pr[0], F), R);
+    mat3(Q1, Q2, A);
+    mat3(Q3, Q4, B);
+    mat3(A, B, Q);
+    mat3(W1, W2, A);
+    mat3(A, W3, W);
+    mat3(Q, R, A);
+    mat3(A, W, B);
+    for (int i = 0; i < 9; ++i)
+        M[node * 9 + i] = B[i];
+    mjd_out[node] = mjd;
+}
+
+void launch_time_nodes(hipStream_t st, const FrameDev& fd, const int* ymdhm, double sec_shift,
+    const double* d_offsets, int64_t len, double* d_M, double* d_mjd)
+{
+    if (len <= 0)
+        return;
+    hipLaunchKernelGGL(time_nodes_kernel, dim3((unsigned)len), dim3(64), 0, st, fd, ymdhm[0],
+        ymdhm[1], ymdhm[2], ymdhm[3], ymdhm[4], sec_shift, d_offsets, len, d_M, d_mjd);
+}
+
+// =========================================================================================
+// Standalone gravity evaluation (GravitySource::ComputeAcceleration)
+// =========================================================================================
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) gravity_aos_kernel(GravityDev g, const double* M, const double* r, int64_t n,
+    double* a, int64_t out_stride)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double p[3] = { r[3 * i], r[3 * i + 1], r[3 * i + 2] };
+    double o[3];
+    if (M)
+        gravity_gcrs<SPLIT>(M, p, g, o);
+    else
+        gravity_itrs<SPLIT>(p[0], p[1], p[2], g, o[0], o[1], o[2]);
+    a[i * out_stride] = o[0];
+    a[i * out_stride + 1] = o[1];
+    a[i * out_stride + 2] = o[2];
+}
+
+void launch_gravity_aos(hipStream_t st, bool split, const GravityDev& g, const double* d_M,
+    const double* d_r, int64_t n, double* d_a, int64_t out_stride, int block)
+{
+    if (n <= 0)
+        return;
+    const dim3 grid((unsigned)((n + block - 1) / block));
+    if (split)
+        hipLaunchKernelGGL(gravity_aos_kernel<true>, grid, dim3(block), 0, st, g, d_M, d_r, n,
+            d_a, out_stride);
+    else
+        hipLaunchKernelGGL(gravity_aos_kernel<false>, grid, dim3(block), 0, st, g, d_M, d_r, n,
+            d_a, out_stride);
+}
+
+// =========================================================================================
+// Startup: RK8 4 forward + 4 backward steps and the startup corrector, one object per lane
+// (gpu_startup.cu:55-312 and src/device/*.cu). Work arrays are [slot][component][object].
+// =========================================================================================
+namespace {
+
+struct Slots {
+    double* base;
+    int64_t N, i;
+    __device__ double& operator()(int slot, int c) const { return base[(slot * 3 + c) * N + i]; }
+};
+
+__device__ __forceinline__ int ring_slot(int64_t j) { return (int)(((j % 9) + 9) % 9); }
+
+template <int KIND>
+__device__ void rk8_step(const ForceCtx& f, const double (*a)[9], double h, const Slots& nr,
+    const Slots& nv, const Slots& Kx, const Slots& Kv, int step)
+{
+    const double hs = step > 0 ? h : -h;
+    const int prev = 4 + step + (step > 0 ? -1 : 1);
+    double x0[3], v0[3], tX[3], tV[3], out[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        x0[c] = nr(prev, c);
+        v0[c] = nv(prev, c);
+        Kx(0, c) = v0[c];
+    }
+    force_gen<KIND>(f, x0, v0, step, 0, out);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        Kv(0, c) = out[c];
+    for (int k = 1; k <= 9; ++k) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double sx = 0.0, sv = 0.0;
+            for (int j = 0; j < k; ++j) {
+                sx = fma(Kx(j, c), a[k - 1][j], sx);
+                sv = fma(Kv(j, c), a[k - 1][j], sv);
+            }
+            tX[c] = fma(hs, sx, x0[c]);
+            tV[c] = fma(hs, sv, v0[c]);
+        }
+        force_gen<KIND>(f, tX, tV, step, k, out);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Kv(k, c) = out[c];
+            Kx(k, c) = tV[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double sv = 0.0, sx = 0.0;
+        for (int k = 0; k < 10; ++k) {
+            sv = fma(Kv(k, c), c_rk8b[k], sv);
+            sx = fma(Kx(k, c), c_rk8b[k], sx);
+        }
+        nv(4 + step, c) = fma(hs, sv, v0[c]);
+        nr(4 + step, c) = fma(hs, sx, x0[c]);
+    }
+}
+
+template <int KIND>
+__device__ void acc_matrix(const ForceCtx& f, const Slots& nr, const Slots& nv, const Slots& acc,
+    bool epoch)
+{
+    for (int j = 0; j < 9; ++j) {
+        if (j == 4 && !epoch)
+            continue;
+        double x[3], v[3], o[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[c] = nr(j, c);
+            v[c] = nv(j, c);
+        }
+        force_prop<KIND>(f, x, v, j, o);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            acc(j, c) = o[c];
+    }
+}
+
+// StartupCorrector::Correct (gpu_startup.cu:247-294) with CalculateS0 (:170-195) and
+// CalculateAllSKernel (calculate_all_s.cu:19-47).
+template <int KIND>
+__device__ void startup_correct(const ForceCtx& f, double h, const Slots& nr, const Slots& nv,
+    const Slots& acc, const Slots& sm, const Slots& Sm)
+{
+    const double invh = 1.0 / h, invh2 = 1.0 / (h * h), hh = h * h;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double ss = 0.0, sS = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            ss = fma(acc(k, c), c_sa[4][k], ss);
+            sS = fma(acc(k, c), c_gj[4][k], sS);
+        }
+        sm(4, c) = fma(invh, nv(4, c), -ss);
+        Sm(4, c) = fma(invh2, nr(4, c), -sS);
+        for (int i = 1; i <= 4; ++i) {
+            sm(4 + i, c) = sm(4 + i - 1, c) + (acc(4 + i - 1, c) + acc(4 + i, c)) / 2;
+            sm(4 - i, c) = sm(4 - i + 1, c) - (acc(4 - i + 1, c) + acc(4 - i, c)) / 2;
+            Sm(4 + i, c) = sm(4 + i - 1, c) + Sm(4 + i - 1, c) + acc(4 + i - 1, c) / 2;
+            Sm(4 - i, c) = -sm(4 - i + 1, c) + Sm(4 - i + 1, c) + acc(4 - i + 1, c) / 2;
+        }
+        for (int j = 0; j < 9; ++j) {
+            if (j == 4)
+                continue;
+            double sv = 0.0, sp = 0.0;
+            for (int k = 0; k < 9; ++k) {
+                sv = fma(acc(k, c), c_sa[j][k], sv);
+                sp = fma(acc(k, c), c_gj[j][k], sp);
+            }
+            nv(j, c) = (sm(j, c) + sv) * h;
+            nr(j, c) = (Sm(j, c) + sp) * hh;
+        }
+    }
+    acc_matrix<KIND>(f, nr, nv, acc, false);
+}
+
+} // namespace
+
+template <int KIND>
+__global__ void __launch_bounds__(256) startup_kernel(ForceCtx f, int rk8_variant, PropState ps, StartupScratch sc,
+    int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps)
+// traj rows 0..3 receive steps 1..4 (rows past traj_steps are not written)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nobj)
+        return;
+    const int64_t i = obj0 + t;
+    const int64_t N = ps.N;
+    const int64_t ns = ps.nsteps[i];
+    if (ns <= 0)
+        return; // no GpuPropagator::Step call: the state is left untouched
+    const Slots nr { sc.node_r, N, i }, nv { sc.node_v, N, i }, acc { sc.acc, N, i };
+    const Slots sm { sc.sm, N, i }, Sm { sc.Sm, N, i }, Kx { sc.Kx, N, i }, Kv { sc.Kv, N, i };
+    const double h = f.h;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        nr(4, c) = ps.r[c * N + i];
+        nv(4, c) = ps.v[c * N + i];
+    }
+    const double(*a)[9] = c_rk8a[rk8_variant];
+    for (int s = 1; s <= 4; ++s) {
+        rk8_step<KIND>(f, a, h, nr, nv, Kx, Kv, s);
+        rk8_step<KIND>(f, a, h, nr, nv, Kx, Kv, -s);
+    }
+    acc_matrix<KIND>(f, nr, nv, acc, true);
+    for (int it = 0; it < ncorr; ++it)
+        startup_correct<KIND>(f, h, nr, nv, acc, sm, Sm);
+
+    // Hand-over to the GJ8 steps (gpu_propagator.cu:31-53, copy_s.cu:19-30).
+    const int last = (int)(ns < 4 ? ns : 4);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ps.s[c * N + i] = sm(8, c);
+        ps.S[c * N + i] = Sm(8, c);
+        ps.r[c * N + i] = nr(4 + last, c);
+        ps.v[c * N + i] = nv(4 + last, c);
+        for (int j = -4; j <= 4; ++j)
+            ps.hist[(ring_slot(j) * 3 + c) * N + i] = acc(j + 4, c);
+    }
+    if (traj) {
+        // steps 1-4 output the startup nodes; the acceleration handed to the conjunction
+        // screen is the acceleration-matrix column 8 (gpu_propagator.cu:68-72)
+        for (int k = 1; k <= last && k <= traj_steps; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                traj[((int64_t)(k - 1) * 9 + c) * N + i] = nr(4 + k, c);
+                traj[((int64_t)(k - 1) * 9 + 3 + c) * N + i] = nv(4 + k, c);
+                traj[((int64_t)(k - 1) * 9 + 6 + c) * N + i] = acc(8, c);
+            }
+    }
+}
+
+// =========================================================================================
+// GJ8 steps k0 .. k1-1 (each k >= 5), persistent over the chunk, one object per lane:
+// Predictor::Predict (prediction.cu:49-74, dev_functions.cu:20-50) then the corrector
+// (gpu_propagator.cu:86-119, dev_functions.cu:36-69). The history is a ring of 9 slots.
+// The first corrector evaluation would see exactly the predicted state and reproduce the
+// predictor's evaluation bit for bit, so it is reused: k distinct evaluations per step.
+// =========================================================================================
+template <int KIND>
+__global__ void __launch_bounds__(256) gj8_chunk_kernel(ForceCtx f, PropState ps, int64_t obj0, int64_t nobj,
+    int64_t k0, int64_t k1, int ncorr, double* traj, int64_t traj_first, int64_t traj_steps)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nobj)
+        return;
+    const int64_t i = obj0 + t;
+    const int64_t N = ps.N;
+    const int64_t ns = ps.nsteps[i];
+    if (ns < k0)
+        return;
+    const double h = f.h, hh = h * h;
+    const double sa88 = c_sa[8][8], gj88 = c_gj[8][8];
+    double r[3], v[3], s[3], S[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r[c] = ps.r[c * N + i];
+        v[c] = ps.v[c * N + i];
+        s[c] = ps.s[c * N + i];
+        S[c] = ps.S[c * N + i];
+    }
+    for (int64_t k = k0; k < k1; ++k) {
+        if (k > ns)
+            continue; // this object's span has ended; the wave continues for the others
+        double Pv[3] = { 0, 0, 0 }, Pr[3] = { 0, 0, 0 }, T1[3] = { 0, 0, 0 }, T2[3] = { 0, 0, 0 };
+        double alast[3];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int slot = ring_slot(k - 9 + q);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double a = ps.hist[(slot * 3 + c) * N + i];
+                Pv[c] = fma(c_sa[9][q], a, Pv[c]);
+                Pr[c] = fma(c_gj[9][q], a, Pr[c]);
+                if (q >= 1) {
+                    T1[c] = fma(c_sa[8][q - 1], a, T1[c]);
+                    T2[c] = fma(c_gj[8][q - 1], a, T2[c]);
+                }
+                if (q == 8)
+                    alast[c] = a;
+            }
+        }
+        double sprev[3], anew[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            S[c] += s[c] + alast[c] / 2;
+            v[c] = ((Pv[c] + s[c]) + alast[c] / 2) * h;
+            r[c] = ((Pr[c] + S[c]) * h) * h;
+            sprev[c] = s[c];
+        }
+        force_prop<KIND>(f, r, v, k + 4, anew);
+        for (int it = 0; it < ncorr; ++it) {
+            if (it > 0)
+                force_prop<KIND>(f, r, v, k + 4, anew);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                s[c] = sprev[c] + (anew[c] + alast[c]) / 2;
+                v[c] = h * ((T1[c] + anew[c] * sa88) + s[c]);
+                r[c] = hh * ((T2[c] + anew[c] * gj88) + S[c]);
+            }
+        }
+        const int wslot = ring_slot(k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            ps.hist[(wslot * 3 + c) * N + i] = anew[c];
+        if (traj && k >= traj_first && k < traj_first + traj_steps) {
+            const int64_t row = k - traj_first;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                traj[(row * 9 + c) * N + i] = r[c];
+                traj[(row * 9 + 3 + c) * N + i] = v[c];
+                traj[(row * 9 + 6 + c) * N + i] = anew[c];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ps.r[c * N + i] = r[c];
+        ps.v[c * N + i] = v[c];
+        ps.s[c * N + i] = s[c];
+        ps.S[c * N + i] = S[c];
+    }
+}
+
+#define OGP_KIND_SWITCH(kind, LAUNCH)                                                              \
+    switch (kind) {                                                                                \
+    case kGravity: LAUNCH(kGravity); break;                                                        \
+    case kGravitySplit: LAUNCH(kGravitySplit); break;                                              \
+    case kNone: LAUNCH(kNone); break;                                                              \
+    case kConst: LAUNCH(kConst); break;                                                            \
+    case kPoly7: LAUNCH(kPoly7); break;                                                            \
+    case kPoly8: LAUNCH(kPoly8); break;                                                            \
+    case kPoly9: LAUNCH(kPoly9); break;                                                            \
+    case kExp: LAUNCH(kExp); break;                                                                \
+    case kDamped: LAUNCH(kDamped); break;                                                          \
+    default: break;                                                                                \
+    }
+
+void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant,
+    const PropState& ps, const StartupScratch& sc, int64_t obj0, int64_t nobj, int ncorr,
+    double* traj, int64_t traj_steps, int block)
+{
+    if (nobj <= 0)
+        return;
+    const dim3 grid((unsigned)((nobj + block - 1) / block));
+#define L_(K)                                                                                      \
+    hipLaunchKernelGGL(startup_kernel<K>, grid, dim3(block), 0, st, f, rk8_variant, ps, sc, obj0,  \
+        nobj, ncorr, traj, traj_steps)
+    OGP_KIND_SWITCH(kind, L_)
+#undef L_
+}
+
+void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
+    int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
+    int64_t traj_first, int64_t traj_steps, int block)
+{
+    if (nobj <= 0 || k1 <= k0)
+        return;
+    const dim3 grid((unsigned)((nobj + block - 1) / block));
+#define L_(K)                                                                                      \
+    hipLaunchKernelGGL(gj8_chunk_kernel<K>, grid, dim3(block), 0, st, f, ps, obj0, nobj, k0, k1,   \
+        ncorr, traj, traj_first, traj_steps)
+    OGP_KIND_SWITCH(kind, L_)
+#undef L_
+}
+
+// =========================================================================================
+// Known-answer hooks: the predictor and corrector arithmetic of gj8_chunk_kernel on given
+// inputs (AoS [n][27] histories, [n][3] vectors).
+// =========================================================================================
+__global__ void kat_predict_kernel(const double* acc27, const double* s3, double* S3, int64_t n,
+    double h, double* r3, double* v3)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int c = 0; c < 3; ++c) {
+        double Pv = 0.0, Pr = 0.0;
+        for (int q = 0; q < 9; ++q) {
+            const double a = acc27[27 * i + 3 * q + c];
+            Pv = fma(c_sa[9][q], a, Pv);
+            Pr = fma(c_gj[9][q], a, Pr);
+        }
+        const double alast = acc27[27 * i + 24 + c];
+        const double s = s3[3 * i + c];
+        const double S = S3[3 * i + c] + (s + alast / 2);
+        S3[3 * i + c] = S;
+        v3[3 * i + c] = ((Pv + s) + alast / 2) * h;
+        r3[3 * i + c] = ((Pr + S) * h) * h;
+    }
+}
+
+__global__ void kat_finalize_kernel(double* r3, double* v3, const double* acc27, const double* s3,
+    const double* S3, int64_t n, double h)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int c = 0; c < 3; ++c) {
+        const double a = acc27[27 * i + 24 + c];
+        v3[3 * i + c] = h * ((v3[3 * i + c] + a * c_sa[8][8]) + s3[3 * i + c]);
+        r3[3 * i + c] = (h * h) * ((r3[3 * i + c] + a * c_gj[8][8]) + S3[3 * i + c]);
+    }
+}
+
+void launch_kat_predict(hipStream_t st, const double* acc27, const double* s3, double* S3,
+    int64_t n, double h, double* r3, double* v3)
+{
+    hipLaunchKernelGGL(kat_predict_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
+        acc27, s3, S3, n, h, r3, v3);
+}
+
+void launch_kat_finalize(hipStream_t st, double* r3, double* v3, const double* acc27,
+    const double* s3, const double* S3, int64_t n, double h)
+{
+    hipLaunchKernelGGL(kat_finalize_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, r3,
+        v3, acc27, s3, S3, n, h);
+}
+
+__global__ void aos_to_soa_kernel(const double* aos, double* soa, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    soa[i] = aos[3 * i];
+    soa[n + i] = aos[3 * i + 1];
+    soa[2 * n + i] = aos[3 * i + 2];
+}
+
+__global__ void soa_to_aos_kernel(const double* soa, double* aos, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    aos[3 * i] = soa[i];
+    aos[3 * i + 1] = soa[n + i];
+    aos[3 * i + 2] = soa[2 * n + i];
+}
+
+__global__ void gather_aos_kernel(const double* aos, const int64_t* perm, double* soa, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int64_t i = perm[p];
+    soa[p] = aos[3 * i];
+    soa[n + p] = aos[3 * i + 1];
+    soa[2 * n + p] = aos[3 * i + 2];
+}
+
+__global__ void scatter_aos_kernel(const double* soa, const int64_t* perm, double* aos, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int64_t i = perm[p];
+    aos[3 * i] = soa[p];
+    aos[3 * i + 1] = soa[n + p];
+    aos[3 * i + 2] = soa[2 * n + p];
+}
+
+__global__ void gather_soa_kernel(const double* src, const int64_t* perm, double* dst, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int64_t i = perm[p];
+    for (int c = 0; c < 3; ++c)
+        dst[c * n + p] = src[c * n + i];
+}
+
+__global__ void scatter_soa_kernel(const double* src, const int64_t* perm, double* dst, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n)
+        return;
+    const int64_t i = perm[p];
+    for (int c = 0; c < 3; ++c)
+        dst[c * n + i] = src[c * n + p];
+}
+
+void launch_permute(hipStream_t st, int mode, const double* src, const int64_t* perm, double* dst,
+    int64_t n)
+{
+    if (n <= 0)
+        return;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    if (mode == 0)
+        hipLaunchKernelGGL(gather_aos_kernel, g, b, 0, st, src, perm, dst, n);
+    else if (mode == 1)
+        hipLaunchKernelGGL(scatter_aos_kernel, g, b, 0, st, src, perm, dst, n);
+    else if (mode == 2)
+        hipLaunchKernelGGL(gather_soa_kernel, g, b, 0, st, src, perm, dst, n);
+    else
+        hipLaunchKernelGGL(scatter_soa_kernel, g, b, 0, st, src, perm, dst, n);
+}
+
+void launch_aos_to_soa(hipStream_t st, const double* aos, double* soa, int64_t n)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(aos_to_soa_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+            aos, soa, n);
+}
+
+void launch_soa_to_aos(hipStream_t st, const double* soa, double* aos, int64_t n)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+            soa, aos, n);
+}
+
+} // namespace ogp

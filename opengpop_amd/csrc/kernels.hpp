@@ -1,0 +1,76 @@
+// Host-visible launchers of the propagation kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace ogp {
+
+// Frame datasets on the device (time_points.cu:73-124 uploads the same tables).
+struct FrameDev {
+    const double* eop;      // 4 doubles per record: mjd, xp, yp, ut1-utc
+    int64_t n_eop;
+    const double* leap_mjd; // leap-second table
+    const double* leap_sec;
+    int n_leap;
+    const double* nut[4];   // lon j=0, lon j=1, obl j=0, obl j=1: 16 doubles per term
+    int n_nut[4];           // (sin, cos, 14 multipliers)
+};
+
+// Structure-of-arrays state of all objects. Component c of object i lives at [c * N + i];
+// history slot q (acceleration at time index j with q = j mod 9) at [(q * 3 + c) * N + i].
+struct PropState {
+    double* r;
+    double* v;
+    double* s;
+    double* S;
+    double* hist;
+    const int64_t* nsteps;
+    int64_t N;
+};
+
+// Per-object startup work areas ([slot][component][object]).
+struct StartupScratch {
+    double* node_r; // 9 x 3
+    double* node_v; // 9 x 3
+    double* acc;    // 9 x 3
+    double* sm;     // 9 x 3
+    double* Sm;     // 9 x 3
+    double* Kx;     // 10 x 3
+    double* Kv;     // 10 x 3
+};
+
+// Kernel variants: force kind (ForceKind; kGravitySplit for the split-precision gravity).
+void launch_time_nodes(hipStream_t st, const FrameDev& fd, const int* ymdhm, double sec_shift,
+    const double* d_offsets, int64_t len, double* d_M, double* d_mjd);
+
+void launch_gravity_aos(hipStream_t st, bool split, const GravityDev& g, const double* d_M,
+    const double* d_r, int64_t n, double* d_a, int64_t out_stride, int block);
+
+void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant,
+    const PropState& ps, const StartupScratch& sc, int64_t obj0, int64_t nobj, int ncorr,
+    double* traj, int64_t traj_steps, int block);
+
+// GJ8 steps k0 .. k1-1 (k0 >= 5) of objects [obj0, obj0 + nobj). With traj != null, steps
+// traj_first .. traj_first + traj_steps - 1 are stored at rows 0.. of traj[row][9][N].
+void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
+    int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
+    int64_t traj_first, int64_t traj_steps, int block);
+
+void launch_kat_predict(hipStream_t st, const double* acc27, const double* s3, double* S3,
+    int64_t n, double h, double* r3, double* v3);
+void launch_kat_finalize(hipStream_t st, double* r3, double* v3, const double* acc27,
+    const double* s3, const double* S3, int64_t n, double h);
+
+// Permutations between the caller's order and the internal (class-sorted) order.
+// mode 0: AoS src -> SoA dst gather (dst[p] = src[perm[p]]); 1: SoA -> AoS scatter;
+// 2: SoA -> SoA gather; 3: SoA -> SoA scatter.
+void launch_permute(hipStream_t st, int mode, const double* src, const int64_t* perm, double* dst,
+    int64_t n);
+
+// SoA <-> AoS helpers for the boundary (runner.cu:27-33 hands AoS arrays).
+void launch_aos_to_soa(hipStream_t st, const double* aos, double* soa, int64_t n);
+void launch_soa_to_aos(hipStream_t st, const double* soa, double* aos, int64_t n);
+
+} // namespace ogp

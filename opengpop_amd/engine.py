@@ -1,0 +1,226 @@
+"""Python mirror of the reference's library entry points over the C ABI.
+
+Names follow the reference: ForceModel (force_model.cuh:28-87), GpuPropagator
+(gpu_propagator.cuh:26-59), GpuStartup (gpu_startup.cuh:24-120), ClosestApproach
+(closest_approach.cuh:36-88) and ClosestApproachData::Get (types.cuh:153-169). All compute runs
+in libogp.so on the GPU; errors raise OgpError with the library's message.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi as C
+from ._capi import DATA_DIR, check, f64, lib, ptr
+
+
+class Context:
+    """One device context: datasets, configuration and the HIP stream (include/ogp.h)."""
+
+    def __init__(self, degree=-1, precision="fp64", n_corrections=7,
+                 quirks=C.QUIRKS_REFERENCE, force=C.FORCE_GRAVITY,
+                 epoch=(2025, 1, 1, 0, 0, 0.0), data_dir=DATA_DIR, split_degree=2):
+        L = lib()
+        cfg = C.OgpConfig()
+        L.ogp_default_config(ctypes.byref(cfg))
+        self._dir = data_dir.encode()
+        cfg.data_dir = self._dir
+        cfg.degree = degree
+        cfg.precision = C.PRECISION_SPLIT if precision == "split" else C.PRECISION_FP64
+        cfg.split_degree = split_degree
+        cfg.n_corrections = n_corrections
+        cfg.quirks = quirks
+        cfg.force = force
+        (cfg.year, cfg.month, cfg.day, cfg.hour, cfg.minute), cfg.second = epoch[:5], float(epoch[5])
+        h = ctypes.c_void_p()
+        check(None, L.ogp_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.epoch = tuple(epoch)
+        self.force = force
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ogp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_bytes(self):
+        return lib().ogp_device_bytes(self.h)
+
+    # -- force model -----------------------------------------------------------------
+    def gravity_itrs(self, r):
+        r = f64(r).reshape(-1, 3)
+        a = np.zeros_like(r)
+        check(self.h, lib().ogp_gravity_itrs(self.h, ptr(r), len(r), ptr(a)))
+        return a
+
+    def accel(self, r, step, node, out_stride=3):
+        r = f64(r).reshape(-1, 3)
+        n = len(r)
+        out = np.zeros(max(out_stride * (n - 1) + 3, 3))
+        check(self.h, lib().ogp_accel(self.h, ptr(r), None, n, step, node, ptr(out), out_stride))
+        return out
+
+    def time_nodes(self, step, before, offsets):
+        offsets = f64(offsets)
+        M = np.zeros((len(offsets), 9))
+        mjd = np.zeros(len(offsets))
+        check(self.h, lib().ogp_time_nodes(self.h, step, before, ptr(offsets), len(offsets),
+                                           ptr(M), ptr(mjd)))
+        return M, mjd
+
+    # -- propagation -----------------------------------------------------------------
+    def propagate(self, r0, v0, step, steps):
+        r0 = f64(r0).reshape(-1, 3)
+        v0 = f64(v0).reshape(-1, 3)
+        n = len(r0)
+        h = np.ascontiguousarray(np.broadcast_to(np.asarray(step, np.float64), (n,)))
+        ns = np.ascontiguousarray(np.broadcast_to(np.asarray(steps, np.int64), (n,)))
+        r = np.zeros_like(r0)
+        v = np.zeros_like(v0)
+        check(self.h, lib().ogp_propagate(self.h, ptr(r0), ptr(v0), n, ptr(h), ptr(ns), ptr(r),
+                                          ptr(v)))
+        return r, v
+
+    def timing(self):
+        t = C.OgpTiming()
+        check(self.h, lib().ogp_last_timing(self.h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in C.OgpTiming._fields_}
+
+    def startup(self, r0, v0, step):
+        """GpuStartup::Startup: node matrices, s, S and accelerations, each [n][9][3]."""
+        r0 = f64(r0).reshape(-1, 3)
+        v0 = f64(v0).reshape(-1, 3)
+        n = len(r0)
+        out = [np.zeros((n, 9, 3)) for _ in range(5)]
+        check(self.h, lib().ogp_startup(self.h, ptr(r0), ptr(v0), n, step, *[ptr(o) for o in out]))
+        return dict(zip(("pos", "vel", "s", "S", "acc"), out))
+
+    def kat_predict(self, acc27, s3, S3, h):
+        acc = f64(acc27).reshape(-1, 27)
+        n = len(acc)
+        s = f64(s3).reshape(n, 3)
+        S = np.array(S3, dtype=np.float64).reshape(n, 3)
+        r = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        check(self.h, lib().ogp_kat_predict(self.h, ptr(acc), ptr(s), ptr(S), n, h, ptr(r), ptr(v)))
+        return r, v, S
+
+    def kat_finalize_correction(self, r3, v3, acc27, s3, S3, h):
+        acc = f64(acc27).reshape(-1, 27)
+        n = len(acc)
+        r = np.array(r3, dtype=np.float64).reshape(n, 3)
+        v = np.array(v3, dtype=np.float64).reshape(n, 3)
+        s = f64(s3).reshape(n, 3)
+        S = f64(S3).reshape(n, 3)
+        check(self.h, lib().ogp_kat_finalize_correction(self.h, ptr(r), ptr(v), ptr(acc), ptr(s),
+                                                        ptr(S), n, h))
+        return r, v
+
+
+class GpuPropagator:
+    """Stepwise propagator (GpuPropagator::Step, gpu_propagator.cu:28-77)."""
+
+    def __init__(self, ctx, r0, v0, step, max_steps):
+        self.ctx = ctx
+        r0 = f64(r0).reshape(-1, 3)
+        v0 = f64(v0).reshape(-1, 3)
+        self.n = len(r0)
+        check(ctx.h, lib().ogp_prop_init(ctx.h, ptr(r0), ptr(v0), self.n, step, max_steps))
+
+    def step(self):
+        r = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        a = np.zeros((self.n, 3))
+        check(self.ctx.h, lib().ogp_prop_step(self.ctx.h, ptr(r), ptr(v), ptr(a)))
+        return r, v, a
+
+
+class ClosestApproach:
+    """ClosestApproach::{Initialize, Propagate, FindClosest} (closest_approach.cu:47-210)."""
+
+    def __init__(self, ctx, step):
+        self.ctx = ctx
+        self.step = step
+
+    def initialize(self, positions, velocities, set_indices):
+        r = f64(positions).reshape(-1, 3)
+        v = f64(velocities).reshape(-1, 3)
+        s = np.ascontiguousarray(set_indices, dtype=np.uint32)
+        check(self.ctx.h, lib().ogp_conj_init(self.ctx.h, ptr(r), ptr(v), ptr(s), len(r),
+                                              self.step))
+        self.sets = s
+
+    def propagate(self, num_steps):
+        check(self.ctx.h, lib().ogp_conj_propagate(self.ctx.h, num_steps))
+
+    def find_closest(self, threshold=0.0):
+        """Returns ClosestApproachData; threshold <= 0 refines every cross-set pair."""
+        cnt = C._I64()
+        check(self.ctx.h, lib().ogp_conj_find(self.ctx.h, threshold, ctypes.byref(cnt)))
+        n = cnt.value
+        a = np.zeros(n, np.uint32)
+        b = np.zeros(n, np.uint32)
+        k = np.zeros(n, np.int32)
+        d = np.zeros(n)
+        t = np.zeros(n)
+        pa = np.zeros((n, 3))
+        pb = np.zeros((n, 3))
+        check(self.ctx.h, lib().ogp_conj_results(self.ctx.h, ptr(a), ptr(b), ptr(k), ptr(d),
+                                                 ptr(t), ptr(pa), ptr(pb)))
+        return ClosestApproachData(a, b, k, d, t, pa, pb)
+
+    def screen(self):
+        np_ = C._I64()
+        check(self.ctx.h, lib().ogp_conj_num_pairs(self.ctx.h, ctypes.byref(np_)))
+        d = np.zeros(np_.value)
+        k = np.zeros(np_.value, np.int32)
+        check(self.ctx.h, lib().ogp_conj_screen(self.ctx.h, ptr(d), ptr(k)))
+        return d, k
+
+    def trajectory(self):
+        steps = C._I64()
+        check(self.ctx.h, lib().ogp_conj_trajectory(self.ctx.h, None, ctypes.byref(steps)))
+        out = np.zeros((steps.value, len(self.sets), 9))
+        check(self.ctx.h, lib().ogp_conj_trajectory(self.ctx.h, ptr(out), ctypes.byref(steps)))
+        return out
+
+    def timing(self):
+        t = C.OgpConjTiming()
+        check(self.ctx.h, lib().ogp_conj_last_timing(self.ctx.h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in C.OgpConjTiming._fields_}
+
+
+class ClosestApproachData:
+    """Results of FindClosest; get(i, j) mirrors ClosestApproachData::Get (types.cuh:153-169)."""
+
+    def __init__(self, a, b, k, d, t, pa, pb):
+        self.a, self.b, self.step, self.distance, self.time = a, b, k, d, t
+        self.pos_a, self.pos_b = pa, pb
+        self._index = {(int(x), int(y)): q for q, (x, y) in enumerate(zip(a, b))}
+
+    def __len__(self):
+        return len(self.a)
+
+    def get(self, i, j):
+        if i == j:
+            return dict(distance=0.0, time=0.0, pos_a=np.zeros(3), pos_b=np.zeros(3))
+        swap = i > j
+        q = self._index.get((min(i, j), max(i, j)))
+        if q is None:
+            return dict(distance=-1.0, time=-1.0, pos_a=-np.ones(3), pos_b=-np.ones(3))
+        pa, pb = self.pos_a[q], self.pos_b[q]
+        # positionA belongs to the first argument (the reference stores (A, B) of the pair)
+        return dict(distance=self.distance[q], time=self.time[q],
+                    pos_a=pb if swap else pa, pos_b=pa if swap else pb)

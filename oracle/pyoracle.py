@@ -28,6 +28,11 @@ def _ptr(a):
     return a.ctypes.data if a is not None else None
 
 
+# The C library holds one dataset at a time; every Oracle re-opens its own degree before a call
+# that depends on it.
+_loaded = {"key": None}
+
+
 class Oracle:
     def __init__(self, degree=-1, data_dir=DATA_DIR, lib_path=LIB_PATH):
         if not os.path.exists(lib_path):
@@ -79,17 +84,26 @@ class Oracle:
         if self.L.orc_open(data_dir.encode(), degree) != 0:
             raise RuntimeError(self.L.orc_error().decode())
         self.loaded = True
+        self._key = (data_dir, degree)
+        _loaded["key"] = self._key
+
+    def _ensure(self):
+        if self.loaded and _loaded["key"] != self._key:
+            self.open(self._key[1], self._key[0])
 
     @property
     def degree(self):
+        self._ensure()
         return self.L.orc_degree()
 
     @property
     def gm(self):
+        self._ensure()
         return self.L.orc_gm()
 
     @property
     def radius(self):
+        self._ensure()
         return self.L.orc_radius()
 
     # -- frames --------------------------------------------------------------------------
@@ -110,6 +124,7 @@ class Oracle:
         return o
 
     def nutation(self, t):
+        self._ensure()
         o = np.zeros(2)
         self.L.orc_nutation(t, _ptr(o))
         return o
@@ -124,14 +139,17 @@ class Oracle:
         return M
 
     def eop(self, mjd):
+        self._ensure()
         o = np.zeros(4)
         self.L.orc_eop(mjd, _ptr(o))
         return o
 
     def leap_seconds(self, mjd):
+        self._ensure()
         return self.L.orc_leap_seconds(mjd)
 
     def time_nodes(self, epoch, shifted_sec, offsets):
+        self._ensure()
         ymdhm = np.array(epoch[:5], dtype=np.int32)
         offsets = np.ascontiguousarray(offsets, dtype=np.float64)
         M = np.zeros((len(offsets), 9))
@@ -143,6 +161,7 @@ class Oracle:
 
     # -- gravity -------------------------------------------------------------------------
     def gravity_itrs(self, r):
+        self._ensure()
         r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 3)
         a = np.zeros_like(r)
         if self.L.orc_gravity_itrs(_ptr(r), len(r), _ptr(a)) != 0:
@@ -150,6 +169,7 @@ class Oracle:
         return a
 
     def potential_noncentral(self, r):
+        self._ensure()
         r = np.ascontiguousarray(r, dtype=np.float64)
         return self.L.orc_potential_itrs(_ptr(r))
 
@@ -173,6 +193,7 @@ class Oracle:
         return r, v
 
     def startup_piece(self, mode, kind, h, pos27, vel27):
+        self._ensure()
         pos = np.array(pos27, dtype=np.float64)
         vel = np.array(vel27, dtype=np.float64)
         acc, s, S = np.zeros(27), np.zeros(27), np.zeros(27)
@@ -182,6 +203,7 @@ class Oracle:
         return dict(pos=pos, vel=vel, acc=acc, s=s, S=S)
 
     def rk8_nodes(self, kind, h, r0, v0, quirks=REFERENCE_QUIRKS, epoch=(2025, 1, 1, 0, 0, 0.0)):
+        self._ensure()
         ymdhm = np.array(epoch[:5], dtype=np.int32)
         r0 = np.ascontiguousarray(r0, dtype=np.float64)
         v0 = np.ascontiguousarray(v0, dtype=np.float64)
@@ -193,6 +215,7 @@ class Oracle:
 
     def startup(self, kind, h, r0, v0, n_corr=7, quirks=REFERENCE_QUIRKS,
                 epoch=(2025, 1, 1, 0, 0, 0.0)):
+        self._ensure()
         ymdhm = np.array(epoch[:5], dtype=np.int32)
         r0 = np.ascontiguousarray(r0, dtype=np.float64)
         v0 = np.ascontiguousarray(v0, dtype=np.float64)
@@ -204,6 +227,7 @@ class Oracle:
 
     def propagate(self, kind, r0, v0, h, steps, n_corr=7, quirks=REFERENCE_QUIRKS,
                   epoch=(2025, 1, 1, 0, 0, 0.0), trajectory=False, threads=0):
+        self._ensure()
         r0 = np.ascontiguousarray(r0, dtype=np.float64).reshape(-1, 3)
         v0 = np.ascontiguousarray(v0, dtype=np.float64).reshape(-1, 3)
         n = len(r0)

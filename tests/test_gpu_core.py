@@ -1,0 +1,223 @@
+"""GPU parity of the force model, the frames and the integrator against the FP64 oracle.
+
+Bars: gravity within 1e-13 relative of the oracle (and the reference's golden vectors at 1e-10);
+time-node matrices within 1e-13; integrator known answers at the reference's tolerances;
+propagation final state within 1 mm / 1e-6 m/s of the oracle.
+"""
+import numpy as np
+import pytest
+
+import opengpop_amd as ogp
+from opengpop_amd import scenarios
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx70():
+    with ogp.Context(degree=70) as c:
+        yield c
+
+
+@pytest.fixture(scope="module")
+def ctx20():
+    with ogp.Context(degree=20) as c:
+        yield c
+
+
+def _rand_points(n, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return d * rng.uniform(6.5e6, 4.3e7, (n, 1))
+
+
+def test_gravity_golden_vectors(ctx70, golden):
+    g = golden["gravity_70x70"]
+    a = ctx70.gravity_itrs(np.array(g["positions"]).reshape(10, 3))
+    assert np.abs(a - np.array(g["accelerations"]).reshape(10, 3)).max() < g["tolerance"]
+
+
+def test_gravity_matches_oracle(ctx70, oracle70):
+    r = _rand_points(4096, 1)
+    # include the poles and the equator exactly
+    r[:4] = [[0, 0, 7.0e6], [0, 0, -7.2e6], [7.1e6, 0, 0], [0, -6.9e6, 0]]
+    a = ctx70.gravity_itrs(r)
+    ref = oracle70.gravity_itrs(r)
+    rel = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert rel.max() < 1e-13
+
+
+def test_gravity_degree_truncation_matches_oracle(ctx20, po):
+    o = po.Oracle(degree=20)
+    r = _rand_points(512, 2)
+    rel = np.linalg.norm(ctx20.gravity_itrs(r) - o.gravity_itrs(r), axis=1) / np.linalg.norm(
+        o.gravity_itrs(r), axis=1)
+    assert rel.max() < 1e-13
+
+
+def test_split_precision_gravity_error_bound(oracle70):
+    with ogp.Context(degree=70, precision="split") as c:
+        r = _rand_points(4096, 3)
+        a = c.gravity_itrs(r)
+    ref = oracle70.gravity_itrs(r)
+    # degree >= 3 terms in FP32 with FP64 column sums: the error stays at the 1e-10 m/s^2
+    # level (about 1e-11 relative) even at 130 km altitude, where degree-70 terms are largest
+    err = np.linalg.norm(a - ref, axis=1)
+    assert np.all(np.isfinite(a))
+    assert err.max() < 5e-10
+    assert np.median(err) < 1e-12
+
+
+def test_split_precision_position_error_two_days():
+    """Split precision against the plain FP64 GPU path: 2 days of LEO at 60 s."""
+    r0, v0 = scenarios.leo(256, seed=2027)
+    with ogp.Context(degree=70) as c:
+        r64, _ = c.propagate(r0, v0, 60.0, 2880)
+    with ogp.Context(degree=70, precision="split") as c:
+        r32, _ = c.propagate(r0, v0, 60.0, 2880)
+    e = np.linalg.norm(r32 - r64, axis=1)
+    assert e.max() < 0.25
+
+
+def test_time_nodes_match_oracle(ctx70, oracle70):
+    offs = np.concatenate([np.arange(0, 86400.0 * 3, 977.0), [1e-3, 59.999]])
+    M, mjd = ctx70.time_nodes(60.0, 4, offs)
+    Mo, mjdo = oracle70.time_nodes(scenarios.EPOCH, 0.0 - 4 * 60.0, offs)
+    assert np.abs(mjd - mjdo).max() == 0.0
+    assert np.abs(M - Mo).max() < 1e-13
+
+
+def test_predictor_and_corrector_kat(ctx70, golden):
+    k = golden["predictor_kat"]
+    r, v, _ = ctx70.kat_predict(k["acc"], k["s"], k["S"], k["h"])
+    assert np.allclose(r, k["r"], rtol=1e-7, atol=0)
+    assert np.allclose(v, k["v"], rtol=2e-15, atol=0)
+    k = golden["corrector_kat"]
+    r, v = ctx70.kat_finalize_correction(k["r"], k["v"], k["acc"], k["s"], k["S"], k["h"])
+    assert np.allclose(r, k["r_out"], rtol=1e-7, atol=0)
+    assert np.allclose(v, k["v_out"], rtol=1e-7, atol=0)
+
+
+def _exact7(t):
+    return t ** 7 - t ** 3
+
+
+def _exact8(t):
+    return t ** 8 - t ** 7 + 2 * t ** 6 - 5 * t ** 5 - 10 * t ** 4 + t ** 3 - t ** 2
+
+
+@pytest.mark.parametrize("force,exact,tol", [(ogp.FORCE_POLY7, _exact7, 1e-13),
+                                             (ogp.FORCE_POLY8, _exact8, 2e-13)])
+def test_gj8_exact_for_polynomials(force, exact, tol):
+    with ogp.Context(force=force, n_corrections=1) as c:
+        err = 0.0
+        for n in range(1, 21):
+            r, _ = c.propagate([0, 0, 0], [0, 0, 0], 0.1, n)
+            err = max(err, np.linalg.norm(r[0] - exact(n * 0.1)))
+    assert err < tol
+
+
+@pytest.mark.parametrize("force,exact,tol", [(ogp.FORCE_POLY7, _exact7, 1e-10),
+                                             (ogp.FORCE_POLY8, _exact8, 1e-13)])
+def test_startup_exact_for_polynomials(force, exact, tol):
+    with ogp.Context(force=force, n_corrections=5) as c:
+        st = c.startup([[0, 0, 0]], [[0, 0, 0]], 1.0)
+    e = np.array([exact(i) for i in range(-4, 5)])
+    assert np.all(np.abs(st["pos"][0, :, 0] - e) <= np.maximum(tol, 2 * np.spacing(np.abs(e))))
+
+
+def test_stationary_and_constant_speed():
+    with ogp.Context(force=ogp.FORCE_NONE) as c:
+        p = ogp.GpuPropagator(c, [[0, 0, 0], [1, 1, 1]], [[1, 1, 1], [1, 1, 1]], 1.0, 10)
+        for n in range(1, 11):
+            r, v, _ = p.step()
+            assert np.allclose(r, np.array([[0, 0, 0], [1, 1, 1]]) + n, rtol=4 * 2.2e-16, atol=4e-16)
+            assert np.allclose(v, 1.0, rtol=4 * 2.2e-16, atol=0)
+        r, v = c.propagate([[0, 0, 0], [1, 1, 1]], [[0, 0, 0], [0, 0, 0]], 1.0, 10)
+        assert np.array_equal(r, [[0, 0, 0], [1, 1, 1]])
+
+
+def test_gj8_error_decreases_with_corrections():
+    def damped(t):
+        return np.exp(-t) * (np.sin(t) + np.cos(t))
+    prev = np.inf
+    for nc in range(1, 6):
+        with ogp.Context(force=ogp.FORCE_DAMPED, n_corrections=nc) as c:
+            p = ogp.GpuPropagator(c, [1, 1, 1], [0, 0, 0], 0.1, 40)
+            err = 0.0
+            for n in range(1, 41):
+                r, _, _ = p.step()
+                err = max(err, np.linalg.norm(r[0] - damped(n * 0.1)))
+        assert err < prev
+        prev = err
+
+
+def test_startup_matches_oracle_with_gravity(ctx70, oracle70):
+    d = scenarios.config0()
+    st = ctx70.startup(d["r"][:8], d["v"][:8], 60.0)
+    for i in range(8):
+        o = oracle70.startup(0, 60.0, d["r"][i], d["v"][i])
+        assert np.abs(st["pos"][i] - o["pos"]).max() < 1e-6
+        assert np.abs(st["vel"][i] - o["vel"]).max() < 1e-9
+        assert np.abs(st["acc"][i] - o["acc"]).max() < 1e-12
+
+
+def test_config0_matches_oracle(ctx20, po):
+    """64 LEO objects, degree 20, 60 s, 6 h: final state within 1 mm and 1e-6 m/s."""
+    d = scenarios.config0()
+    r, v = ctx20.propagate(d["r"], d["v"], d["step"], d["steps"])
+    o = po.Oracle(degree=20)
+    ro, vo = o.propagate(0, d["r"], d["v"], d["step"], d["steps"])
+    assert np.abs(r - ro).max() < 1e-3
+    assert np.abs(v - vo).max() < 1e-6
+
+
+def test_24h_degree70_matches_oracle(ctx70, oracle70):
+    r0, v0 = scenarios.leo(6, seed=11)
+    r, v = ctx70.propagate(r0, v0, 60.0, 1440)
+    ro, vo = oracle70.propagate(0, r0, v0, 60.0, 1440)
+    assert np.abs(r - ro).max() < 1e-3
+    assert np.abs(v - vo).max() < 1e-6
+
+
+def test_mixed_steps_and_uneven_spans_match_oracle(ctx20, po):
+    d = scenarios.mixed(40, seed=5)
+    steps = np.minimum(d["steps"], 300)  # keep the oracle run short
+    steps[:3] = [0, 2, 4]                # spans that end inside the startup
+    r, v = ctx20.propagate(d["r"], d["v"], d["step"], steps)
+    o = po.Oracle(degree=20)
+    ro, vo = o.propagate(0, d["r"], d["v"], d["step"], steps)
+    assert np.abs(r - ro).max() < 1e-3
+    assert np.abs(v - vo).max() < 1e-6
+    assert np.array_equal(r[0], d["r"][0])
+
+
+def test_stepwise_equals_bulk(ctx20):
+    d = scenarios.config0()
+    p = ogp.GpuPropagator(ctx20, d["r"][:16], d["v"][:16], 60.0, 30)
+    for _ in range(30):
+        rs, vs, _ = p.step()
+    rb, vb = ctx20.propagate(d["r"][:16], d["v"][:16], 60.0, 30)
+    assert np.array_equal(rs, rb) and np.array_equal(vs, vb)
+
+
+def test_identical_objects_bit_identical(ctx70):
+    r0 = np.tile([-3884872.0, -5749111.0, 14723.0], (8, 1))
+    v0 = np.tile([-795.304, 573.772, 7518.179], (8, 1))
+    r, v = ctx70.propagate(r0, v0, 30.0, 10)
+    assert all(np.array_equal(r[0], r[i]) and np.array_equal(v[0], v[i]) for i in range(8))
+
+
+def test_errors_are_reported_not_fatal(ctx20):
+    with pytest.raises(ogp.OgpError):
+        ctx20.propagate([[7e6, 0, 0]], [[0, 7.5e3, 0]], -1.0, 10)
+    with pytest.raises(ogp.OgpError):
+        ctx20.propagate([[np.nan, 0, 0]], [[0, 7.5e3, 0]], 60.0, 10)
+    with pytest.raises(ogp.OgpError):
+        ogp.Context(degree=71)
+    with pytest.raises(ogp.OgpError):
+        ogp.Context(data_dir="/nonexistent")
+    # the context is still usable afterwards
+    r, _ = ctx20.propagate([[7e6, 0, 0]], [[0, 7.5e3, 0]], 60.0, 5)
+    assert np.all(np.isfinite(r))
