@@ -123,6 +123,16 @@ typedef struct ogp_timing {
 } ogp_timing;
 int ogp_last_timing(const ogp_context* ctx, ogp_timing* t);
 
+/* Launch shape of the propagation kernels (no counterpart in the reference, which launches
+ * 256-thread blocks per helper kernel): `waves` = waves per group of 64 objects that share one
+ * object group's harmonic columns (1, 2, 4, 8; 0 = autotuned), chunk_steps = GJ8 steps per
+ * launch (0 = 64). Results are bit-identical for every choice. With waves = 0 the context times
+ * four chunks, one per candidate, the first time it meets a group count (per power of two) and
+ * keeps the fastest; ogp_set_tuning clears that table. ogp_get_tuning reports the choice for
+ * n objects. */
+int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps);
+int ogp_get_tuning(const ogp_context* ctx, int64_t n, int* waves, int64_t* chunk_steps);
+
 /* Stepwise GpuPropagator (gpu_propagator.cuh:50-58): init with n objects and one step size;
  * each ogp_prop_step performs one GpuPropagator::Step (startup on the first call, startup
  * nodes for steps 1-4, GJ8 from step 5) and optionally returns r, v and the acceleration

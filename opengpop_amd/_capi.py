@@ -75,6 +75,9 @@ def lib():
         "ogp_propagate": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P]),
         "ogp_propagate_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P]),
         "ogp_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(OgpTiming)]),
+        "ogp_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, _I64]),
+        "ogp_get_tuning": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(_I64)]),
         "ogp_prop_init": (ctypes.c_int, [_P, _P, _P, _I64, _D, _I64]),
         "ogp_prop_step": (ctypes.c_int, [_P, _P, _P, _P]),
         "ogp_startup": (ctypes.c_int, [_P, _P, _P, _I64, _D, _P, _P, _P, _P, _P]),

@@ -21,7 +21,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(ROOT, "include")]
 
-SOURCES = ["datasets.cpp", "kernels.hip", "conjunction.hip", "adjoint.hip", "capi.cpp"]
+SOURCES = ["datasets.cpp", "kernels.hip", "propagate.hip", "conjunction.hip", "adjoint.hip",
+           "capi.cpp"]
 
 
 def _run(cmd):

@@ -108,13 +108,13 @@ struct ogp_context {
 
     GravityModel grav;
     FrameData frames;
-    DevBuf d_rec, d_recf, d_diag, d_eop, d_leap_mjd, d_leap_sec, d_nut[4];
+    DevBuf d_rec, d_recf, d_diag, d_parts, d_eop, d_leap_mjd, d_leap_sec, d_nut[4];
     GravityDev gdev {};
     FrameDev fdev {};
     int kind = kGravity;  // kernel variant of the force
     int rk8_variant = 0;  // 0 = reference row 7
-    int block = 64;       // objects per block of the propagation kernels
-    int64_t chunk = 64;   // GJ8 steps per launch
+    int waves_override = 0; // waves per 64-object group; 0 = chosen from the object count
+    int64_t chunk = 64;     // GJ8 steps per launch
 
     // node tables keyed by (h, t0)
     std::map<std::pair<double, double>, std::unique_ptr<NodeTable>> nodes;
@@ -148,6 +148,47 @@ struct ogp_context {
 
     void sync() { HIPC(hipStreamSynchronize(stream)); }
     void check_launch() { HIPC(hipGetLastError()); }
+
+    // Autotuned waves per 64-object group, keyed by floor(log2(number of groups)). Results do
+    // not depend on the choice (the harmonic parts are summed in a fixed order), so the tuner
+    // may switch between chunks of one propagation.
+    std::map<int, int> tuned_waves;
+
+    static int group_bucket(int64_t n)
+    {
+        int64_t groups = (n + 63) / 64;
+        int b = 0;
+        while (groups > 1) {
+            groups >>= 1;
+            ++b;
+        }
+        return b;
+    }
+
+    // Waves per 64-object group: the override, else the tuned value for this size, else enough
+    // waves for about 1200 resident on the chip's 1024 SIMDs, which hides most of the latency of
+    // the scalar coefficient loads.
+    int waves_for(int64_t n) const
+    {
+        if (kind != kGravity && kind != kGravitySplit)
+            return 1;
+        if (waves_override > 0)
+            return waves_override;
+        auto it = tuned_waves.find(group_bucket(n));
+        if (it != tuned_waves.end())
+            return it->second;
+        const int64_t groups = (n + 63) / 64;
+        int w = 1;
+        while (w < 8 && groups * w < 1200)
+            w *= 2;
+        return w;
+    }
+
+    bool needs_tuning(int64_t n) const
+    {
+        return (kind == kGravity || kind == kGravitySplit) && waves_override == 0 && n > 0
+            && tuned_waves.find(group_bucket(n)) == tuned_waves.end();
+    }
 
     ForceCtx force_ctx(const NodeTable* nt, double h) const
     {
@@ -265,9 +306,11 @@ struct ogp_context {
     };
 
     // Startup and GJ8 steps for all groups of the internal state (class-sorted order).
-    // traj (optional) receives steps 1..traj_steps of the single group.
+    // traj (optional) receives steps 1..traj_steps of the single group. ns (optional, host,
+    // class-sorted order, descending within a group) trims every chunk launch to the prefix of
+    // objects that still have steps in it.
     void run_groups(const std::vector<Group>& groups, int64_t N, double t0, double* traj,
-        int64_t traj_steps)
+        int64_t traj_steps, const int64_t* ns = nullptr)
     {
         Event e0, e1, e2, e3;
         const PropState ps = prop_state(N);
@@ -279,20 +322,64 @@ struct ogp_context {
         HIPC(hipEventRecord(e1.e, stream));
         for (size_t q = 0; q < groups.size(); ++q) {
             const Group& g = groups[q];
+            const int W = waves_for(g.count);
             launch_startup(stream, kind, force_ctx(tabs[q], g.h), rk8_variant, ps, scratch(),
-                g.first, g.count, cfg.n_corrections, traj, traj_steps, block);
+                g.first, g.count, cfg.n_corrections, traj, traj_steps, W);
             check_launch();
         }
         HIPC(hipEventRecord(e2.e, stream));
-        double launches = 0, gj_steps = 0;
+        double launches = 0;
         for (size_t q = 0; q < groups.size(); ++q) {
             const Group& g = groups[q];
-            for (int64_t k0 = 5; k0 <= g.max_steps; k0 += chunk) {
+            const ForceCtx f = force_ctx(tabs[q], g.h);
+            auto active_at = [&](int64_t k0) {
+                if (!ns)
+                    return g.count;
+                const int64_t* b = ns + g.first;
+                return (int64_t)(std::partition_point(b, b + g.count,
+                                     [k0](int64_t s) { return s >= k0; }) - b);
+            };
+            auto launch = [&](int64_t k0, int64_t active, int W) {
                 const int64_t k1 = std::min<int64_t>(k0 + chunk, g.max_steps + 1);
-                launch_chunk(stream, kind, force_ctx(tabs[q], g.h), ps, g.first, g.count, k0, k1,
-                    cfg.n_corrections, traj, 1, traj_steps, block);
+                launch_chunk(stream, kind, f, ps, g.first, active, k0, k1, cfg.n_corrections,
+                    traj, 1, traj_steps, W);
                 check_launch();
                 launches += 1;
+                return k1 - k0;
+            };
+            static const int kTrial[4] = { 1, 2, 4, 8 };
+            int64_t k0 = 5;
+            while (k0 <= g.max_steps) {
+                const int64_t active = active_at(k0);
+                if (needs_tuning(active) && g.max_steps - k0 + 1 >= 4 * chunk) {
+                    // time the next four chunks, one per candidate, and keep the fastest per
+                    // object-step; the chunks do real work (results do not depend on W)
+                    Event t[5];
+                    double work[4];
+                    HIPC(hipEventRecord(t[0].e, stream));
+                    for (int c = 0; c < 4; ++c) {
+                        const int64_t a = active_at(k0);
+                        const int64_t steps = launch(k0, a, kTrial[c]);
+                        work[c] = (double)std::max<int64_t>(a, 1) * (double)steps;
+                        k0 += steps;
+                        HIPC(hipEventRecord(t[c + 1].e, stream));
+                    }
+                    sync();
+                    int best = 0;
+                    double best_rate = 0.0;
+                    for (int c = 0; c < 4; ++c) {
+                        float ms = 0;
+                        HIPC(hipEventElapsedTime(&ms, t[c].e, t[c + 1].e));
+                        const double rate = work[c] / std::max(1e-3, (double)ms);
+                        if (rate > best_rate) {
+                            best_rate = rate;
+                            best = kTrial[c];
+                        }
+                    }
+                    tuned_waves[group_bucket(active)] = best;
+                    continue;
+                }
+                k0 += launch(k0, active, waves_for(active));
             }
         }
         HIPC(hipEventRecord(e3.e, stream));
@@ -306,7 +393,6 @@ struct ogp_context {
         timing.steps_ms = ms2;
         timing.total_ms = (double)ms0 + ms1 + ms2;
         timing.step_launches = launches;
-        (void)gj_steps;
     }
 };
 
@@ -408,6 +494,10 @@ int ogp_create(const ogp_config* cfg, ogp_context** out)
             c.gdev.rec = c.d_rec.as<double>();
             c.gdev.recf = c.d_recf.as<float>();
             c.gdev.diag = c.d_diag.as<double>();
+            std::vector<int> parts(kParts + 1);
+            column_parts(c.grav.L, parts.data());
+            upload(&c, c.d_parts, parts);
+            c.gdev.parts = c.d_parts.as<int>();
             c.gdev.L = c.grav.L;
             c.gdev.nsplit = cfg->precision == OGP_PRECISION_SPLIT ? std::max(cfg->split_degree, 1)
                                                                   : c.grav.L;
@@ -624,7 +714,7 @@ int ogp_propagate(ogp_context* ctx, const double* r0, const double* v0, int64_t 
             ctx->st_v.as<double>(), n);
         ctx->check_launch();
         HIPC(hipEventRecord(h1.e, ctx->stream));
-        ctx->run_groups(groups, n, 0.0, nullptr, 0);
+        ctx->run_groups(groups, n, 0.0, nullptr, 0, ns.data());
         HIPC(hipEventRecord(d0.e, ctx->stream));
         launch_permute(ctx->stream, 1, ctx->st_r.as<double>(), ctx->perm.as<int64_t>(),
             ctx->io_a.as<double>(), n);
@@ -675,7 +765,7 @@ int ogp_propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v
         launch_permute(ctx->stream, 2, d_r0, ctx->perm.as<int64_t>(), ctx->st_r.as<double>(), n);
         launch_permute(ctx->stream, 2, d_v0, ctx->perm.as<int64_t>(), ctx->st_v.as<double>(), n);
         ctx->check_launch();
-        ctx->run_groups(groups, n, 0.0, nullptr, 0);
+        ctx->run_groups(groups, n, 0.0, nullptr, 0, ns.data());
         launch_permute(ctx->stream, 3, ctx->st_r.as<double>(), ctx->perm.as<int64_t>(), d_r, n);
         launch_permute(ctx->stream, 3, ctx->st_v.as<double>(), ctx->perm.as<int64_t>(), d_v, n);
         ctx->check_launch();
@@ -689,6 +779,29 @@ int ogp_propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v
         ctx->timing.gj_object_steps = gj;
         ctx->timing.h2d_ms = ctx->timing.d2h_ms = 0;
     });
+}
+
+int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    if (!(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8) || chunk_steps < 0)
+        return fail(ctx, OGP_ERR_ARGUMENT, "waves must be 0, 1, 2, 4 or 8; chunk_steps >= 0");
+    ctx->waves_override = waves;
+    ctx->chunk = chunk_steps > 0 ? chunk_steps : 64;
+    ctx->tuned_waves.clear();
+    return OGP_OK;
+}
+
+int ogp_get_tuning(const ogp_context* ctx, int64_t n, int* waves, int64_t* chunk_steps)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    if (waves)
+        *waves = ctx->waves_for(n);
+    if (chunk_steps)
+        *chunk_steps = ctx->chunk;
+    return OGP_OK;
 }
 
 int ogp_last_timing(const ogp_context* ctx, ogp_timing* t)
@@ -745,12 +858,13 @@ int ogp_prop_step(ogp_context* ctx, double* r_out, double* v_out, double* a_out)
         if (k > ctx->sw_max)
             throw ApiError(OGP_ERR_STATE, "step beyond max_steps of ogp_prop_init");
         NodeTable* nt = ctx->node_table(ctx->sw_h, ctx->sw_max);
+        const int W = ctx->waves_for(n);
         const ForceCtx f = ctx->force_ctx(nt, ctx->sw_h);
         const PropState ps = ctx->prop_state(n);
         double* traj = ctx->sw_traj.as<double>();
         if (k == 1) {
             launch_startup(ctx->stream, ctx->kind, f, ctx->rk8_variant, ps, ctx->scratch(), 0, n,
-                ctx->cfg.n_corrections, traj, 4, ctx->block);
+                ctx->cfg.n_corrections, traj, 4, W);
             ctx->check_launch();
         }
         const double* row;
@@ -758,7 +872,7 @@ int ogp_prop_step(ogp_context* ctx, double* r_out, double* v_out, double* a_out)
             row = traj + (k - 1) * 9 * n;
         } else {
             launch_chunk(ctx->stream, ctx->kind, f, ps, 0, n, k, k + 1, ctx->cfg.n_corrections,
-                traj, k, 1, ctx->block);
+                traj, k, 1, W);
             ctx->check_launch();
             row = traj;
         }
@@ -800,9 +914,9 @@ int ogp_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
         HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
             ctx->stream));
         NodeTable* nt = ctx->node_table(step_s, 4);
+        const int W = ctx->waves_for(n);
         launch_startup(ctx->stream, ctx->kind, ctx->force_ctx(nt, step_s), ctx->rk8_variant,
-            ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0,
-            ctx->block);
+            ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0, W);
         ctx->check_launch();
         ctx->sync();
         // scratch [slot][c][obj] -> caller [obj][slot][c]

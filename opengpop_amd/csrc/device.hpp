@@ -4,33 +4,39 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-namespace ogp {
+#include "coefficients.hpp"
 
-// Integrator tables (Berry & Healy 2004 ordinate form, 8th order), same values as
-// lib/common/gpu_constants/gpu_constants/gpu_constants.cuh:22-98. Row 9 is the predictor,
-// rows 0-8 the mid-corrector rows j = -4..4. Stored [row][k].
-extern __constant__ double c_gj[10][9];
-extern __constant__ double c_sa[10][9];
-// RK8 (10 stages): a[i][j] for stage i+1, b, nodes c. a[6][0] is selected per context (quirk).
-extern __constant__ double c_rk8b[10];
-extern __constant__ double c_rk8c[10];
+namespace ogp {
 
 // Wave-uniform reads: pointers in the constant address space make the compiler emit scalar
 // (SMEM) loads, so a value needed by all 64 lanes travels once per wave through the scalar
 // cache instead of once per lane through LDS or the vector memory path.
 typedef const __attribute__((address_space(4))) double* uniform_dptr;
 typedef const __attribute__((address_space(4))) float* uniform_fptr;
+typedef const __attribute__((address_space(4))) int* uniform_iptr;
 
 // ------------------------------------------------------------------------------------------
 // Gravity field (Gottlieb 1993, normalized derived Legendre functions)
 // ------------------------------------------------------------------------------------------
+constexpr int kMaxWaves = 8; // waves sharing one object group (column split)
+// The columns m = 0..L form kParts contiguous parts of about equal cost. Each part is summed
+// from zero and the part sums are added in part order, whatever the number of waves that share
+// the work, so the acceleration is bit-identical for every launch shape.
+constexpr int kParts = 8;
+
 struct GravityDev {
     const double* rec;  // 8 doubles per record, see datasets.hpp (GravityModel::records)
     const float* recf;  // the same records in FP32
     const double* diag; // Pbar_mm / Pbar_{m-1,m-1}
+    const int* parts;   // kParts + 1 column bounds: part q = columns [parts[q], parts[q + 1])
     int L;              // degree and order
     int nsplit;         // split precision: degrees <= nsplit in FP64, above in FP32
     double gm, radius;
+};
+
+// Partial harmonic sums of one column range (gravity_source_device.cu:122-210 restated).
+struct GravSums {
+    double G, H, J, K;
 };
 
 // Fused multiply-add in the precision of the operands (no promotion of FP32 to FP64).
@@ -56,12 +62,31 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
         q += 8;
         ++k;
     }
-#pragma unroll 2
+    // Records are fetched BATCH at a time into scalar registers and then consumed: one wait for
+    // the scalar loads per batch instead of one per record.
+    constexpr int BATCH = 4;
+    for (; k + BATCH <= k1; k += BATCH, q += 8 * BATCH) {
+        T rec[BATCH][8];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b)
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                rec[b][e] = q[8 * b + e];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const T Pn = fmat(rec[b][0] * er, P, -(rec[b][1] * rr2) * Pp);
+            Pp = P;
+            P = Pn;
+            A1 = fmat(P, rec[b][2], A1);
+            A2 = fmat(P, rec[b][3], A2);
+            B1 = fmat(P, rec[b][4], B1);
+            B2 = fmat(P, rec[b][5], B2);
+            E1 = fmat(P, rec[b][6], E1);
+            E2 = fmat(P, rec[b][7], E2);
+        }
+    }
     for (; k < k1; ++k, q += 8) {
-        const T xi = q[0], eta = q[1];
-        const T a = xi * er;
-        const T b = eta * rr2;
-        const T Pn = fmat(a, P, -b * Pp);
+        const T Pn = fmat(q[0] * er, P, -(q[1] * rr2) * Pp);
         Pp = P;
         P = Pn;
         A1 = fmat(P, q[2], A1);
@@ -84,107 +109,209 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
 //  * the H term pairing (n, m-1) with Pbar_n^m is accumulated while column m is visited.
 // SPLIT: degrees n <= nsplit in FP64, n > nsplit in FP32; the per-column partial sums are
 // promoted to FP64 before they meet the longitude terms.
-template <bool SPLIT>
-__device__ __forceinline__ void gravity_itrs(
-    double x, double y, double z, const GravityDev& g, double& ax, double& ay, double& az)
+// Geometry shared by all columns of one evaluation point.
+struct GravPoint {
+    double ir, eps, rho, er, rr2, cphi, U1, W1;
+};
+
+__device__ __forceinline__ GravPoint grav_point(double x, double y, double z, double radius)
 {
+    GravPoint p;
     const double pxy2 = __builtin_fma(y, y, x * x);
     const double r2 = __builtin_fma(z, z, pxy2);
     const double r = __builtin_sqrt(r2);
-    const double ir = 1.0 / r;
-    const double eps = z * ir;
-    const double rho = g.radius * ir;
-    const double er = eps * rho;
-    const double rr2 = rho * rho;
+    p.ir = 1.0 / r;
+    p.eps = z * p.ir;
+    p.rho = radius * p.ir;
+    p.er = p.eps * p.rho;
+    p.rr2 = p.rho * p.rho;
     const double pxy = __builtin_sqrt(pxy2);
-    const double cphi = pxy * ir;
-    double U1 = 1.0, W1 = 0.0;
+    p.cphi = pxy * p.ir;
+    p.U1 = 1.0;
+    p.W1 = 0.0;
     if (pxy > 0.0) {
         const double ip = 1.0 / pxy;
-        U1 = x * ip;
-        W1 = y * ip;
+        p.U1 = x * ip;
+        p.W1 = y * ip;
     }
-    const float erf_ = (float)er, rr2f = (float)rr2;
+    return p;
+}
 
-    double G = 0.0, H = 0.0, J = 0.0, K = 0.0;
+// gravity_source_device.cu:219-238 on the complete sums.
+__device__ __forceinline__ void grav_assemble(double x, double y, double z, const GravPoint& p,
+    const GravSums& s, double gm, double& ax, double& ay, double& az)
+{
+    const double Lam = (s.G + 1.0) + z * s.H * p.ir;
+    const double coef = -gm * p.ir * p.ir;
+    ax = coef * __builtin_fma(Lam * x, p.ir, -s.J);
+    ay = coef * __builtin_fma(Lam * y, p.ir, -s.K);
+    az = coef * __builtin_fma(Lam * z, p.ir, -s.H);
+}
+
+// Sums of parts [q0, q1) (q0 < q1, wave-uniform); sink(q, sums) receives the sums of part q, in
+// part order. The longitude terms and the scaled diagonal of the first column are reached by
+// running their recurrences from m = 0 (a few operations per skipped column).
+template <bool SPLIT, class Sink>
+__device__ __forceinline__ void grav_parts(const GravPoint& p, const GravityDev& g, int q0,
+    int q1, Sink&& sink)
+{
+    const double er = p.er, rr2 = p.rr2, cphi = p.cphi, rho = p.rho, U1 = p.U1, W1 = p.W1;
+    const float erf_ = (float)er, rr2f = (float)rr2;
     double Um = 1.0, Wm = 0.0;   // cos(m lambda), sin(m lambda)
     double Um1 = 1.0, Wm1 = 0.0; // (m - 1)
     double diagP = 1.0;          // scaled diagonal cos(phi)^(m-1) (R/r)^m Pbar_mm
     const uniform_dptr diag = (uniform_dptr)g.diag;
+    const uniform_iptr parts = (uniform_iptr)g.parts;
     const int L = g.L;
+    const int m0 = parts[q0];
     int kcol = 0; // first record of column m
-    for (int m = 0; m <= L; ++m) {
+    for (int m = 0; m < m0; ++m) {
         if (m == 1)
             diagP = rho * diag[1];
         else if (m >= 2)
             diagP *= cphi * rho * diag[m];
-        const int ncol = L - m + 1;
-        double A1, A2, B1, B2, E1, E2;
-        if (!SPLIT) {
-            double P = diagP, Pp = 0.0;
-            A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
-            column_terms<double>(g.rec, kcol, kcol + ncol, er, rr2, P, Pp, A1, A2, B1, B2, E1,
-                E2, true);
-        } else {
-            double P = diagP, Pp = 0.0;
-            A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
-            int k32 = kcol; // first FP32 record
-            if (m <= g.nsplit) {
-                const int n64 = (g.nsplit < L ? g.nsplit : L) - m + 1;
-                column_terms<double>(g.rec, kcol, kcol + n64, er, rr2, P, Pp, A1, A2, B1, B2,
-                    E1, E2, true);
-                k32 = kcol + n64;
-            }
-            if (k32 < kcol + ncol) {
-                float Pf = (float)P, Ppf = (float)Pp;
-                float a1 = 0.f, a2 = 0.f, b1 = 0.f, b2 = 0.f, e1 = 0.f, e2 = 0.f;
-                column_terms<float>(g.recf, k32, kcol + ncol, erf_, rr2f, Pf, Ppf, a1, a2, b1,
-                    b2, e1, e2, k32 == kcol);
-                A1 += (double)a1;
-                A2 += (double)a2;
-                B1 += (double)b1;
-                B2 += (double)b2;
-                E1 += (double)e1;
-                E2 += (double)e2;
-            }
-        }
-        if (m == 0) {
-            G += B1;
-        } else {
-            const double dm = (double)m;
-            G += cphi * __builtin_fma(Um, B1, Wm * B2);
-            J += dm * __builtin_fma(Um1, A1, Wm1 * A2);
-            K -= dm * __builtin_fma(Wm1, A1, -Um1 * A2);
-            H += __builtin_fma(Um1, E1, Wm1 * E2);
-        }
-        // advance the longitude terms: (U + iW)_{m+1} = (U + iW)_m (U1 + i W1)
         Um1 = Um;
         Wm1 = Wm;
         const double un = __builtin_fma(Um, U1, -Wm * W1);
         Wm = __builtin_fma(Wm, U1, Um * W1);
         Um = un;
-        kcol += ncol;
+        kcol += L - m + 1;
     }
-    // gravity_source_device.cu:219-238
-    const double Lam = (G + 1.0) + z * H * ir;
-    const double coef = -g.gm * ir * ir;
-    ax = coef * __builtin_fma(Lam * x, ir, -J);
-    ay = coef * __builtin_fma(Lam * y, ir, -K);
-    az = coef * __builtin_fma(Lam * z, ir, -H);
+    int m = m0;
+    for (int q = q0; q < q1; ++q) {
+        double G = 0.0, H = 0.0, J = 0.0, K = 0.0;
+        for (const int mend = parts[q + 1]; m < mend; ++m) {
+            if (m == 1)
+                diagP = rho * diag[1];
+            else if (m >= 2)
+                diagP *= cphi * rho * diag[m];
+            const int ncol = L - m + 1;
+            double A1, A2, B1, B2, E1, E2;
+            if (!SPLIT) {
+                double P = diagP, Pp = 0.0;
+                A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
+                column_terms<double>(g.rec, kcol, kcol + ncol, er, rr2, P, Pp, A1, A2, B1, B2, E1,
+                    E2, true);
+            } else {
+                double P = diagP, Pp = 0.0;
+                A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
+                int k32 = kcol; // first FP32 record
+                if (m <= g.nsplit) {
+                    const int n64 = (g.nsplit < L ? g.nsplit : L) - m + 1;
+                    column_terms<double>(g.rec, kcol, kcol + n64, er, rr2, P, Pp, A1, A2, B1, B2,
+                        E1, E2, true);
+                    k32 = kcol + n64;
+                }
+                if (k32 < kcol + ncol) {
+                    float Pf = (float)P, Ppf = (float)Pp;
+                    float a1 = 0.f, a2 = 0.f, b1 = 0.f, b2 = 0.f, e1 = 0.f, e2 = 0.f;
+                    column_terms<float>(g.recf, k32, kcol + ncol, erf_, rr2f, Pf, Ppf, a1, a2, b1,
+                        b2, e1, e2, k32 == kcol);
+                    A1 += (double)a1;
+                    A2 += (double)a2;
+                    B1 += (double)b1;
+                    B2 += (double)b2;
+                    E1 += (double)e1;
+                    E2 += (double)e2;
+                }
+            }
+            if (m == 0) {
+                G += B1;
+            } else {
+                const double dm = (double)m;
+                G += cphi * __builtin_fma(Um, B1, Wm * B2);
+                J += dm * __builtin_fma(Um1, A1, Wm1 * A2);
+                K -= dm * __builtin_fma(Wm1, A1, -Um1 * A2);
+                H += __builtin_fma(Um1, E1, Wm1 * E2);
+            }
+            // advance the longitude terms: (U + iW)_{m+1} = (U + iW)_m (U1 + i W1)
+            Um1 = Um;
+            Wm1 = Wm;
+            const double un = __builtin_fma(Um, U1, -Wm * W1);
+            Wm = __builtin_fma(Wm, U1, Um * W1);
+            Um = un;
+            kcol += ncol;
+        }
+        sink(q, GravSums { G, H, J, K });
+    }
+}
+
+// Adds the part sums in part order (the fold every launch shape performs).
+struct PartFold {
+    GravSums t { 0.0, 0.0, 0.0, 0.0 };
+    __device__ __forceinline__ void operator()(int, const GravSums& s)
+    {
+        t.G += s.G;
+        t.H += s.H;
+        t.J += s.J;
+        t.K += s.K;
+    }
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ void gravity_itrs(
+    double x, double y, double z, const GravityDev& g, double& ax, double& ay, double& az)
+{
+    const GravPoint p = grav_point(x, y, z, g.radius);
+    PartFold f;
+    grav_parts<SPLIT>(p, g, 0, kParts, f);
+    grav_assemble(x, y, z, p, f.t, g.gm, ax, ay, az);
+}
+
+// Column-split evaluation for a workgroup of W waves that all hold the same 64 objects: wave w
+// sums parts [w kParts / W, (w + 1) kParts / W), every part sum goes to its LDS slot, and every
+// wave adds the kParts slots in part order, which is the fold of the one-wave evaluation, so the
+// result does not depend on W. red holds kParts * 4 * 64 doubles; the first barrier keeps the
+// writes of this evaluation behind the reads of the previous one (the waves arrive together:
+// they run the same integrator arithmetic between evaluations).
+template <bool SPLIT, int W>
+__device__ __forceinline__ void gravity_itrs_split(double x, double y, double z,
+    const GravityDev& g, double* red, double& ax, double& ay, double& az)
+{
+    const GravPoint p = grav_point(x, y, z, g.radius);
+    if (W == 1) {
+        PartFold f;
+        grav_parts<SPLIT>(p, g, 0, kParts, f);
+        grav_assemble(x, y, z, p, f.t, g.gm, ax, ay, az);
+        return;
+    }
+    static_assert(kParts % W == 0, "waves must divide the part count");
+    // The wave index is uniform; readfirstlane keeps it, the part range and every record
+    // address in scalar registers.
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+    const int lane = threadIdx.x % 64;
+    __syncthreads();
+    grav_parts<SPLIT>(p, g, w * (kParts / W), (w + 1) * (kParts / W),
+        [&](int q, const GravSums& s) {
+            red[(q * 4 + 0) * 64 + lane] = s.G;
+            red[(q * 4 + 1) * 64 + lane] = s.H;
+            red[(q * 4 + 2) * 64 + lane] = s.J;
+            red[(q * 4 + 3) * 64 + lane] = s.K;
+        });
+    __syncthreads();
+    PartFold f;
+#pragma unroll
+    for (int q = 0; q < kParts; ++q)
+        f(q, GravSums { red[(q * 4 + 0) * 64 + lane], red[(q * 4 + 1) * 64 + lane],
+                 red[(q * 4 + 2) * 64 + lane], red[(q * 4 + 3) * 64 + lane] });
+    grav_assemble(x, y, z, p, f.t, g.gm, ax, ay, az);
 }
 
 // a_GCRS = M g(M^T r) with M the column-major ITRS->GCRS matrix of the node
 // (gravity_source.cu:176-178, :208-210). M is wave-uniform.
-template <bool SPLIT>
-__device__ __forceinline__ void gravity_gcrs(
-    const double* Mg, const double* r, const GravityDev& g, double* a)
+template <bool SPLIT, int W = 1>
+__device__ __forceinline__ void gravity_gcrs(const double* Mg, const double* r,
+    const GravityDev& g, double* a, double* red = nullptr)
 {
     const uniform_dptr M = (uniform_dptr)Mg;
     double ri[3], gi[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
-    gravity_itrs<SPLIT>(ri[0], ri[1], ri[2], g, gi[0], gi[1], gi[2]);
+    if (W == 1)
+        gravity_itrs<SPLIT>(ri[0], ri[1], ri[2], g, gi[0], gi[1], gi[2]);
+    else
+        gravity_itrs_split<SPLIT, W>(ri[0], ri[1], ri[2], g, red, gi[0], gi[1], gi[2]);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
@@ -234,15 +361,20 @@ struct ForceCtx {
     double h;
 };
 
+// Per-workgroup exchange area of the column-split evaluation (unused when W == 1).
+struct Exchange {
+    double* red;
+};
+
 // ForceModel::ComputeGenerationAcceleration (force_model.cu:59-89): RK8 stage k of startup
 // step s (s = +-1..4).
-template <int KIND>
+template <int KIND, int W = 1>
 __device__ __forceinline__ void force_gen(const ForceCtx& f, const double* x, const double* v,
-    int s, int k, double* a)
+    int s, int k, double* a, Exchange& ex)
 {
     if (KIND == kGravity || KIND == kGravitySplit) {
         const int idx = ((s > 0 ? s - 1 : s) + 4) * 10 + k;
-        gravity_gcrs<KIND == kGravitySplit>(f.gen_M + 9 * idx, x, f.grav, a);
+        gravity_gcrs<KIND == kGravitySplit, W>(f.gen_M + 9 * idx, x, f.grav, a, ex.red);
     } else {
         const int dir = s > 0 ? -1 : 1;
         const double t = (double)(s + dir) * f.h + (double)(-dir) * c_rk8c[k] * f.h;
@@ -251,12 +383,12 @@ __device__ __forceinline__ void force_gen(const ForceCtx& f, const double* x, co
 }
 
 // ForceModel::ComputePropagationAcceleration (force_model.cu:91-109) at node j.
-template <int KIND>
-__device__ __forceinline__ void force_prop(
-    const ForceCtx& f, const double* x, const double* v, int64_t node, double* a)
+template <int KIND, int W = 1>
+__device__ __forceinline__ void force_prop(const ForceCtx& f, const double* x, const double* v,
+    int64_t node, double* a, Exchange& ex)
 {
     if (KIND == kGravity || KIND == kGravitySplit) {
-        gravity_gcrs<KIND == kGravitySplit>(f.prop_M + 9 * node, x, f.grav, a);
+        gravity_gcrs<KIND == kGravitySplit, W>(f.prop_M + 9 * node, x, f.grav, a, ex.red);
     } else {
         const double t = (double)(node - 4) * f.h;
         analytic_force<KIND>(x, v, t, a);

@@ -5,99 +5,6 @@
 
 namespace ogp {
 
-// gpu_constants.cuh:37-98 (Gauss-Jackson and summed-Adams ordinate tables, [row][k]).
-__constant__ double c_gj[10][9] = {
-    { 3250433.0 / 53222400.0, 572741.0 / 5702400.0, -8701681.0 / 39916800.0,
-        4026311.0 / 13305600.0, -917039.0 / 3193344.0, 7370669.0 / 39916800.0,
-        -1025779.0 / 13305600.0, 754331.0 / 39916800.0, -330157.0 / 159667200.0 },
-    { -330157.0 / 159667200.0, 530113.0 / 6652800.0, 518887.0 / 19958400.0, -27631.0 / 623700.0,
-        44773.0 / 1064448.0, -531521.0 / 19958400.0, 109343.0 / 9979200.0, -1261.0 / 475200.0,
-        45911.0 / 159667200.0 },
-    { 45911.0 / 159667200.0, -185839.0 / 39916800.0, 171137.0 / 1900800.0,
-        73643.0 / 39916800.0, -25775.0 / 3193344.0, 77597.0 / 13305600.0,
-        -98911.0 / 39916800.0, 24173.0 / 39916800.0, -3499.0 / 53222400.0 },
-    { -3499.0 / 53222400.0, 4387.0 / 4989600.0, -35039.0 / 4989600.0, 90817.0 / 950400.0,
-        -20561.0 / 3193344.0, 2117.0 / 9979200.0, 2059.0 / 6652800.0, -317.0 / 2851200.0,
-        317.0 / 22809600.0 },
-    { 317.0 / 22809600.0, -2539.0 / 13305600.0, 55067.0 / 39916800.0, -326911.0 / 39916800.0,
-        14797.0 / 152064.0, -326911.0 / 39916800.0, 55067.0 / 39916800.0,
-        -2539.0 / 13305600.0, 317.0 / 22809600.0 },
-    { 317.0 / 22809600.0, -317.0 / 2851200.0, 2059.0 / 6652800.0, 2117.0 / 9979200.0,
-        -20561.0 / 3193344.0, 90817.0 / 950400.0, -35039.0 / 4989600.0, 4387.0 / 4989600.0,
-        -3499.0 / 53222400.0 },
-    { -3499.0 / 53222400.0, 24173.0 / 39916800.0, -98911.0 / 39916800.0,
-        77597.0 / 13305600.0, -25775.0 / 3193344.0, 73643.0 / 39916800.0,
-        171137.0 / 1900800.0, -185839.0 / 39916800.0, 45911.0 / 159667200.0 },
-    { 45911.0 / 159667200.0, -1261.0 / 475200.0, 109343.0 / 9979200.0,
-        -531521.0 / 19958400.0, 44773.0 / 1064448.0, -27631.0 / 623700.0,
-        518887.0 / 19958400.0, 530113.0 / 6652800.0, -330157.0 / 159667200.0 },
-    { -330157.0 / 159667200.0, 754331.0 / 39916800.0, -1025779.0 / 13305600.0,
-        7370669.0 / 39916800.0, -917039.0 / 3193344.0, 4026311.0 / 13305600.0,
-        -8701681.0 / 39916800.0, 572741.0 / 5702400.0, 3250433.0 / 53222400.0 },
-    { 3250433.0 / 53222400.0, -11011481.0 / 19958400.0, 6322573.0 / 2851200.0,
-        -8660609.0 / 1663200.0, 25162927.0 / 3193344.0, -159314453.0 / 19958400.0,
-        18071351.0 / 3326400.0, -24115843.0 / 9979200.0, 103798439.0 / 159667200.0 }
-};
-
-__constant__ double c_sa[10][9] = {
-    { 19087.0 / 89600.0, -427487.0 / 725760.0, 3498217.0 / 3628800.0, -500327.0 / 403200.0,
-        6467.0 / 5670.0, -2616161.0 / 3628800.0, 24019.0 / 80640.0, -263077.0 / 3628800.0,
-        8183.0 / 1036800.0 },
-    { 8183.0 / 1036800.0, 57251.0 / 403200.0, -1106377.0 / 3628800.0, 218483.0 / 725760.0,
-        -69.0 / 280.0, 530177.0 / 3628800.0, -210359.0 / 3628800.0, 5533.0 / 403200.0,
-        -425.0 / 290304.0 },
-    { -425.0 / 290304.0, 76453.0 / 3628800.0, 5143.0 / 57600.0, -660127.0 / 3628800.0,
-        661.0 / 5670.0, -4997.0 / 80640.0, 83927.0 / 3628800.0, -19109.0 / 3628800.0,
-        7.0 / 12800.0 },
-    { 7.0 / 12800.0, -23173.0 / 3628800.0, 29579.0 / 725760.0, 2497.0 / 57600.0,
-        -2563.0 / 22680.0, 172993.0 / 3628800.0, -6463.0 / 403200.0, 2497.0 / 725760.0,
-        -2497.0 / 7257600.0 },
-    { -2497.0 / 7257600.0, 1469.0 / 403200.0, -68119.0 / 3628800.0, 252769.0 / 3628800.0, 0.0,
-        -252769.0 / 3628800.0, 68119.0 / 3628800.0, -1469.0 / 403200.0, 2497.0 / 7257600.0 },
-    { 2497.0 / 7257600.0, -2497.0 / 725760.0, 6463.0 / 403200.0, -172993.0 / 3628800.0,
-        2563.0 / 22680.0, -2497.0 / 57600.0, -29579.0 / 725760.0, 23173.0 / 3628800.0,
-        -7.0 / 12800.0 },
-    { -7.0 / 12800.0, 19109.0 / 3628800.0, -83927.0 / 3628800.0, 4997.0 / 80640.0,
-        -661.0 / 5670.0, 660127.0 / 3628800.0, -5143.0 / 57600.0, -76453.0 / 3628800.0,
-        425.0 / 290304.0 },
-    { 425.0 / 290304.0, -5533.0 / 403200.0, 210359.0 / 3628800.0, -530177.0 / 3628800.0,
-        69.0 / 280.0, -218483.0 / 725760.0, 1106377.0 / 3628800.0, -57251.0 / 403200.0,
-        -8183.0 / 1036800.0 },
-    { -8183.0 / 1036800.0, 263077.0 / 3628800.0, -24019.0 / 80640.0, 2616161.0 / 3628800.0,
-        -6467.0 / 5670.0, 500327.0 / 403200.0, -3498217.0 / 3628800.0, 427487.0 / 725760.0,
-        -19087.0 / 89600.0 },
-    { 25713.0 / 89600.0, -9401029.0 / 3628800.0, 5393233.0 / 518400.0, -9839609.0 / 403200.0,
-        167287.0 / 4536.0, -135352319.0 / 3628800.0, 10219841.0 / 403200.0,
-        -40987771.0 / 3628800.0, 3288521.0 / 1036800.0 }
-};
-
-__constant__ double c_rk8b[10] = { 41.0 / 840.0, 0.0, 0.0, 27.0 / 840.0, 272.0 / 840.0,
-    27.0 / 840.0, 216.0 / 840.0, 0.0, 216.0 / 840.0, 41.0 / 840.0 };
-__constant__ double c_rk8c[10] = { 0.0, 4.0 / 27.0, 2.0 / 9.0, 1.0 / 3.0, 1.0 / 2.0,
-    2.0 / 3.0, 1.0 / 6.0, 1.0, 5.0 / 6.0, 1.0 };
-// RK8 stage matrix (gpu_constants.cuh:25-35). Variant 0 = the reference's row 7 (-234/20,
-// which sums to 17/20 although c7 = 1); variant 1 = -231/20 (row sum 1, 8th-order linear
-// conditions hold).
-__constant__ double c_rk8a[2][9][9] = {
-    { { 4.0 / 27, 0, 0, 0, 0, 0, 0, 0, 0 }, { 1.0 / 18, 3.0 / 18, 0, 0, 0, 0, 0, 0, 0 },
-        { 1.0 / 12, 0, 3.0 / 12, 0, 0, 0, 0, 0, 0 }, { 1.0 / 8, 0, 0, 3.0 / 8, 0, 0, 0, 0, 0 },
-        { 13.0 / 54, 0, -27.0 / 54, 42.0 / 54, 8.0 / 54, 0, 0, 0, 0 },
-        { 389.0 / 4320, 0, -54.0 / 4320, 966.0 / 4320, -824.0 / 4320.0, 243.0 / 4320, 0, 0, 0 },
-        { -234.0 / 20, 0, 81.0 / 20, -1164.0 / 20, 656.0 / 20, -122.0 / 20, 800.0 / 20, 0, 0 },
-        { -127.0 / 288, 0, 18.0 / 288, -678.0 / 288, 456.0 / 288, -9.0 / 288, 576.0 / 288,
-            4.0 / 288, 0 },
-        { 1481.0 / 820, 0, -81.0 / 820, 7104.0 / 820, -3376.0 / 820, 72.0 / 820, -5040.0 / 820,
-            -60.0 / 820, 720.0 / 820 } },
-    { { 4.0 / 27, 0, 0, 0, 0, 0, 0, 0, 0 }, { 1.0 / 18, 3.0 / 18, 0, 0, 0, 0, 0, 0, 0 },
-        { 1.0 / 12, 0, 3.0 / 12, 0, 0, 0, 0, 0, 0 }, { 1.0 / 8, 0, 0, 3.0 / 8, 0, 0, 0, 0, 0 },
-        { 13.0 / 54, 0, -27.0 / 54, 42.0 / 54, 8.0 / 54, 0, 0, 0, 0 },
-        { 389.0 / 4320, 0, -54.0 / 4320, 966.0 / 4320, -824.0 / 4320.0, 243.0 / 4320, 0, 0, 0 },
-        { -231.0 / 20, 0, 81.0 / 20, -1164.0 / 20, 656.0 / 20, -122.0 / 20, 800.0 / 20, 0, 0 },
-        { -127.0 / 288, 0, 18.0 / 288, -678.0 / 288, 456.0 / 288, -9.0 / 288, 576.0 / 288,
-            4.0 / 288, 0 },
-        { 1481.0 / 820, 0, -81.0 / 820, 7104.0 / 820, -3376.0 / 820, 72.0 / 820, -5040.0 / 820,
-            -60.0 / 820, 720.0 / 820 } }
-};
 
 // =========================================================================================
 // Time nodes: ITRS->GCRS matrix per node (time_points.cu:26-71, ref_sys_trans.cu)
@@ -373,314 +280,6 @@ void launch_gravity_aos(hipStream_t st, bool split, const GravityDev& g, const d
     else
         hipLaunchKernelGGL(gravity_aos_kernel<false>, grid, dim3(block), 0, st, g, d_M, d_r, n,
             d_a, out_stride);
-}
-
-// =========================================================================================
-// Startup: RK8 4 forward + 4 backward steps and the startup corrector, one object per lane
-// (gpu_startup.cu:55-312 and src/device/*.cu). Work arrays are [slot][component][object].
-// =========================================================================================
-namespace {
-
-struct Slots {
-    double* base;
-    int64_t N, i;
-    __device__ double& operator()(int slot, int c) const { return base[(slot * 3 + c) * N + i]; }
-};
-
-__device__ __forceinline__ int ring_slot(int64_t j) { return (int)(((j % 9) + 9) % 9); }
-
-template <int KIND>
-__device__ void rk8_step(const ForceCtx& f, const double (*a)[9], double h, const Slots& nr,
-    const Slots& nv, const Slots& Kx, const Slots& Kv, int step)
-{
-    const double hs = step > 0 ? h : -h;
-    const int prev = 4 + step + (step > 0 ? -1 : 1);
-    double x0[3], v0[3], tX[3], tV[3], out[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        x0[c] = nr(prev, c);
-        v0[c] = nv(prev, c);
-        Kx(0, c) = v0[c];
-    }
-    force_gen<KIND>(f, x0, v0, step, 0, out);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        Kv(0, c) = out[c];
-    for (int k = 1; k <= 9; ++k) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double sx = 0.0, sv = 0.0;
-            for (int j = 0; j < k; ++j) {
-                sx = fma(Kx(j, c), a[k - 1][j], sx);
-                sv = fma(Kv(j, c), a[k - 1][j], sv);
-            }
-            tX[c] = fma(hs, sx, x0[c]);
-            tV[c] = fma(hs, sv, v0[c]);
-        }
-        force_gen<KIND>(f, tX, tV, step, k, out);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            Kv(k, c) = out[c];
-            Kx(k, c) = tV[c];
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double sv = 0.0, sx = 0.0;
-        for (int k = 0; k < 10; ++k) {
-            sv = fma(Kv(k, c), c_rk8b[k], sv);
-            sx = fma(Kx(k, c), c_rk8b[k], sx);
-        }
-        nv(4 + step, c) = fma(hs, sv, v0[c]);
-        nr(4 + step, c) = fma(hs, sx, x0[c]);
-    }
-}
-
-template <int KIND>
-__device__ void acc_matrix(const ForceCtx& f, const Slots& nr, const Slots& nv, const Slots& acc,
-    bool epoch)
-{
-    for (int j = 0; j < 9; ++j) {
-        if (j == 4 && !epoch)
-            continue;
-        double x[3], v[3], o[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            x[c] = nr(j, c);
-            v[c] = nv(j, c);
-        }
-        force_prop<KIND>(f, x, v, j, o);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            acc(j, c) = o[c];
-    }
-}
-
-// StartupCorrector::Correct (gpu_startup.cu:247-294) with CalculateS0 (:170-195) and
-// CalculateAllSKernel (calculate_all_s.cu:19-47).
-template <int KIND>
-__device__ void startup_correct(const ForceCtx& f, double h, const Slots& nr, const Slots& nv,
-    const Slots& acc, const Slots& sm, const Slots& Sm)
-{
-    const double invh = 1.0 / h, invh2 = 1.0 / (h * h), hh = h * h;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        double ss = 0.0, sS = 0.0;
-        for (int k = 0; k < 9; ++k) {
-            ss = fma(acc(k, c), c_sa[4][k], ss);
-            sS = fma(acc(k, c), c_gj[4][k], sS);
-        }
-        sm(4, c) = fma(invh, nv(4, c), -ss);
-        Sm(4, c) = fma(invh2, nr(4, c), -sS);
-        for (int i = 1; i <= 4; ++i) {
-            sm(4 + i, c) = sm(4 + i - 1, c) + (acc(4 + i - 1, c) + acc(4 + i, c)) / 2;
-            sm(4 - i, c) = sm(4 - i + 1, c) - (acc(4 - i + 1, c) + acc(4 - i, c)) / 2;
-            Sm(4 + i, c) = sm(4 + i - 1, c) + Sm(4 + i - 1, c) + acc(4 + i - 1, c) / 2;
-            Sm(4 - i, c) = -sm(4 - i + 1, c) + Sm(4 - i + 1, c) + acc(4 - i + 1, c) / 2;
-        }
-        for (int j = 0; j < 9; ++j) {
-            if (j == 4)
-                continue;
-            double sv = 0.0, sp = 0.0;
-            for (int k = 0; k < 9; ++k) {
-                sv = fma(acc(k, c), c_sa[j][k], sv);
-                sp = fma(acc(k, c), c_gj[j][k], sp);
-            }
-            nv(j, c) = (sm(j, c) + sv) * h;
-            nr(j, c) = (Sm(j, c) + sp) * hh;
-        }
-    }
-    acc_matrix<KIND>(f, nr, nv, acc, false);
-}
-
-} // namespace
-
-template <int KIND>
-__global__ void __launch_bounds__(256) startup_kernel(ForceCtx f, int rk8_variant, PropState ps, StartupScratch sc,
-    int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps)
-// traj rows 0..3 receive steps 1..4 (rows past traj_steps are not written)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nobj)
-        return;
-    const int64_t i = obj0 + t;
-    const int64_t N = ps.N;
-    const int64_t ns = ps.nsteps[i];
-    if (ns <= 0)
-        return; // no GpuPropagator::Step call: the state is left untouched
-    const Slots nr { sc.node_r, N, i }, nv { sc.node_v, N, i }, acc { sc.acc, N, i };
-    const Slots sm { sc.sm, N, i }, Sm { sc.Sm, N, i }, Kx { sc.Kx, N, i }, Kv { sc.Kv, N, i };
-    const double h = f.h;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        nr(4, c) = ps.r[c * N + i];
-        nv(4, c) = ps.v[c * N + i];
-    }
-    const double(*a)[9] = c_rk8a[rk8_variant];
-    for (int s = 1; s <= 4; ++s) {
-        rk8_step<KIND>(f, a, h, nr, nv, Kx, Kv, s);
-        rk8_step<KIND>(f, a, h, nr, nv, Kx, Kv, -s);
-    }
-    acc_matrix<KIND>(f, nr, nv, acc, true);
-    for (int it = 0; it < ncorr; ++it)
-        startup_correct<KIND>(f, h, nr, nv, acc, sm, Sm);
-
-    // Hand-over to the GJ8 steps (gpu_propagator.cu:31-53, copy_s.cu:19-30).
-    const int last = (int)(ns < 4 ? ns : 4);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        ps.s[c * N + i] = sm(8, c);
-        ps.S[c * N + i] = Sm(8, c);
-        ps.r[c * N + i] = nr(4 + last, c);
-        ps.v[c * N + i] = nv(4 + last, c);
-        for (int j = -4; j <= 4; ++j)
-            ps.hist[(ring_slot(j) * 3 + c) * N + i] = acc(j + 4, c);
-    }
-    if (traj) {
-        // steps 1-4 output the startup nodes; the acceleration handed to the conjunction
-        // screen is the acceleration-matrix column 8 (gpu_propagator.cu:68-72)
-        for (int k = 1; k <= last && k <= traj_steps; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                traj[((int64_t)(k - 1) * 9 + c) * N + i] = nr(4 + k, c);
-                traj[((int64_t)(k - 1) * 9 + 3 + c) * N + i] = nv(4 + k, c);
-                traj[((int64_t)(k - 1) * 9 + 6 + c) * N + i] = acc(8, c);
-            }
-    }
-}
-
-// =========================================================================================
-// GJ8 steps k0 .. k1-1 (each k >= 5), persistent over the chunk, one object per lane:
-// Predictor::Predict (prediction.cu:49-74, dev_functions.cu:20-50) then the corrector
-// (gpu_propagator.cu:86-119, dev_functions.cu:36-69). The history is a ring of 9 slots.
-// The first corrector evaluation would see exactly the predicted state and reproduce the
-// predictor's evaluation bit for bit, so it is reused: k distinct evaluations per step.
-// =========================================================================================
-template <int KIND>
-__global__ void __launch_bounds__(256) gj8_chunk_kernel(ForceCtx f, PropState ps, int64_t obj0, int64_t nobj,
-    int64_t k0, int64_t k1, int ncorr, double* traj, int64_t traj_first, int64_t traj_steps)
-{
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nobj)
-        return;
-    const int64_t i = obj0 + t;
-    const int64_t N = ps.N;
-    const int64_t ns = ps.nsteps[i];
-    if (ns < k0)
-        return;
-    const double h = f.h, hh = h * h;
-    const double sa88 = c_sa[8][8], gj88 = c_gj[8][8];
-    double r[3], v[3], s[3], S[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        r[c] = ps.r[c * N + i];
-        v[c] = ps.v[c * N + i];
-        s[c] = ps.s[c * N + i];
-        S[c] = ps.S[c * N + i];
-    }
-    for (int64_t k = k0; k < k1; ++k) {
-        if (k > ns)
-            continue; // this object's span has ended; the wave continues for the others
-        double Pv[3] = { 0, 0, 0 }, Pr[3] = { 0, 0, 0 }, T1[3] = { 0, 0, 0 }, T2[3] = { 0, 0, 0 };
-        double alast[3];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            const int slot = ring_slot(k - 9 + q);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double a = ps.hist[(slot * 3 + c) * N + i];
-                Pv[c] = fma(c_sa[9][q], a, Pv[c]);
-                Pr[c] = fma(c_gj[9][q], a, Pr[c]);
-                if (q >= 1) {
-                    T1[c] = fma(c_sa[8][q - 1], a, T1[c]);
-                    T2[c] = fma(c_gj[8][q - 1], a, T2[c]);
-                }
-                if (q == 8)
-                    alast[c] = a;
-            }
-        }
-        double sprev[3], anew[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            S[c] += s[c] + alast[c] / 2;
-            v[c] = ((Pv[c] + s[c]) + alast[c] / 2) * h;
-            r[c] = ((Pr[c] + S[c]) * h) * h;
-            sprev[c] = s[c];
-        }
-        force_prop<KIND>(f, r, v, k + 4, anew);
-        for (int it = 0; it < ncorr; ++it) {
-            if (it > 0)
-                force_prop<KIND>(f, r, v, k + 4, anew);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                s[c] = sprev[c] + (anew[c] + alast[c]) / 2;
-                v[c] = h * ((T1[c] + anew[c] * sa88) + s[c]);
-                r[c] = hh * ((T2[c] + anew[c] * gj88) + S[c]);
-            }
-        }
-        const int wslot = ring_slot(k);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            ps.hist[(wslot * 3 + c) * N + i] = anew[c];
-        if (traj && k >= traj_first && k < traj_first + traj_steps) {
-            const int64_t row = k - traj_first;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                traj[(row * 9 + c) * N + i] = r[c];
-                traj[(row * 9 + 3 + c) * N + i] = v[c];
-                traj[(row * 9 + 6 + c) * N + i] = anew[c];
-            }
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        ps.r[c * N + i] = r[c];
-        ps.v[c * N + i] = v[c];
-        ps.s[c * N + i] = s[c];
-        ps.S[c * N + i] = S[c];
-    }
-}
-
-#define OGP_KIND_SWITCH(kind, LAUNCH)                                                              \
-    switch (kind) {                                                                                \
-    case kGravity: LAUNCH(kGravity); break;                                                        \
-    case kGravitySplit: LAUNCH(kGravitySplit); break;                                              \
-    case kNone: LAUNCH(kNone); break;                                                              \
-    case kConst: LAUNCH(kConst); break;                                                            \
-    case kPoly7: LAUNCH(kPoly7); break;                                                            \
-    case kPoly8: LAUNCH(kPoly8); break;                                                            \
-    case kPoly9: LAUNCH(kPoly9); break;                                                            \
-    case kExp: LAUNCH(kExp); break;                                                                \
-    case kDamped: LAUNCH(kDamped); break;                                                          \
-    default: break;                                                                                \
-    }
-
-void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant,
-    const PropState& ps, const StartupScratch& sc, int64_t obj0, int64_t nobj, int ncorr,
-    double* traj, int64_t traj_steps, int block)
-{
-    if (nobj <= 0)
-        return;
-    const dim3 grid((unsigned)((nobj + block - 1) / block));
-#define L_(K)                                                                                      \
-    hipLaunchKernelGGL(startup_kernel<K>, grid, dim3(block), 0, st, f, rk8_variant, ps, sc, obj0,  \
-        nobj, ncorr, traj, traj_steps)
-    OGP_KIND_SWITCH(kind, L_)
-#undef L_
-}
-
-void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
-    int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
-    int64_t traj_first, int64_t traj_steps, int block)
-{
-    if (nobj <= 0 || k1 <= k0)
-        return;
-    const dim3 grid((unsigned)((nobj + block - 1) / block));
-#define L_(K)                                                                                      \
-    hipLaunchKernelGGL(gj8_chunk_kernel<K>, grid, dim3(block), 0, st, f, ps, obj0, nobj, k0, k1,   \
-        ncorr, traj, traj_first, traj_steps)
-    OGP_KIND_SWITCH(kind, L_)
-#undef L_
 }
 
 // =========================================================================================

@@ -48,15 +48,21 @@ void launch_time_nodes(hipStream_t st, const FrameDev& fd, const int* ymdhm, dou
 void launch_gravity_aos(hipStream_t st, bool split, const GravityDev& g, const double* d_M,
     const double* d_r, int64_t n, double* d_a, int64_t out_stride, int block);
 
+// Startup and GJ8 kernels (propagate.hip). `waves` = W waves per group of 64 objects
+// (1, 2, 4 or 8; analytic test forces always use 1). Results do not depend on W.
 void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant,
     const PropState& ps, const StartupScratch& sc, int64_t obj0, int64_t nobj, int ncorr,
-    double* traj, int64_t traj_steps, int block);
+    double* traj, int64_t traj_steps, int waves);
 
 // GJ8 steps k0 .. k1-1 (k0 >= 5) of objects [obj0, obj0 + nobj). With traj != null, steps
 // traj_first .. traj_first + traj_steps - 1 are stored at rows 0.. of traj[row][9][N].
 void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
     int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
-    int64_t traj_first, int64_t traj_steps, int block);
+    int64_t traj_first, int64_t traj_steps, int waves);
+
+// The kParts balanced contiguous column ranges of the harmonic sums (fills kParts + 1 bounds,
+// parts[0] = 0, parts[kParts] = L + 1, non-decreasing).
+void column_parts(int L, int* parts);
 
 void launch_kat_predict(hipStream_t st, const double* acc27, const double* s3, double* S3,
     int64_t n, double h, double* r3, double* v3);

@@ -1,0 +1,381 @@
+// RK8 startup and GJ8 step kernels for gfx950.
+//
+// Layout: a workgroup of W waves (W = 1, 2, 4 or 8) holds one group of 64 objects; lane l of
+// every wave owns object 64 * blockIdx.x + l. The spherical-harmonic columns of each force
+// evaluation are divided between the W waves and their partial sums meet in LDS
+// (device.hpp, gravity_itrs_split); everything else - the integrator arithmetic, its global
+// loads and stores - runs redundantly and bit-identically in all W waves, so no wave ever reads
+// a value written by another wave through global memory.
+#include "kernels.hpp"
+
+#include <math.h>
+
+#include <vector>
+
+namespace ogp {
+
+namespace {
+
+struct Slots {
+    double* base;
+    int64_t N, i;
+    __device__ double& operator()(int slot, int c) const { return base[(slot * 3 + c) * N + i]; }
+};
+
+__device__ __forceinline__ int ring_slot(int64_t j) { return (int)(((j % 9) + 9) % 9); }
+
+// RK8::step (gpu_startup.cu:108-144, reset_temp_vectors.cu, device_functions.cu,
+// copy_prev_state.cu): stage states from the previous node, stage accelerations into Kv,
+// stage velocities into Kx, then node(4 + step) = prev + h * sum b_k K_k.
+template <int KIND, int W>
+__device__ void rk8_step(const ForceCtx& f, const double (*a)[9], double h, const Slots& nr,
+    const Slots& nv, const Slots& Kx, const Slots& Kv, int step, Exchange& ex)
+{
+    const double hs = step > 0 ? h : -h;
+    const int prev = 4 + step + (step > 0 ? -1 : 1);
+    double x0[3], v0[3], tX[3], tV[3], out[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        x0[c] = nr(prev, c);
+        v0[c] = nv(prev, c);
+    }
+    force_gen<KIND, W>(f, x0, v0, step, 0, out, ex);
+    // Kx(0) is written only after the stage-0 evaluation (and its workgroup barrier): another
+    // wave of the group may still be reading the previous step's Kx(0) before that barrier.
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Kx(0, c) = v0[c];
+        Kv(0, c) = out[c];
+    }
+#pragma unroll 1
+    for (int k = 1; k <= 9; ++k) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double sx = 0.0, sv = 0.0;
+#pragma unroll 1
+            for (int j = 0; j < k; ++j) {
+                sx = fma(Kx(j, c), a[k - 1][j], sx);
+                sv = fma(Kv(j, c), a[k - 1][j], sv);
+            }
+            tX[c] = fma(hs, sx, x0[c]);
+            tV[c] = fma(hs, sv, v0[c]);
+        }
+        force_gen<KIND, W>(f, tX, tV, step, k, out, ex);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Kv(k, c) = out[c];
+            Kx(k, c) = tV[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double sv = 0.0, sx = 0.0;
+        for (int k = 0; k < 10; ++k) {
+            sv = fma(Kv(k, c), c_rk8b[k], sv);
+            sx = fma(Kx(k, c), c_rk8b[k], sx);
+        }
+        nv(4 + step, c) = fma(hs, sv, v0[c]);
+        nr(4 + step, c) = fma(hs, sx, x0[c]);
+    }
+}
+
+// StartupCorrector::CalculateAccelerationMatrix (gpu_startup.cu:196-224)
+template <int KIND, int W>
+__device__ void acc_matrix(const ForceCtx& f, const Slots& nr, const Slots& nv, const Slots& acc,
+    bool epoch, Exchange& ex)
+{
+#pragma unroll 1
+    for (int j = 0; j < 9; ++j) {
+        if (j == 4 && !epoch)
+            continue;
+        double x[3], v[3], o[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[c] = nr(j, c);
+            v[c] = nv(j, c);
+        }
+        force_prop<KIND, W>(f, x, v, j, o, ex);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            acc(j, c) = o[c];
+    }
+}
+
+// StartupCorrector::Correct (gpu_startup.cu:247-294) with CalculateS0 (:170-195) and
+// CalculateAllSKernel (calculate_all_s.cu:19-47).
+template <int KIND, int W>
+__device__ void startup_correct(const ForceCtx& f, double h, const Slots& nr, const Slots& nv,
+    const Slots& acc, const Slots& sm, const Slots& Sm, Exchange& ex)
+{
+    const double invh = 1.0 / h, invh2 = 1.0 / (h * h), hh = h * h;
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        double ss = 0.0, sS = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            ss = fma(acc(k, c), c_sa[4][k], ss);
+            sS = fma(acc(k, c), c_gj[4][k], sS);
+        }
+        sm(4, c) = fma(invh, nv(4, c), -ss);
+        Sm(4, c) = fma(invh2, nr(4, c), -sS);
+        for (int i = 1; i <= 4; ++i) {
+            sm(4 + i, c) = sm(4 + i - 1, c) + (acc(4 + i - 1, c) + acc(4 + i, c)) / 2;
+            sm(4 - i, c) = sm(4 - i + 1, c) - (acc(4 - i + 1, c) + acc(4 - i, c)) / 2;
+            Sm(4 + i, c) = sm(4 + i - 1, c) + Sm(4 + i - 1, c) + acc(4 + i - 1, c) / 2;
+            Sm(4 - i, c) = -sm(4 - i + 1, c) + Sm(4 - i + 1, c) + acc(4 - i + 1, c) / 2;
+        }
+        for (int j = 0; j < 9; ++j) {
+            if (j == 4)
+                continue;
+            double sv = 0.0, sp = 0.0;
+            for (int k = 0; k < 9; ++k) {
+                sv = fma(acc(k, c), c_sa[j][k], sv);
+                sp = fma(acc(k, c), c_gj[j][k], sp);
+            }
+            nv(j, c) = (sm(j, c) + sv) * h;
+            nr(j, c) = (Sm(j, c) + sp) * hh;
+        }
+    }
+    acc_matrix<KIND, W>(f, nr, nv, acc, false, ex);
+}
+
+template <int W> struct ExchangeArea {
+    static constexpr int size = W > 1 ? kParts * 4 * 64 : 1;
+};
+
+} // namespace
+
+// GpuStartup::Startup (gpu_startup.cu:55-68): 4 forward and 4 backward RK8 steps, the
+// acceleration matrix and ncorr corrections; then the hand-over to the GJ8 steps
+// (gpu_propagator.cu:31-53, copy_s.cu:19-30).
+template <int KIND, int W>
+__global__ void __launch_bounds__(64 * W) startup_kernel(ForceCtx f, int rk8_variant, PropState ps,
+    StartupScratch sc, int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps)
+{
+    __shared__ double red[ExchangeArea<W>::size];
+    Exchange ex { red };
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
+    if (t >= nobj)
+        return;
+    const int64_t i = obj0 + t;
+    const int64_t N = ps.N;
+    const int64_t ns = ps.nsteps[i];
+    if (ns <= 0)
+        return; // no GpuPropagator::Step call: the state is left untouched
+    const Slots nr { sc.node_r, N, i }, nv { sc.node_v, N, i }, acc { sc.acc, N, i };
+    const Slots sm { sc.sm, N, i }, Sm { sc.Sm, N, i }, Kx { sc.Kx, N, i }, Kv { sc.Kv, N, i };
+    const double h = f.h;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        nr(4, c) = ps.r[c * N + i];
+        nv(4, c) = ps.v[c * N + i];
+    }
+    const double(*a)[9] = c_rk8a[rk8_variant];
+#pragma unroll 1
+    for (int s = 1; s <= 4; ++s) {
+        rk8_step<KIND, W>(f, a, h, nr, nv, Kx, Kv, s, ex);
+        rk8_step<KIND, W>(f, a, h, nr, nv, Kx, Kv, -s, ex);
+    }
+    acc_matrix<KIND, W>(f, nr, nv, acc, true, ex);
+#pragma unroll 1
+    for (int it = 0; it < ncorr; ++it)
+        startup_correct<KIND, W>(f, h, nr, nv, acc, sm, Sm, ex);
+
+    const int last = (int)(ns < 4 ? ns : 4);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ps.s[c * N + i] = sm(8, c);
+        ps.S[c * N + i] = Sm(8, c);
+        ps.r[c * N + i] = nr(4 + last, c);
+        ps.v[c * N + i] = nv(4 + last, c);
+        for (int j = -4; j <= 4; ++j)
+            ps.hist[(ring_slot(j) * 3 + c) * N + i] = acc(j + 4, c);
+    }
+    if (traj) {
+        // steps 1-4 output the startup nodes; the acceleration handed to the conjunction
+        // screen is the acceleration-matrix column 8 (gpu_propagator.cu:68-72)
+        for (int k = 1; k <= last && k <= traj_steps; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                traj[((int64_t)(k - 1) * 9 + c) * N + i] = nr(4 + k, c);
+                traj[((int64_t)(k - 1) * 9 + 3 + c) * N + i] = nv(4 + k, c);
+                traj[((int64_t)(k - 1) * 9 + 6 + c) * N + i] = acc(8, c);
+            }
+    }
+}
+
+// GJ8 steps k0 .. k1-1 (each k >= 5), persistent over the chunk: Predictor::Predict
+// (prediction.cu:49-74, dev_functions.cu:20-50) and the corrector (gpu_propagator.cu:86-119,
+// dev_functions.cu:36-69). The acceleration history is a ring of 9 slots in HBM.
+#ifndef OGP_CHUNK_WAVES_PER_EU
+#define OGP_CHUNK_WAVES_PER_EU 3 // resident waves per SIMD the register allocation aims for
+#endif
+template <int KIND, int W>
+__global__ void __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(OGP_CHUNK_WAVES_PER_EU)))
+gj8_chunk_kernel(ForceCtx f, PropState ps, int64_t obj0,
+    int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj, int64_t traj_first,
+    int64_t traj_steps)
+{
+    __shared__ double red[ExchangeArea<W>::size];
+    Exchange ex { red };
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
+    if (t >= nobj)
+        return;
+    const int64_t i = obj0 + t;
+    const int64_t N = ps.N;
+    const int64_t ns = ps.nsteps[i];
+    if (ns < k0)
+        return;
+    const double h = f.h, hh = h * h;
+    const double sa88 = c_sa[8][8], gj88 = c_gj[8][8];
+    double r[3], v[3], s[3], S[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r[c] = ps.r[c * N + i];
+        v[c] = ps.v[c * N + i];
+        s[c] = ps.s[c * N + i];
+        S[c] = ps.S[c * N + i];
+    }
+    for (int64_t k = k0; k < k1; ++k) {
+        if (k > ns)
+            continue; // this object's span has ended; its workgroup continues for the others
+        double Pv[3] = { 0, 0, 0 }, Pr[3] = { 0, 0, 0 }, T1[3] = { 0, 0, 0 }, T2[3] = { 0, 0, 0 };
+        double alast[3];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int slot = ring_slot(k - 9 + q);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double a = ps.hist[(slot * 3 + c) * N + i];
+                Pv[c] = fma(c_sa[9][q], a, Pv[c]);
+                Pr[c] = fma(c_gj[9][q], a, Pr[c]);
+                if (q >= 1) {
+                    T1[c] = fma(c_sa[8][q - 1], a, T1[c]);
+                    T2[c] = fma(c_gj[8][q - 1], a, T2[c]);
+                }
+                if (q == 8)
+                    alast[c] = a;
+            }
+        }
+        double sprev[3], anew[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            S[c] += s[c] + alast[c] / 2;
+            v[c] = ((Pv[c] + s[c]) + alast[c] / 2) * h;
+            r[c] = ((Pr[c] + S[c]) * h) * h;
+            sprev[c] = s[c];
+        }
+        // P E (E C)^ncorr: the predictor's evaluation is the first corrector evaluation.
+        for (int it = 0; it < ncorr; ++it) {
+            force_prop<KIND, W>(f, r, v, k + 4, anew, ex);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                s[c] = sprev[c] + (anew[c] + alast[c]) / 2;
+                v[c] = h * ((T1[c] + anew[c] * sa88) + s[c]);
+                r[c] = hh * ((T2[c] + anew[c] * gj88) + S[c]);
+            }
+        }
+        const int wslot = ring_slot(k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            ps.hist[(wslot * 3 + c) * N + i] = anew[c];
+        if (traj && k >= traj_first && k < traj_first + traj_steps) {
+            const int64_t row = k - traj_first;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                traj[(row * 9 + c) * N + i] = r[c];
+                traj[(row * 9 + 3 + c) * N + i] = v[c];
+                traj[(row * 9 + 6 + c) * N + i] = anew[c];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ps.r[c * N + i] = r[c];
+        ps.v[c * N + i] = v[c];
+        ps.s[c * N + i] = s[c];
+        ps.S[c * N + i] = S[c];
+    }
+}
+
+// Instantiation over force kinds (analytic test forces use W = 1 only).
+#define OGP_DISPATCH(kind, waves, LAUNCH)                                                          \
+    do {                                                                                           \
+        if (kind == kGravity || kind == kGravitySplit) {                                           \
+            const bool split_ = kind == kGravitySplit;                                             \
+            switch (waves) {                                                                       \
+            case 2: if (split_) LAUNCH(kGravitySplit, 2); else LAUNCH(kGravity, 2); break;         \
+            case 4: if (split_) LAUNCH(kGravitySplit, 4); else LAUNCH(kGravity, 4); break;         \
+            case 8: if (split_) LAUNCH(kGravitySplit, 8); else LAUNCH(kGravity, 8); break;         \
+            default: if (split_) LAUNCH(kGravitySplit, 1); else LAUNCH(kGravity, 1); break;        \
+            }                                                                                      \
+        } else {                                                                                   \
+            switch (kind) {                                                                        \
+            case kNone: LAUNCH(kNone, 1); break;                                                   \
+            case kConst: LAUNCH(kConst, 1); break;                                                 \
+            case kPoly7: LAUNCH(kPoly7, 1); break;                                                 \
+            case kPoly8: LAUNCH(kPoly8, 1); break;                                                 \
+            case kPoly9: LAUNCH(kPoly9, 1); break;                                                 \
+            case kExp: LAUNCH(kExp, 1); break;                                                     \
+            case kDamped: LAUNCH(kDamped, 1); break;                                               \
+            default: break;                                                                        \
+            }                                                                                      \
+        }                                                                                          \
+    } while (0)
+
+int effective_waves(int kind, int waves)
+{
+    if (kind != kGravity && kind != kGravitySplit)
+        return 1;
+    return waves == 2 || waves == 4 || waves == 8 ? waves : 1;
+}
+
+void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant,
+    const PropState& ps, const StartupScratch& sc, int64_t obj0, int64_t nobj, int ncorr,
+    double* traj, int64_t traj_steps, int waves)
+{
+    if (nobj <= 0)
+        return;
+    const dim3 grid((unsigned)((nobj + 63) / 64));
+    const int W_ = effective_waves(kind, waves);
+#define L_(K, Wv)                                                                                  \
+    hipLaunchKernelGGL((startup_kernel<K, Wv>), grid, dim3(64 * (Wv)), 0, st, f, rk8_variant, ps,   \
+        sc, obj0, nobj, ncorr, traj, traj_steps)
+    OGP_DISPATCH(kind, W_, L_);
+#undef L_
+}
+
+void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
+    int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
+    int64_t traj_first, int64_t traj_steps, int waves)
+{
+    if (nobj <= 0 || k1 <= k0)
+        return;
+    const dim3 grid((unsigned)((nobj + 63) / 64));
+    const int W_ = effective_waves(kind, waves);
+#define L_(K, Wv)                                                                                  \
+    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv>), grid, dim3(64 * (Wv)), 0, st, f, ps, obj0, nobj, \
+        k0, k1, ncorr, traj, traj_first, traj_steps)
+    OGP_DISPATCH(kind, W_, L_);
+#undef L_
+}
+
+void column_parts(int L, int* parts)
+{
+    // Contiguous column ranges of equal cost: about 10 vector operations per record (column m
+    // has L - m + 1 records) plus about 25 per column for its set-up and epilogue.
+    std::vector<double> C(L + 2, 0.0);
+    for (int m = 0; m <= L; ++m)
+        C[m + 1] = C[m] + 10.0 * (L - m + 1) + 25.0;
+    parts[0] = 0;
+    for (int q = 1; q < kParts; ++q) {
+        const double target = C[L + 1] * q / kParts;
+        int m = parts[q - 1];
+        while (m <= L && C[m] < target)
+            ++m;
+        parts[q] = m;
+    }
+    parts[kParts] = L + 1;
+}
+
+} // namespace ogp

@@ -93,6 +93,15 @@ class Context:
                                           ptr(v)))
         return r, v
 
+    def set_tuning(self, waves=0, chunk_steps=0):
+        check(self.h, lib().ogp_set_tuning(self.h, waves, chunk_steps))
+
+    def get_tuning(self, n):
+        w = ctypes.c_int()
+        ch = C._I64()
+        check(self.h, lib().ogp_get_tuning(self.h, n, ctypes.byref(w), ctypes.byref(ch)))
+        return w.value, ch.value
+
     def timing(self):
         t = C.OgpTiming()
         check(self.h, lib().ogp_last_timing(self.h, ctypes.byref(t)))

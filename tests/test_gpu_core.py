@@ -209,6 +209,37 @@ def test_identical_objects_bit_identical(ctx70):
     assert all(np.array_equal(r[0], r[i]) and np.array_equal(v[0], v[i]) for i in range(8))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_launch_shape_does_not_change_results(precision):
+    # 300 objects = four full 64-object groups and a partial one; 140 steps = three chunks
+    # with chunk_steps 64, the last one partial
+    d = scenarios.mixed(300, seed=11)
+    steps = np.minimum(d["steps"], 140)
+    with ogp.Context(degree=70, precision=precision) as c:
+        out = []
+        for w in (1, 2, 4, 8):
+            c.set_tuning(w, 64)
+            out.append(c.propagate(d["r"], d["v"], d["step"], steps))
+            assert c.get_tuning(len(steps))[0] == w
+        for r, v in out[1:]:
+            assert np.array_equal(r, out[0][0]) and np.array_equal(v, out[0][1])
+
+
+def test_autotuner_keeps_results_and_records_choice():
+    d = scenarios.config0()
+    r0, v0 = np.tile(d["r"], (10, 1)), np.tile(d["v"], (10, 1))  # 640 objects, 10 groups
+    with ogp.Context(degree=20) as c:
+        c.set_tuning(1, 16)
+        r1, v1 = c.propagate(r0, v0, 60.0, 120)
+        c.set_tuning(0, 16)  # autotune: 115 GJ steps = 8 chunks of 16, four of them trials
+        ra, va = c.propagate(r0, v0, 60.0, 120)
+        assert np.array_equal(r1, ra) and np.array_equal(v1, va)
+        w, chunk = c.get_tuning(640)
+        assert w in (1, 2, 4, 8) and chunk == 16
+        rb, vb = c.propagate(r0, v0, 60.0, 120)  # tuned table reused
+        assert np.array_equal(r1, rb) and np.array_equal(v1, vb)
+
+
 def test_errors_are_reported_not_fatal(ctx20):
     with pytest.raises(ogp.OgpError):
         ctx20.propagate([[7e6, 0, 0]], [[0, 7.5e3, 0]], -1.0, 10)
