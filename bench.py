@@ -1,0 +1,289 @@
+"""Benchmark of the propagation and conjunction hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs fp64,split,pairs,cpu,refpoint]
+
+Prints ONE JSON line. The headline `value` is FP64 object-steps/s of the mixed-regime workload:
+100,000 objects (80% LEO at 30 s, 15% MEO and 5% GEO at 300 s, spans U(6, 24) h), JGM3
+degree/order 70, inputs resident in HBM (ogp_propagate_device), startup included. One step = one
+complete propagation of that population. Side legs (same line):
+  split     50,000 LEO, 60 s, 7 days, degree 70: FP64 and split-precision object-steps/s and
+            the split path's final-position error (against the FP64 GPU path for all objects,
+            and against the oracle for a sample)
+  pairs     20,000 objects with 1,000 forced encounters, set 0 = first 2,000, 3 days at 60 s,
+            degree 20: propagation, screen (10 km) and refinement; pair-steps/s of the screen
+  cpu       the FP64 host oracle on a stratified 1,000-object sample of the headline workload,
+            timed on this host's cores; also the GPU-vs-oracle difference on that sample
+  refpoint  10,000 LEO, 30 s, 720 steps, degree 70, end to end from host arrays (the workload
+            of the reference's published H100 figure)
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "object-steps/s at deg-70 gravity: FP64 vs split-prec (≤1 m @7 d); pairs/s"
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (equal to its FP64 matrix figure)
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def terms(L):
+    return (L + 1) * (L + 2) // 2 - 3
+
+
+def flops_eval(L):
+    """Algorithmic FLOPs of one force evaluation: 25 per harmonic term (2 <= n <= L) plus the
+    per-column and per-evaluation work (FMA = 2)."""
+    return 25 * terms(L) + 15 * L + 60
+
+
+def flops_step(L, k=7):
+    """Algorithmic FLOPs per GJ8 object-step: k distinct evaluations + predictor/corrector."""
+    return k * flops_eval(L) + 228 + 33 * k
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--legs", default="fp64,split,pairs,cpu,refpoint")
+    p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def leg_fp64(ogp, scenarios, torch, args):
+    c1 = scenarios.config1()
+    n = len(c1["r"])
+    steps = c1["steps"]
+    ctx = ogp.Context(degree=70)
+    dev = torch.device("cuda:0")
+    r0 = torch.tensor(np.ascontiguousarray(c1["r"].T), device=dev)
+    v0 = torch.tensor(np.ascontiguousarray(c1["v"].T), device=dev)
+    r = torch.empty_like(r0)
+    v = torch.empty_like(v0)
+    torch.cuda.synchronize()
+
+    def run():
+        ctx.propagate_device(r0, v0, n, c1["step"], steps, r, v)
+
+    for i in range(args.warmup):
+        run()
+        log(f"fp64 warmup {i + 1}/{args.warmup}: {ctx.timing()['total_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tm = []
+    for i in range(args.steps):
+        run()
+        tm.append(ctx.timing())
+        log(f"fp64 step {i + 1}/{args.steps}: {tm[-1]['total_ms']:.1f} ms (device)")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    obj_steps = float(steps.sum())
+    ms_per_step = wall * 1e3 / args.steps
+    gj = tm[-1]["gj_object_steps"]
+    steps_ms = sum(t["steps_ms"] for t in tm) / len(tm)
+    launches = tm[-1]["step_launches"]
+    avg_launch_ms = steps_ms / launches
+    flops_launch = flops_step(70) * gj / launches
+    achieved = flops_launch / (avg_launch_ms * 1e-3) / 1e12
+    dev_ms = sum(t["total_ms"] for t in tm) / len(tm)
+    out = dict(
+        value=obj_steps / (ms_per_step * 1e-3),
+        ms_per_step=ms_per_step,
+        device=dict(total_ms=dev_ms, startup_ms=sum(t["startup_ms"] for t in tm) / len(tm),
+                    gj8_ms=steps_ms, object_steps_per_s=obj_steps / (dev_ms * 1e-3)),
+        roofline=dict(bound="valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                      frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
+                      kernel="gj8_chunk_kernel<kGravity>", flops_per_object_step=flops_step(70),
+                      avg_launch_ms=avg_launch_ms, launches_per_step=launches,
+                      object_steps_per_launch=gj / launches),
+        object_steps=obj_steps,
+        tuning={str(b): ctx.get_tuning(64 * 2 ** b)[0] for b in range(0, 11)},
+    )
+    # keep the final states of the stratified sample for the cpu leg's comparison
+    idx = np.arange(0, n, 100)
+    out["_sample"] = dict(idx=idx, r=r.cpu().numpy().T[idx].copy(), v=v.cpu().numpy().T[idx].copy(),
+                          c1=c1)
+    ctx.close()
+    return out
+
+
+def leg_split(ogp, scenarios, args, oracle_threads):
+    c2 = scenarios.config2()
+    n = len(c2["r"])
+    res = {}
+    for prec in ("fp64", "split"):
+        with ogp.Context(degree=70, precision=prec) as ctx:
+            # tuning pass on a 1-day span, then the timed 7-day run
+            ctx.propagate(c2["r"], c2["v"], c2["step"], np.minimum(c2["steps"], 1440))
+            t0 = time.perf_counter()
+            r, v = ctx.propagate(c2["r"], c2["v"], c2["step"], c2["steps"])
+            wall = time.perf_counter() - t0
+            t = ctx.timing()
+            res[prec] = dict(r=r, v=v, dev_ms=t["total_ms"], wall=wall,
+                             rate=float(c2["steps"].sum()) / (t["total_ms"] * 1e-3),
+                             gj8_ms=t["steps_ms"], gj=t["gj_object_steps"],
+                             launches=t["step_launches"])
+            log(f"split leg {prec}: {t['total_ms']:.0f} ms device, {res[prec]['rate']:.3e} obj-steps/s")
+    err = np.linalg.norm(res["split"]["r"] - res["fp64"]["r"], axis=1)
+    out = dict(workload="50,000 LEO, 60 s, 10,080 steps (7 d), JGM3 70x70",
+               fp64_object_steps_per_s=res["fp64"]["rate"],
+               split_object_steps_per_s=res["split"]["rate"],
+               speedup=res["split"]["rate"] / res["fp64"]["rate"],
+               split_vs_fp64_gpu_m=dict(max=float(err.max()), p99=float(np.percentile(err, 99)),
+                                        median=float(np.median(err))))
+    # split-path roofline: FP32 terms n >= 3 against the FP32 peak, the rest against FP64
+    L, ns = 70, 2
+    f32 = 7 * 25 * (terms(L) - terms(ns))
+    f64 = flops_step(L) - f32
+    s = res["split"]
+    t_launch = s["gj8_ms"] / s["launches"] * 1e-3
+    per_launch = s["gj"] / s["launches"]
+    bound_s = (f32 / (PEAK_FP32_TFLOPS * 1e12) + f64 / (PEAK_FP64_TFLOPS * 1e12)) * per_launch
+    out["split_roofline"] = dict(frac=bound_s / t_launch, fp32_flops_per_object_step=f32,
+                                 fp64_flops_per_object_step=f64, avg_launch_ms=t_launch * 1e3)
+    if oracle_threads:
+        from oracle import pyoracle as po
+        o = po.Oracle(degree=70)
+        k = np.arange(0, n, n // 32)[:32]
+        t0 = time.perf_counter()
+        ro, _ = o.propagate(po.GRAVITY, c2["r"][k], c2["v"][k], c2["step"][k], c2["steps"][k],
+                            threads=oracle_threads)
+        log(f"split leg oracle sample: {time.perf_counter() - t0:.1f} s")
+        e_s = np.linalg.norm(res["split"]["r"][k] - ro, axis=1)
+        e_d = np.linalg.norm(res["fp64"]["r"][k] - ro, axis=1)
+        out["vs_oracle_sample_m"] = dict(objects=len(k), split_max=float(e_s.max()),
+                                         fp64_max=float(e_d.max()))
+    return out
+
+
+def leg_pairs(ogp, scenarios):
+    c4 = scenarios.config4()
+    with ogp.Context(degree=20) as ctx:
+        # warm-up on a short window (node tables, tuning, allocations)
+        ca = ogp.ClosestApproach(ctx, c4["step"])
+        ca.initialize(c4["r"], c4["v"], c4["sets"])
+        ca.propagate(64)
+        ca.find_closest(threshold=10e3)
+        ca = ogp.ClosestApproach(ctx, c4["step"])
+        ca.initialize(c4["r"], c4["v"], c4["sets"])
+        t0 = time.perf_counter()
+        ca.propagate(c4["steps"])
+        t1 = time.perf_counter()
+        res = ca.find_closest(threshold=10e3)
+        t2 = time.perf_counter()
+        t = ca.timing()
+    pair_steps = t["pair_steps"]
+    return dict(
+        workload="20,000 objects (1,000 forced pairs), set 0 = first 2,000, 3 d at 60 s, "
+                 "JGM3 20x20, 10 km screen",
+        pairs=int(t["pairs"]), pair_steps=pair_steps, candidates=int(t["candidates"]),
+        results=len(res),
+        screen_ms=t["screen_ms"], refine_ms=t["refine_ms"], propagate_ms=t["propagate_ms"],
+        pair_steps_per_s=pair_steps / (t["screen_ms"] * 1e-3),
+        pairs_per_s_end_to_end=t["pairs"] / (t2 - t0),
+        propagate_wall_s=t1 - t0, find_wall_s=t2 - t1,
+        screen_roofline=dict(bound="valu",
+                             achieved=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12,
+                             peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                             frac=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS))
+
+
+def leg_cpu(sample, threads):
+    """FP64 oracle on every 100th object of the headline workload (stratified over regimes)."""
+    from oracle import pyoracle as po
+    c1, idx = sample["c1"], sample["idx"]
+    o = po.Oracle(degree=70)
+    t0 = time.perf_counter()
+    r, v = o.propagate(po.GRAVITY, c1["r"][idx], c1["v"][idx], c1["step"][idx], c1["steps"][idx],
+                       threads=threads)
+    dt = time.perf_counter() - t0
+    work = float(c1["steps"][idx].sum())
+    dr = np.linalg.norm(r - sample["r"], axis=1)
+    dv = np.linalg.norm(v - sample["v"], axis=1)
+    return dict(value=work / dt, unit="object-steps/s", cores=threads, kind="port",
+                sample=f"{len(idx)} objects (every 100th of the 100,000-object mixed workload), "
+                       f"{int(work)} object-steps, FP64 oracle, OpenMP over objects, {dt:.1f} s",
+                cpu_model=_cpu_model()), dict(objects=len(idx), max_dr_m=float(dr.max()),
+                                             max_dv_mps=float(dv.max()))
+
+
+def leg_refpoint(ogp, scenarios):
+    r0, v0 = scenarios.leo(10_000, seed=1)
+    t0 = time.perf_counter()
+    with ogp.Context(degree=70) as ctx:
+        ctx.propagate(r0, v0, 30.0, 720)
+        t = ctx.timing()
+    wall = time.perf_counter() - t0
+    return dict(workload="10,000 LEO, 30 s, 720 steps (6 h), JGM3 70x70, host arrays, "
+                         "context creation included (first use of the wave tuner)",
+                wall_s=wall, object_steps_per_s_end_to_end=7.2e6 / wall,
+                device_ms=t["total_ms"], published_h100_wall_s=18.0,
+                published_h100_object_steps_per_s=3.99e5)
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (objects shard "
+              "across GPUs with no exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    legs = set(args.legs.split(","))
+    import torch
+    if not torch.cuda.is_available():
+        raise SystemExit("no HIP device visible")
+    import opengpop_amd as ogp
+    from opengpop_amd import scenarios
+    ogp.lib()  # fails loudly when libogp.so is missing
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+
+    line = dict(metric=METRIC, value=None, unit="object-steps/s", n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=None, higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="f64", data="synthetic",
+                config=dict(workload="100,000 mixed objects (80% LEO at 30 s, 15% MEO and 5% GEO "
+                                     "at 300 s), spans U(6,24) h, JGM3 70x70, FP64, inputs in HBM",
+                            objects=100_000, degree=70, object_steps=None))
+    sample = None
+    if "fp64" in legs:
+        f = leg_fp64(ogp, scenarios, torch, args)
+        sample = f.pop("_sample")
+        line.update(value=f["value"], ms_per_step=f["ms_per_step"], roofline=f["roofline"])
+        line["config"]["object_steps"] = f["object_steps"]
+        line["device"] = f["device"]
+        line["wave_tuning"] = f["tuning"]
+    if "split" in legs:
+        line["split"] = leg_split(ogp, scenarios, args, threads if "cpu" in legs else 0)
+    if "pairs" in legs:
+        line["pairs"] = leg_pairs(ogp, scenarios)
+        log(f"pairs leg: {line['pairs']['pair_steps_per_s']:.3e} pair-steps/s")
+    if "cpu" in legs and sample is not None:
+        line["cpu_baseline"], line["parity_sample"] = leg_cpu(sample, threads)
+        log(f"cpu leg: {line['cpu_baseline']['value']:.3e} object-steps/s on {threads} threads")
+    if "refpoint" in legs:
+        line["reference_point"] = leg_refpoint(ogp, scenarios)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

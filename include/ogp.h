@@ -157,7 +157,8 @@ int ogp_kat_finalize_correction(ogp_context* ctx, double* r3, double* v3, const 
 /* ---- conjunction (ClosestApproach, closest_approach.cuh:36-88) --------------------------- */
 
 /* ClosestApproach::Initialize: set ids must start at 0, be non-decreasing with steps of +1
- * (partition_parameters.cuh:94-135); at least two sets. */
+ * (partition_parameters.cuh:94-135); at least two sets. A context holds one search: calling
+ * it again starts a new one (objects, trajectory and results are replaced). */
 int ogp_conj_init(ogp_context* ctx, const double* r, const double* v, const uint32_t* set_ids,
     int64_t n, double step_s);
 /* ClosestApproach::Propagate: continues from the current state; may be called repeatedly. */

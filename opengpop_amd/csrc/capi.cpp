@@ -994,9 +994,15 @@ int ogp_conj_init(ogp_context* ctx, const double* r, const double* v, const uint
     if (!ctx)
         return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
     return guarded(ctx, [&] {
-        // closest_approach.cu:50-81 and partition_parameters.cuh:94-135
-        if (ctx->cj_n != 0)
-            throw ApiError(OGP_ERR_STATE, "ClosestApproach already initialised");
+        // closest_approach.cu:50-81 and partition_parameters.cuh:94-135. A context holds one
+        // search; initialising again replaces it (objects, trajectory and results).
+        ctx->cj_n = 0;
+        ctx->cj_steps = 0;
+        ctx->res_a.clear();
+        ctx->res_b.clear();
+        ctx->res_k.clear();
+        ctx->res_out.clear();
+        ctx->cj_timing = ogp_conj_timing {};
         arg_check(n > 0 && r && v && set_ids, "positions array is empty");
         arg_check(std::isfinite(step_s) && step_s > 0, "step size must be positive");
         arg_check(set_ids[0] == 0, "set ids must start at 0");
@@ -1027,6 +1033,8 @@ int ogp_conj_init(ogp_context* ctx, const double* r, const double* v, const uint
         ctx->cj_n = n;
         ctx->cj_h = step_s;
         ctx->cj_steps = 0;
+        // capacity of the existing trajectory store, in steps of this object count
+        ctx->cj_cap_steps = (int64_t)(ctx->cj_traj.bytes / (sizeof(double) * 9 * (size_t)n));
     });
 }
 

@@ -93,6 +93,17 @@ class Context:
                                           ptr(v)))
         return r, v
 
+    def propagate_device(self, r0_soa, v0_soa, n, step, steps, r_soa, v_soa):
+        """ogp_propagate_device on device-resident SoA arrays (x[n], y[n], z[n] blocks).
+        Array arguments are device addresses: ints or objects with data_ptr() (torch tensors).
+        step and steps are host values (scalars or per-object arrays)."""
+        addr = [a.data_ptr() if hasattr(a, "data_ptr") else int(a)
+                for a in (r0_soa, v0_soa, r_soa, v_soa)]
+        h = np.ascontiguousarray(np.broadcast_to(np.asarray(step, np.float64), (n,)))
+        ns = np.ascontiguousarray(np.broadcast_to(np.asarray(steps, np.int64), (n,)))
+        check(self.h, lib().ogp_propagate_device(self.h, addr[0], addr[1], n, ptr(h), ptr(ns),
+                                                 addr[2], addr[3]))
+
     def set_tuning(self, waves=0, chunk_steps=0):
         check(self.h, lib().ogp_set_tuning(self.h, waves, chunk_steps))
 

@@ -1,0 +1,156 @@
+"""GPU parity of the conjunction path (ClosestApproach, closest_approach.cu:47-210) against the
+oracle: the reference's analytic encounters, the sampled screen bit-exact on the same stored
+trajectory, the Hermite/secant refinement, and the candidate pair set of a forced-encounter
+population propagated independently on the GPU and by the oracle.
+"""
+import numpy as np
+import pytest
+
+import opengpop_amd as ogp
+from opengpop_amd import scenarios
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_phases(case):
+    with ogp.Context(force=ogp.FORCE_NONE) as c:
+        ca = ogp.ClosestApproach(c, case["h"])
+        ca.initialize(case["pos"], case["vel"], case["sets"])
+        out = []
+        for n in case["phases"]:
+            ca.propagate(n)
+            out.append(ca.find_closest())
+        return out
+
+
+def test_analytic_encounters(golden):
+    # closest_approach_test.cu:60-230: straight-line encounters with no acceleration
+    for case in golden["encounters"]["cases"]:
+        res = _gpu_phases(case)
+        tol = case["tol"]
+        if "expect_phases" in case:
+            for out, (dist, tim) in zip(res, case["expect_phases"]):
+                ap = out.get(0, 1)
+                assert abs(ap["distance"] - dist) <= tol, case["name"]
+                if tim is not None:
+                    assert abs(ap["time"] - tim) <= tol, case["name"]
+            continue
+        for a, b, dist, tim in case["expect"]:
+            ap = res[-1].get(a, b)
+            if tol == 0:
+                assert ap["distance"] == dist, case["name"]
+            else:
+                assert abs(ap["distance"] - dist) <= tol, case["name"]
+            if tim is not None:
+                assert abs(ap["time"] - tim) <= tol, case["name"]
+            # Get(j, i) swaps the positions (types.cuh:153-169)
+            sw = res[-1].get(b, a)
+            assert sw["distance"] == ap["distance"]
+            assert np.array_equal(sw["pos_a"], ap["pos_b"])
+
+
+def test_get_invalid_and_same_object():
+    case = dict(h=0.5, phases=[10], pos=[[0, 0, 0], [10, 0, 0], [20, 0, 0]],
+                vel=[[1, 0, 0], [-1, 0, 0], [0, 0, 0]], sets=[0, 1, 1])
+    out = _gpu_phases(case)[-1]
+    assert out.get(1, 1)["distance"] == 0.0
+    assert out.get(1, 2)["distance"] == -1.0  # same set: not a pair
+    assert len(out) == 2
+
+
+def test_reinitialise_replaces_the_search():
+    # a second initialise with more objects starts a new search in the same context
+    small = _population(128, seed=1)
+    big = _population(640, seed=2)
+    with ogp.Context(degree=20) as c:
+        ca = ogp.ClosestApproach(c, 60.0)
+        ca.initialize(*small)
+        ca.propagate(40)
+        ca.find_closest()
+        ca.initialize(*big)
+        ca.propagate(40)
+        again = ca.screen()
+    with ogp.Context(degree=20) as c:
+        ca = ogp.ClosestApproach(c, 60.0)
+        ca.initialize(*big)
+        ca.propagate(40)
+        fresh = ca.screen()
+    assert np.array_equal(again[0], fresh[0]) and np.array_equal(again[1], fresh[1])
+
+
+def _population(n=384, seed=7):
+    r, v = scenarios.leo(n, seed=seed)
+    sets = np.repeat(np.arange(3, dtype=np.uint32), [n // 4, n // 4, n - n // 2])
+    return r, v, sets
+
+
+@pytest.fixture(scope="module")
+def screened():
+    r, v, sets = _population()
+    with ogp.Context(degree=20) as c:
+        ca = ogp.ClosestApproach(c, 60.0)
+        ca.initialize(r, v, sets)
+        ca.propagate(150)
+        ca.propagate(90)  # second phase continues the same trajectory
+        traj = ca.trajectory()
+        mind, mink = ca.screen()
+        res = ca.find_closest()
+        thr = ca.find_closest(threshold=500e3)
+        yield dict(r=r, v=v, sets=sets, traj=traj, mind=mind, mink=mink, res=res, thr=thr)
+
+
+def test_screen_bit_exact_on_same_trajectory(screened, po):
+    o = po.Oracle(degree=20)
+    mind, mink = o.screen(screened["traj"], screened["sets"])
+    assert screened["traj"].shape[0] == 240
+    assert np.array_equal(mind, screened["mind"])
+    assert np.array_equal(mink, screened["mink"])
+
+
+def test_refine_matches_oracle_on_same_trajectory(screened, po):
+    o = po.Oracle(degree=20)
+    res, sets, traj = screened["res"], screened["sets"], screened["traj"]
+    assert len(res) == len(screened["mind"])
+    rng = np.random.default_rng(3)
+    for q in rng.choice(len(res), 400, replace=False):
+        a, b = int(res.a[q]), int(res.b[q])
+        ok, pa, pb = o.partition_pair(sets, int(o.partition_idx(sets, a, b)))
+        assert ok and (pa, pb) == (a, b)
+        ref = o.refine(traj, 60.0, a, b, int(res.step[q]), float(screened["mind"][
+            o.partition_idx(sets, a, b)]))
+        assert abs(res.distance[q] - ref["distance"]) <= 1e-6
+        assert abs(res.time[q] - ref["time"]) <= 1e-6
+        if 0 < res.step[q] < traj.shape[0] - 1:
+            assert np.abs(res.pos_a[q] - ref["pos_a"]).max() <= 1e-6
+
+
+def test_threshold_selects_sampled_minimum_below(screened):
+    thr, mind = screened["thr"], screened["mind"]
+    assert len(thr) == int((mind < 500e3).sum())
+    assert np.all(np.diff(thr.a.astype(np.int64) * 10**6 + thr.b) > 0)  # sorted, unique
+
+
+def test_candidate_pairs_bit_exact_vs_oracle_end_to_end(po):
+    # forced encounters; GPU and oracle each propagate, screen at 10 km and refine
+    d = scenarios.config4(n_random=360, n_pairs=40, n_setA=80, days=0.25)
+    with ogp.Context(degree=20) as c:
+        ca = ogp.ClosestApproach(c, d["step"])
+        ca.initialize(d["r"], d["v"], d["sets"])
+        ca.propagate(d["steps"])
+        gpu = ca.find_closest(threshold=10e3)
+    o = po.Oracle(degree=20)
+    _, _, traj = o.propagate(po.GRAVITY, d["r"], d["v"], d["step"], d["steps"], trajectory=True)
+    mind, mink = o.screen(traj, d["sets"])
+    # no pair sits within 1 m of the threshold, so the propagation difference (< 1 mm) cannot
+    # move a pair across it
+    assert np.all(np.abs(mind - 10e3) > 1.0)
+    cand = np.nonzero(mind < 10e3)[0]
+    pairs = [o.partition_pair(d["sets"], int(q))[1:] for q in cand]  # pair order = (a, b) order
+    assert len(pairs) >= 15
+    assert [(int(a), int(b)) for a, b in zip(gpu.a, gpu.b)] == pairs
+    assert np.array_equal(gpu.step, mink[cand])
+    for (a, b), q in zip(pairs, cand):
+        ref = o.refine(traj, d["step"], a, b, int(mink[q]), float(mind[q]))
+        g = gpu.get(a, b)
+        assert abs(g["distance"] - ref["distance"]) <= 1.0
+        assert abs(g["time"] - ref["time"]) <= 1e-3
