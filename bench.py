@@ -233,6 +233,20 @@ def leg_refpoint(ogp, scenarios):
                 published_h100_object_steps_per_s=3.99e5)
 
 
+def leg_copy(torch):
+    """A plain device copy of known size (1 GiB read + 1 GiB written): the calibration dispatch
+    for the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of the profiler runs."""
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dict(bytes_read=a.numel() * 8, bytes_written=b.numel() * 8,
+                gbs=2 * a.numel() * 8 / dt / 1e9)
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -282,6 +296,8 @@ def main():
         log(f"cpu leg: {line['cpu_baseline']['value']:.3e} object-steps/s on {threads} threads")
     if "refpoint" in legs:
         line["reference_point"] = leg_refpoint(ogp, scenarios)
+    if "copy" in legs:
+        line["copy_calibration"] = leg_copy(torch)
     print(json.dumps(line), flush=True)
 
 

@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(64 * W) startup_kernel(ForceCtx f, int rk8_var
 // (prediction.cu:49-74, dev_functions.cu:20-50) and the corrector (gpu_propagator.cu:86-119,
 // dev_functions.cu:36-69). The acceleration history is a ring of 9 slots in HBM.
 #ifndef OGP_CHUNK_WAVES_PER_EU
-#define OGP_CHUNK_WAVES_PER_EU 3 // resident waves per SIMD the register allocation aims for
+#define OGP_CHUNK_WAVES_PER_EU 4 // resident waves per SIMD the register allocation aims for
 #endif
 template <int KIND, int W>
 __global__ void __launch_bounds__(64 * W)
