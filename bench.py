@@ -48,6 +48,19 @@ def flops_step(L, k=7):
     return k * flops_eval(L) + 228 + 33 * k
 
 
+def measured_traffic_per_object_step():
+    """HBM bytes per GJ8 object-step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated against a copy of known size), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "gj8_chunk_mixed100k_deg70_hbm_bytes.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_object_step:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -98,16 +111,20 @@ def leg_fp64(ogp, scenarios, torch, args):
     flops_launch = flops_step(70) * gj / launches
     achieved = flops_launch / (avg_launch_ms * 1e-3) / 1e12
     dev_ms = sum(t["total_ms"] for t in tm) / len(tm)
+    bps = measured_traffic_per_object_step()
+    traffic = None if bps is None else bps * gj / launches
     out = dict(
         value=obj_steps / (ms_per_step * 1e-3),
         ms_per_step=ms_per_step,
         device=dict(total_ms=dev_ms, startup_ms=sum(t["startup_ms"] for t in tm) / len(tm),
                     gj8_ms=steps_ms, object_steps_per_s=obj_steps / (dev_ms * 1e-3)),
         roofline=dict(bound="valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                      frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
+                      frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic,
                       kernel="gj8_chunk_kernel<kGravity>", flops_per_object_step=flops_step(70),
                       avg_launch_ms=avg_launch_ms, launches_per_step=launches,
-                      object_steps_per_launch=gj / launches),
+                      object_steps_per_launch=gj / launches,
+                      traffic_unit="bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+                      algorithmic_bytes_per_launch=9.75 * gj / launches),
         object_steps=obj_steps,
         tuning={str(b): ctx.get_tuning(64 * 2 ** b)[0] for b in range(0, 11)},
     )
