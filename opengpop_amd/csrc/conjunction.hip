@@ -24,11 +24,47 @@
 
 namespace ogp {
 
+#ifndef OGP_SCREEN_PREFETCH
+#define OGP_SCREEN_PREFETCH 1
+#endif
+
 namespace {
 constexpr int TA = 64, TB = 64, PA = 4, PB = 4, STAGE = 8, THREADS = 256;
+// elements of one stage (STAGE steps x 3 components x (TA + TB) objects) per thread
+constexpr int STAGE_ELEMS = STAGE * 3 * (TA + TB) / THREADS;
+static_assert(STAGE * 3 * (TA + TB) % THREADS == 0, "stage must divide over the block");
+// d2 < best * kShrink guarantees sqrt(d2) < sqrt(best) for correctly rounded square roots
+// (the two roots then differ by more than one rounding each), so the roots are only formed for
+// samples within a few ulps of the running minimum.
+constexpr double kShrink = 1.0 - 0x1p-50;
+
+// Stage element e -> (tile, step, component, object) and its trajectory value (0 past the end).
+__device__ __forceinline__ double stage_load(const double* __restrict__ traj, int64_t N,
+    int64_t nsteps, int64_t k0, int e, int64_t ta0, int64_t tb0, int64_t alim, int64_t blim)
+{
+    const int which = e / (STAGE * 3 * TA); // 0 = A, 1 = B (TA == TB)
+    const int rem = e % (STAGE * 3 * TA);
+    const int st = rem / (3 * TA);
+    const int c = (rem / TA) % 3;
+    const int o = rem % TA;
+    const int64_t k = k0 + st;
+    const int64_t obj = (which == 0 ? ta0 : tb0) + o;
+    if (k < nsteps && obj < (which == 0 ? alim : blim))
+        return traj[(k * 9 + c) * N + obj];
+    return 0.0;
+}
 }
 
-__global__ void __launch_bounds__(THREADS) screen_kernel(const double* __restrict__ traj,
+// One block screens a TA x TB tile of pairs over the whole time axis. Thread (ty, tx) owns A
+// objects ty*PA .. ty*PA+3 and B objects {2tx, 2tx+1, 32+2tx, 32+2tx+1}: neighbouring lanes read
+// neighbouring 16-byte pairs of the staged B positions (no LDS bank conflicts). The next stage's
+// positions are loaded into registers while the current stage is screened.
+#ifndef OGP_SCREEN_WAVES_PER_EU
+#define OGP_SCREEN_WAVES_PER_EU 2
+#endif
+__global__ void __launch_bounds__(THREADS)
+__attribute__((amdgpu_waves_per_eu(OGP_SCREEN_WAVES_PER_EU)))
+screen_kernel(const double* __restrict__ traj,
     int64_t N, int64_t nsteps, int64_t a0, int64_t na, int64_t b0, int64_t nb, int64_t pair0,
     int64_t row_len, double threshold, double* __restrict__ min_dist, int32_t* __restrict__ min_step,
     uint32_t* __restrict__ cand_count, int64_t cand_cap, uint32_t* __restrict__ cand_a,
@@ -41,6 +77,11 @@ __global__ void __launch_bounds__(THREADS) screen_kernel(const double* __restric
     const int ty = tid / (TB / PB); // 16 rows of A sub-tiles
     const int64_t ta0 = a0 + (int64_t)blockIdx.y * TA;
     const int64_t tb0 = b0 + (int64_t)blockIdx.x * TB;
+    const int64_t alim = a0 + na, blim = b0 + nb;
+    int bcol[PB];
+#pragma unroll
+    for (int j = 0; j < PB; ++j)
+        bcol[j] = (j / 2) * 32 + 2 * tx + (j % 2);
     double best[PA][PB];
     int32_t bk[PA][PB];
 #pragma unroll
@@ -50,26 +91,34 @@ __global__ void __launch_bounds__(THREADS) screen_kernel(const double* __restric
             best[i][j] = __builtin_inf();
             bk[i][j] = -1;
         }
+    double pre[STAGE_ELEMS];
+#pragma unroll
+    for (int q = 0; q < STAGE_ELEMS; ++q)
+        pre[q] = stage_load(traj, N, nsteps, 0, tid + q * THREADS, ta0, tb0, alim, blim);
     for (int64_t k0 = 0; k0 < nsteps; k0 += STAGE) {
-        // stage positions of STAGE steps for both tiles: STAGE * 3 * (TA + TB) doubles
-        for (int e = tid; e < STAGE * 3 * (TA + TB); e += THREADS) {
-            const int which = e / (STAGE * 3 * TA); // 0 = A, 1 = B (TA == TB)
+        __syncthreads(); // the previous stage has been screened by every thread
+#if !OGP_SCREEN_PREFETCH
+        if (k0 > 0) {
+#pragma unroll
+            for (int q = 0; q < STAGE_ELEMS; ++q)
+                pre[q] = stage_load(traj, N, nsteps, k0, tid + q * THREADS, ta0, tb0, alim, blim);
+        }
+#endif
+#pragma unroll
+        for (int q = 0; q < STAGE_ELEMS; ++q) {
+            const int e = tid + q * THREADS;
+            const int which = e / (STAGE * 3 * TA);
             const int rem = e % (STAGE * 3 * TA);
-            const int st = rem / (3 * TA);
-            const int c = (rem / TA) % 3;
-            const int o = rem % TA;
-            const int64_t k = k0 + st;
-            const int64_t obj = (which == 0 ? ta0 : tb0) + o;
-            const int64_t lim = which == 0 ? a0 + na : b0 + nb;
-            double val = 0.0;
-            if (k < nsteps && obj < lim)
-                val = traj[(k * 9 + c) * N + obj];
-            if (which == 0)
-                sA[st][c][o] = val;
-            else
-                sB[st][c][o] = val;
+            double* dst = which == 0 ? &sA[0][0][0] : &sB[0][0][0];
+            dst[rem] = pre[q];
         }
         __syncthreads();
+        if (OGP_SCREEN_PREFETCH && k0 + STAGE < nsteps) { // flies while this stage is screened
+#pragma unroll
+            for (int q = 0; q < STAGE_ELEMS; ++q)
+                pre[q] = stage_load(traj, N, nsteps, k0 + STAGE, tid + q * THREADS, ta0, tb0,
+                    alim, blim);
+        }
         const int smax = (int)((nsteps - k0) < STAGE ? (nsteps - k0) : STAGE);
         for (int st = 0; st < smax; ++st) {
             double ax[PA], ay[PA], az[PA], bx[PB], by[PB], bz[PB];
@@ -81,11 +130,15 @@ __global__ void __launch_bounds__(THREADS) screen_kernel(const double* __restric
             }
 #pragma unroll
             for (int j = 0; j < PB; ++j) {
-                bx[j] = sB[st][0][tx * PB + j];
-                by[j] = sB[st][1][tx * PB + j];
-                bz[j] = sB[st][2][tx * PB + j];
+                bx[j] = sB[st][0][bcol[j]];
+                by[j] = sB[st][1][bcol[j]];
+                bz[j] = sB[st][2][bcol[j]];
             }
             const int32_t k = (int32_t)(k0 + st);
+            // Branch-free update: a sample below best * kShrink is an improvement under the
+            // reference's rule; one within a few ulps of the minimum is flagged and the step
+            // is decided exactly below (updates already made are no-ops there).
+            bool unsure = false;
 #pragma unroll
             for (int i = 0; i < PA; ++i)
 #pragma unroll
@@ -94,23 +147,35 @@ __global__ void __launch_bounds__(THREADS) screen_kernel(const double* __restric
                     const double dy = ay[i] - by[j];
                     const double dz = az[i] - bz[j];
                     const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                    if (d2 < best[i][j]) {
-                        if (bk[i][j] < 0 || __builtin_sqrt(d2) < __builtin_sqrt(best[i][j])) {
+                    const bool take = d2 < best[i][j] * kShrink;
+                    unsure |= (d2 < best[i][j]) & !take;
+                    best[i][j] = take ? d2 : best[i][j];
+                    bk[i][j] = take ? k : bk[i][j];
+                }
+            if (unsure) {
+#pragma unroll
+                for (int i = 0; i < PA; ++i)
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) {
+                        const double dx = ax[i] - bx[j];
+                        const double dy = ay[i] - by[j];
+                        const double dz = az[i] - bz[j];
+                        const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                        if (d2 < best[i][j] && __builtin_sqrt(d2) < __builtin_sqrt(best[i][j])) {
                             best[i][j] = d2;
                             bk[i][j] = k;
                         }
                     }
-                }
+            }
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < PA; ++i)
 #pragma unroll
         for (int j = 0; j < PB; ++j) {
             const int64_t ia = ta0 + ty * PA + i;
-            const int64_t jb = tb0 + tx * PB + j;
-            if (ia >= a0 + na || jb >= b0 + nb)
+            const int64_t jb = tb0 + bcol[j];
+            if (ia >= alim || jb >= blim)
                 continue;
             // pairs that never took a sample keep the reference's initial DBL_MAX
             const double d = bk[i][j] < 0 ? 1.7976931348623157e308 : __builtin_sqrt(best[i][j]);
