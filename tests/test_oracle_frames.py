@@ -91,6 +91,49 @@ def test_eop_interpolation_and_leap_seconds(oracle70):
     assert oracle70.leap_seconds(41318.0) == 10.0
 
 
+def _nodes_2000_2026(oracle70):
+    """Time nodes on several dates 2000-2026 (inside the EOP table), 7 h apart within a date."""
+    out = []
+    for ymd in [(2000, 1, 1), (2006, 7, 15), (2013, 3, 20), (2019, 11, 2), (2025, 1, 1),
+                (2026, 6, 30)]:
+        offs = np.arange(0, 86400.0, 7 * 3600.0)
+        M, mjd = oracle70.time_nodes(ymd + (0, 0), 0.0, offs)
+        out += list(zip(M, mjd))
+    return out
+
+
+def test_time_node_x_axis_sits_at_earth_rotation_angle(oracle70):
+    # The equinox-based chain Q Rz(-GAST) W must equal the CIO-based Q' Rz(-ERA) W: the image
+    # of the ITRS x axis lies at the Earth Rotation Angle (IERS Conventions 2010 eq. 5.15) in
+    # the GCRS equator, up to the CIO locator s and second-order CIP terms (< 1e-5 rad). A
+    # GAST sign error, a wrong UT1, a missing equation of the equinoxes or a reversed product
+    # order each move it by orders of magnitude more.
+    for m, mjd_utc in _nodes_2000_2026(oracle70):
+        A = m.reshape(3, 3).T
+        mjd_ut1 = mjd_utc + oracle70.eop(mjd_utc)[3] / 86400.0
+        du = mjd_ut1 - 51544.5
+        era = 2 * np.pi * ((0.7790572732640 + 0.00273781191135448 * du + du % 1.0) % 1.0)
+        ang = np.arctan2(A[1, 0], A[0, 0])
+        d = (ang - era + np.pi) % (2 * np.pi) - np.pi
+        assert abs(d) < 1e-5, (mjd_utc, d)
+
+
+def test_time_node_pole_is_the_cip(oracle70):
+    # The image of the ITRS z axis is the CIP (plus polar motion, ~1e-6 rad): its GCRS X, Y
+    # follow the IAU 2006/2000A series (IERS Conventions 2010 eq. 5.16), here the polynomial
+    # part and the two 18.6-year nutation terms, good to about 1 arcsec.
+    asec = np.pi / 180.0 / 3600.0
+    for m, mjd_utc in _nodes_2000_2026(oracle70):
+        A = m.reshape(3, 3).T
+        mjd_tt = mjd_utc + (oracle70.leap_seconds(mjd_utc) + 32.184) / 86400.0
+        t = (mjd_tt - 51544.5) / 36525.0
+        om = (450160.398036 - 6962890.5431 * t) * asec  # mean longitude of the lunar node
+        X = (-0.016617 + 2004.191898 * t - 0.4297829 * t**2 - 6.8443 * np.sin(om)) * asec
+        Y = (-0.006951 - 0.025896 * t - 22.4072747 * t**2 + 9.2052 * np.cos(om)) * asec
+        assert abs(A[0, 2] - X) < 1e-5 and abs(A[1, 2] - Y) < 1e-5, (mjd_utc, A[0, 2] - X,
+                                                                     A[1, 2] - Y)
+
+
 def test_time_node_matrices_are_rotations(oracle70):
     offs = np.arange(0, 86400.0 * 2, 3600.0)
     M, mjd = oracle70.time_nodes((2025, 1, 1, 0, 0), 0.0, offs)
