@@ -1,0 +1,141 @@
+"""The `opengpop` command-line tool: the reference's `propagate` / `closest-approach` subcommands
+(exec/main.cpp:52-107, exec/runner.cu:11-119) with the reference JSON schema (runner.cuh:8-47)
+written like nlohmann::json::dump(4).
+
+CPU tests cover the number formatting and the argument errors; GPU tests run both subcommands on
+inputs in the reference's format and check the results against the oracle.
+"""
+import json
+import os
+import random
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "opengpop_amd", "opengpop")
+DATA = os.path.join(ROOT, "data")
+
+
+def _run(args, **kw):
+    return subprocess.run([CLI] + args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                          timeout=600, **kw)
+
+
+def _nlohmann(x):
+    """nlohmann::json's number format (detail::to_chars / format_buffer, min_exp = -4,
+    max_exp = 15) applied to the shortest round-trip digits."""
+    if x == 0.0:
+        return "-0.0" if struct.pack(">d", x)[0] & 0x80 else "0.0"
+    sign = "-" if x < 0 else ""
+    # shortest round-trip digits, then the decimal exponent n with value = 0.d1d2..dk * 10^n
+    for p in range(1, 18):
+        s = f"{abs(x):.{p - 1}e}"
+        if float(s) == abs(x):
+            break
+    mant, ex = s.split("e")
+    ds = mant.replace(".", "")
+    ds = ds.rstrip("0") or "0"
+    k = len(ds)
+    n = int(ex) + 1
+    if k <= n <= 15:
+        return sign + ds + "0" * (n - k) + ".0"
+    if 0 < n <= 15:
+        return sign + ds[:n] + "." + ds[n:]
+    if -4 < n <= 0:
+        return sign + "0." + "0" * (-n) + ds
+    e = n - 1
+    es = ("+" if e >= 0 else "-") + f"{abs(e):02d}"
+    return sign + (ds if k == 1 else ds[0] + "." + ds[1:]) + "e" + es
+
+
+def test_number_format_matches_nlohmann_rules():
+    rng = random.Random(5)
+    xs = [0.0, -0.0, 1.0, 100.0, -6878136.3, 0.1, 1e-5, 1e-4, 1.5e16, 123456789012345.0, 1e15,
+          5e-324, 1.7976931348623157e308, 7612.6081732238688, 0.0025, 3e21, 46800.0, 42.0]
+    for _ in range(400):
+        xs.append(rng.uniform(-1, 1) * 10 ** rng.uniform(-12, 22))
+    out = _run(["--format-double"] + [repr(x) for x in xs])
+    assert out.returncode == 0
+    got = out.stdout.split()
+    assert got == [_nlohmann(x) for x in xs]
+    assert all(float(g) == x for g, x in zip(got, xs))  # round trip
+
+
+def test_argument_errors_exit_with_usage():
+    r = _run(["propagate"])
+    assert r.returncode == 2 and "--input is required" in r.stderr
+    r = _run(["bogus", "-i", "x"])
+    assert r.returncode == 2 and "subcommand" in r.stderr
+    r = _run(["propagate", "-i", "x", "--precision", "fp16"])
+    assert r.returncode == 2
+
+
+def _check_dump4_layout(text):
+    # four-space indentation, sorted keys at every level, no trailing newline issues
+    obj = json.loads(text)
+    again = json.dumps(obj, indent=4, sort_keys=True)
+    assert text.splitlines()[1].startswith("    \"")
+    keys = [l.strip().split('"')[1] for l in text.splitlines() if l.startswith('    "')]
+    assert keys == sorted(keys)
+    return obj, again
+
+
+@pytest.mark.gpu
+def test_propagate_reference_sample_matches_oracle(tmp_path, po):
+    # the reference's own sample input (data/sat_data.json): 46,800 s from 2023-04-29 22:59:42
+    out = tmp_path / "out.json"
+    r = _run(["propagate", "-i", os.path.join(DATA, "sat_data.json"), "-o", str(out)])
+    assert r.returncode == 0, r.stderr
+    assert "Saved to:" in r.stdout
+    text = out.read_text()
+    obj, _ = _check_dump4_layout(text)
+    inp = json.load(open(os.path.join(DATA, "sat_data.json")))
+    assert obj["propagationStart"] == inp["propagationStart"]
+    assert '"Sec": 42.0' in text and '"propagationSeconds": 46800' in text
+    assert set(obj) == {"propagationStart", "propagationSeconds", "positionsStart",
+                        "velocitiesStart", "positionsEnd", "velocitiesEnd"}
+    # runner.cu:13-15: n = floor(T / 30), h = T / n
+    n = 46800 // 30
+    h = 46800 / n
+    o = po.Oracle()
+    st = inp["propagationStart"]
+    epoch = (st["Year"], st["Month"], st["Day"], st["Hour"], st["Min"], st["Sec"])
+    ro, vo = o.propagate(po.GRAVITY, inp["positions"], inp["velocities"], h, n, epoch=epoch)
+    assert np.abs(np.array(obj["positionsEnd"]) - ro).max() < 1e-3
+    assert np.abs(np.array(obj["velocitiesEnd"]) - vo).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_closest_approach_records_every_cross_set_pair(tmp_path, po):
+    from opengpop_amd import scenarios
+    r, v = scenarios.leo(6, seed=9)
+    inp = dict(propagationStart=dict(Year=2025, Month=1, Day=1, Hour=0, Min=0, Sec=0.0),
+               propagationSeconds=5400, positions=r.tolist(), velocities=v.tolist(),
+               setIndices=[0, 0, 1, 1, 1, 2])
+    src = tmp_path / "in.json"
+    src.write_text(json.dumps(inp))
+    out = tmp_path / "out.json"
+    res = _run(["closest-approach", "-i", str(src), "-o", str(out), "--degree", "20"])
+    assert res.returncode == 0, res.stderr
+    obj, _ = _check_dump4_layout(out.read_text())
+    recs = obj["closestApproaches"]
+    # runner.cu:96-106: every pair i < j of different sets, in (i, j) order
+    want = [(i, j) for i in range(6) for j in range(i + 1, 6)
+            if inp["setIndices"][i] != inp["setIndices"][j]]
+    assert [(q["objectIndex1"], q["objectIndex2"]) for q in recs] == want
+    assert sorted(recs[0]) == sorted(["objectIndex1", "objectIndex2", "distance", "time",
+                                      "positionAx", "positionAy", "positionAz", "positionBx",
+                                      "positionBy", "positionBz"])
+    o = po.Oracle(degree=20)
+    n = 5400 // 30
+    _, _, traj = o.propagate(po.GRAVITY, r, v, 30.0, n, trajectory=True)
+    mind, mink = o.screen(traj, inp["setIndices"])
+    for q, rec in enumerate(recs):
+        a, b = rec["objectIndex1"], rec["objectIndex2"]
+        idx = o.partition_idx(inp["setIndices"], a, b)
+        ref = o.refine(traj, 30.0, a, b, int(mink[idx]), float(mind[idx]))
+        assert abs(rec["distance"] - ref["distance"]) < 1.0
+        assert abs(rec["time"] - ref["time"]) < 1e-3
