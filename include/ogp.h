@@ -182,6 +182,40 @@ typedef struct ogp_conj_timing {
 } ogp_conj_timing;
 int ogp_conj_last_timing(const ogp_context* ctx, ogp_conj_timing* t);
 
+/* ---- derivatives (no counterpart in the reference) ------------------------------------------
+ * Gravity-only FP64 contexts (OGP_FORCE_GRAVITY, OGP_PRECISION_FP64); one step size per call,
+ * per-object step counts; host arrays. Jacobians are [n][6][6] row-major: row = final component
+ * (x, y, z, vx, vy, vz), column = initial component. */
+
+/* Gravity acceleration and its Jacobian G[i][j] = d a_i / d r_j in the ITRS ([n][3], [n][3][3]);
+ * the acceleration is bit-identical to ogp_gravity_itrs. */
+int ogp_gravity_jacobian_itrs(ogp_context* ctx, const double* r, int64_t n, double* a, double* G);
+
+/* State-transition matrix from the variational equations: the six tangents integrated with the
+ * same RK8 startup and GJ8 steps as the state (forward mode). Also returns the final state. */
+int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
+    const int64_t* steps, double* r_out, double* v_out, double* stm);
+
+/* d(r, v)_final / d(r, v)_initial by the discrete adjoint of the GJ8 steps (reverse mode, the
+ * state checkpointed every checkpoint_steps steps, 0 = 64) composed with the forward-mode
+ * derivative of the startup. Also returns the final state. */
+int ogp_adjoint_jacobian(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double step_s, const int64_t* steps, int64_t checkpoint_steps, double* r_out, double* v_out,
+    double* jac);
+
+/* Vector-Jacobian product by the same adjoint: given the gradients g_r, g_v ([n][3]) of a scalar
+ * with respect to the final state, returns its gradients with respect to the initial state. */
+int ogp_adjoint_vjp(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double step_s, const int64_t* steps, int64_t checkpoint_steps, const double* g_r,
+    const double* g_v, double* g_r0, double* g_v0);
+
+/* Device time of the last derivative call: forward pass (variational startup, checkpointed GJ8
+ * steps), adjoint pass (recording + reverse + assembly), or the whole variational run. */
+typedef struct ogp_adjoint_timing {
+    double forward_ms, adjoint_ms, stm_ms;
+} ogp_adjoint_timing;
+int ogp_adjoint_last_timing(const ogp_context* ctx, ogp_adjoint_timing* t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,11 @@ def lib():
         "ogp_conj_num_pairs": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
         "ogp_conj_trajectory": (ctypes.c_int, [_P, _P, ctypes.POINTER(_I64)]),
         "ogp_conj_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(OgpConjTiming)]),
+        "ogp_gravity_jacobian_itrs": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
+        "ogp_stm": (ctypes.c_int, [_P, _P, _P, _I64, _D, _P, _P, _P, _P]),
+        "ogp_adjoint_jacobian": (ctypes.c_int, [_P, _P, _P, _I64, _D, _P, _I64, _P, _P, _P]),
+        "ogp_adjoint_vjp": (ctypes.c_int, [_P, _P, _P, _I64, _D, _P, _I64, _P, _P, _P, _P]),
+        "ogp_adjoint_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(OgpAdjointTiming)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

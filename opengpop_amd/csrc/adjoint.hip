@@ -1,0 +1,370 @@
+// Derivatives of the propagation: gravity Jacobian, variational-equation (tangent) layout, and
+// the discrete adjoint of the GJ8 steps. No counterpart in the reference.
+//
+// d(r, v)_final / d(r, v)_initial is assembled from two pieces:
+//  * the startup (RK8 + startup corrector, gpu_startup.cu) is differentiated forward: the startup
+//    kernel runs on the variational-equation layout (kGravityTangent, device.hpp), giving the
+//    derivatives of the post-startup state Z = (s, S, a_{-4..4}) w.r.t. the six initial
+//    components;
+//  * the GJ8 steps are differentiated backward: a forward pass stores the state (s, S and the
+//    acceleration ring, 33 doubles per object) every C steps; per segment, last to first, a
+//    recording pass re-runs the steps from the checkpoint and stores the gravity Jacobian at
+//    every corrector iterate, and the reverse pass carries the cotangents of (s, S, ring) back
+//    through the segment. Cotangents of Z times dZ/dX give the Jacobian rows.
+//
+// Reverse step (transpose of gj8_sums / gj8_predict / gj8_correct, gj8.hpp): with e_it the
+// acceleration at iterate r_it (it = 0 .. K-1), r_{it+1} = hh((T2 + gj88 e_it) + S'),
+// v_{it+1} = h((T1 + sa88 e_it) + s'_it), s'_it = s + (e_it + a_{k-1})/2, S' = S + s + a_{k-1}/2,
+// r_0 = hh(Pr + S'); the predicted velocity does not reach the gravity, so Pv carries no
+// cotangent. The cotangent of e_it is mapped to r_it by G(r_it), symmetric (a Hessian).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gj8.hpp"
+#include "kernels.hpp"
+
+namespace ogp {
+
+namespace {
+
+constexpr int kMaxCorr = 16;
+
+// Lane of the 7-lane layout for object o (tangent t = 0: the state itself).
+__device__ __forceinline__ int64_t tlane(int64_t o, int t) { return (o / 9) * 64 + (o % 9) * 7 + t; }
+
+} // namespace
+
+__global__ void gravity_jac_kernel(GravityDev g, const double* __restrict__ r, int64_t n,
+    double* __restrict__ a, double* __restrict__ G)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double ai[3], Gi[3][3];
+    gravity_itrs_jac(r[3 * i], r[3 * i + 1], r[3 * i + 2], g, ai, Gi);
+    for (int c = 0; c < 3; ++c) {
+        a[3 * i + c] = ai[c];
+        for (int j = 0; j < 3; ++j)
+            G[9 * i + 3 * c + j] = Gi[c][j];
+    }
+}
+
+// Variational-equation layout: state lanes take (r0, v0), tangent lane t (1..6) the unit vector
+// of initial component t-1; unused lanes get no steps.
+__global__ void tangent_init_kernel(const double* __restrict__ r0, const double* __restrict__ v0,
+    const int64_t* __restrict__ steps, int64_t n, PropState ps, int64_t* __restrict__ ns_out)
+{
+    const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= ps.N)
+        return;
+    const int l = (int)(L % 64);
+    const int64_t o = (L / 64) * 9 + l / 7;
+    const bool used = l < 63 && o < n;
+    const int t = l % 7;
+    for (int c = 0; c < 3; ++c) {
+        double rc = 0.0, vc = 0.0;
+        if (used) {
+            rc = t == 0 ? r0[3 * o + c] : (t - 1 == c ? 1.0 : 0.0);
+            vc = t == 0 ? v0[3 * o + c] : (t - 4 == c ? 1.0 : 0.0);
+        }
+        ps.r[c * ps.N + L] = rc;
+        ps.v[c * ps.N + L] = vc;
+    }
+    ns_out[L] = used ? steps[o] : 0;
+}
+
+// After the full variational run: final state and the 6x6 matrix [o][row][col].
+__global__ void tangent_final_kernel(PropState ps, int64_t n, double* __restrict__ r_out,
+    double* __restrict__ v_out, double* __restrict__ stm)
+{
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n)
+        return;
+    const int64_t N = ps.N, L0 = tlane(o, 0);
+    for (int c = 0; c < 3; ++c) {
+        r_out[3 * o + c] = ps.r[c * N + L0];
+        v_out[3 * o + c] = ps.v[c * N + L0];
+    }
+    for (int col = 0; col < 6; ++col) {
+        const int64_t L = L0 + 1 + col;
+        for (int c = 0; c < 3; ++c) {
+            stm[36 * o + 6 * c + col] = ps.r[c * N + L];
+            stm[36 * o + 6 * (3 + c) + col] = ps.v[c * N + L];
+        }
+    }
+}
+
+// After the variational startup: the state lanes become the compact state (n objects) of the
+// forward pass; the tangent lanes give dZ/dX, Zd[(col * 33 + e) * n + o] with e = 0..2 s,
+// 3..5 S, 6 + 3 slot + c the ring; and d(r, v)/dX after the startup, RVd[(col * 6 + e) * n + o]
+// (the final derivative of spans of at most four steps).
+__global__ void tangent_split_kernel(PropState ps7, int64_t n, PropState pc,
+    int64_t* __restrict__ pc_ns, double* __restrict__ Zd, double* __restrict__ RVd)
+{
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n)
+        return;
+    const int64_t N7 = ps7.N, L0 = tlane(o, 0);
+    for (int c = 0; c < 3; ++c) {
+        pc.r[c * n + o] = ps7.r[c * N7 + L0];
+        pc.v[c * n + o] = ps7.v[c * N7 + L0];
+        pc.s[c * n + o] = ps7.s[c * N7 + L0];
+        pc.S[c * n + o] = ps7.S[c * N7 + L0];
+        for (int sl = 0; sl < 9; ++sl)
+            pc.hist[(sl * 3 + c) * n + o] = ps7.hist[(sl * 3 + c) * N7 + L0];
+    }
+    pc_ns[o] = ps7.nsteps[L0];
+    for (int col = 0; col < 6; ++col) {
+        const int64_t L = L0 + 1 + col;
+        for (int c = 0; c < 3; ++c) {
+            Zd[(col * 33 + c) * n + o] = ps7.s[c * N7 + L];
+            Zd[(col * 33 + 3 + c) * n + o] = ps7.S[c * N7 + L];
+            for (int sl = 0; sl < 9; ++sl)
+                Zd[(col * 33 + 6 + sl * 3 + c) * n + o] = ps7.hist[(sl * 3 + c) * N7 + L];
+            RVd[(col * 6 + c) * n + o] = ps7.r[c * N7 + L];
+            RVd[(col * 6 + 3 + c) * n + o] = ps7.v[c * N7 + L];
+        }
+    }
+}
+
+// Copies (s, S, ring) between the state and a checkpoint slot ck[e * n + o], e = 0..32.
+__global__ void checkpoint_kernel(PropState ps, double* __restrict__ ck, int to_checkpoint)
+{
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = ps.N;
+    if (o >= n)
+        return;
+    for (int c = 0; c < 3; ++c) {
+        double* p[2] = { &ps.s[c * n + o], &ps.S[c * n + o] };
+        for (int q = 0; q < 2; ++q) {
+            if (to_checkpoint)
+                ck[(3 * q + c) * n + o] = *p[q];
+            else
+                *p[q] = ck[(3 * q + c) * n + o];
+        }
+        for (int sl = 0; sl < 9; ++sl) {
+            double& hv = ps.hist[(sl * 3 + c) * n + o];
+            if (to_checkpoint)
+                ck[(6 + sl * 3 + c) * n + o] = hv;
+            else
+                hv = ck[(6 + sl * 3 + c) * n + o];
+        }
+    }
+}
+
+// GJ8 steps k0 .. k1-1 exactly as gj8_chunk_kernel<kGravity> (same arithmetic, gj8.hpp, and a
+// bit-identical acceleration), storing the GCRS gravity Jacobian of every corrector iterate:
+// Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + o], e = xx, xy, xz, yy, yz, zz.
+__global__ void __launch_bounds__(64) gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
+    int64_t k1, int ncorr, double* __restrict__ Gbuf)
+{
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t n = ps.N;
+    if (i >= n)
+        return;
+    const int64_t ns = ps.nsteps[i];
+    if (ns < k0)
+        return;
+    const double h = f.h, hh = h * h;
+    double r[3], v[3], s[3], S[3];
+    for (int c = 0; c < 3; ++c) {
+        s[c] = ps.s[c * n + i];
+        S[c] = ps.S[c * n + i];
+    }
+    for (int64_t k = k0; k < k1; ++k) {
+        if (k > ns)
+            break;
+        GJ8Sums q;
+        gj8_sums(ps.hist, n, i, k, q);
+        double sprev[3], anew[3];
+        gj8_predict(q, h, s, S, r, v, sprev);
+        for (int it = 0; it < ncorr; ++it) {
+            double G[3][3];
+            gravity_gcrs_jac(f.prop_M + 9 * (k + 4), r, f.grav, anew, G);
+            const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
+            for (int e = 0; e < 6; ++e)
+                Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
+            gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+        }
+        const int wslot = ring_slot(k);
+        for (int c = 0; c < 3; ++c)
+            ps.hist[(wslot * 3 + c) * n + i] = anew[c];
+    }
+}
+
+// Reverse pass over steps k1-1 .. k0 for R cotangent rows. lam[(rho * 33 + e) * n + o] holds the
+// cotangents of (s, S, ring) after the step being reversed; seeds[(rho * 6 + e) * n + o] the
+// cotangent of the final (r, v), injected at the object's last step.
+__global__ void __launch_bounds__(64) gj8_reverse_kernel(const int64_t* __restrict__ nsteps,
+    int64_t n, double h, int64_t k0, int64_t k1, int ncorr, const double* __restrict__ Gbuf,
+    int R, const double* __restrict__ seeds, double* __restrict__ lam)
+{
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n)
+        return;
+    const int64_t ns = nsteps[i];
+    const double hh = h * h, sa88 = c_sa[8][8], gj88 = c_gj[8][8];
+    for (int64_t k = k1 - 1; k >= k0; --k) {
+        if (k > ns)
+            continue;
+        const int sk = ring_slot(k);
+        for (int rho = 0; rho < R; ++rho) {
+            double* L = lam + (int64_t)rho * 33 * n;
+            double le[kMaxCorr][3];
+            for (int it = 0; it < ncorr; ++it)
+                for (int c = 0; c < 3; ++c)
+                    le[it][c] = 0.0;
+            double lr[3], lv[3], lsp[3], lT1[3], lT2[3], ls[3], lal[3], lS[3];
+            for (int c = 0; c < 3; ++c) {
+                // the new ring entry a_k is the last iterate's acceleration
+                le[ncorr - 1][c] = L[(6 + sk * 3 + c) * n + i];
+                L[(6 + sk * 3 + c) * n + i] = 0.0; // the slot now accumulates a_{k-9}
+                lr[c] = k == ns ? seeds[(rho * 6 + c) * n + i] : 0.0;
+                lv[c] = k == ns ? seeds[(rho * 6 + 3 + c) * n + i] : 0.0;
+                lsp[c] = L[c * n + i]; // the new s is s'_{K-1}
+                lS[c] = L[(3 + c) * n + i];
+                lT1[c] = lT2[c] = ls[c] = lal[c] = 0.0;
+            }
+            for (int it = ncorr - 1; it >= 0; --it) {
+                for (int c = 0; c < 3; ++c) {
+                    const double t = hh * lr[c]; // r_{it+1} = hh ((T2 + gj88 e) + S')
+                    lT2[c] += t;
+                    le[it][c] += gj88 * t;
+                    lS[c] += t;
+                    const double u = h * lv[c]; // v_{it+1} = h ((T1 + sa88 e) + s'_it)
+                    lT1[c] += u;
+                    le[it][c] += sa88 * u;
+                    lsp[c] += u;
+                    ls[c] += lsp[c]; // s'_it = s + (e + a_{k-1}) / 2
+                    le[it][c] += lsp[c] / 2;
+                    lal[c] += lsp[c] / 2;
+                }
+                const double* gp = Gbuf + ((((k - k0) * ncorr + it) * 6) * n + i);
+                const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
+                             gyz = gp[4 * n], gzz = gp[5 * n];
+                lr[0] = gxx * le[it][0] + gxy * le[it][1] + gxz * le[it][2];
+                lr[1] = gxy * le[it][0] + gyy * le[it][1] + gyz * le[it][2];
+                lr[2] = gxz * le[it][0] + gyz * le[it][1] + gzz * le[it][2];
+                for (int c = 0; c < 3; ++c)
+                    lv[c] = lsp[c] = 0.0;
+            }
+            double lPr[3];
+            for (int c = 0; c < 3; ++c) {
+                lPr[c] = hh * lr[c]; // r_0 = hh (Pr + S')
+                lS[c] += hh * lr[c];
+                ls[c] += lS[c]; // S' = S + s + a_{k-1} / 2
+                lal[c] += lS[c] / 2;
+            }
+            for (int j = 0; j < 9; ++j) { // Pr, T1, T2 and a_{k-1} over a_{k-9} .. a_{k-1}
+                const int sl = ring_slot(k - 9 + j);
+                for (int c = 0; c < 3; ++c) {
+                    double add = c_gj[9][j] * lPr[c];
+                    if (j >= 1)
+                        add += c_sa[8][j - 1] * lT1[c] + c_gj[8][j - 1] * lT2[c];
+                    if (j == 8)
+                        add += lal[c];
+                    L[(6 + sl * 3 + c) * n + i] += add;
+                }
+            }
+            for (int c = 0; c < 3; ++c) {
+                L[c * n + i] = ls[c];
+                L[(3 + c) * n + i] = lS[c];
+            }
+        }
+    }
+}
+
+// Rows of the Jacobian (or the vector-Jacobian product when R = 1): out[(o * R + rho) * 6 + col].
+// Spans longer than four steps: cotangents of Z times dZ/dX; shorter spans: the seeds times
+// d(r, v)/dX after the startup (zero steps: the identity).
+__global__ void adjoint_combine_kernel(const int64_t* __restrict__ nsteps, int64_t n, int R,
+    const double* __restrict__ lam, const double* __restrict__ Zd, const double* __restrict__ RVd,
+    const double* __restrict__ seeds, double* __restrict__ out)
+{
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n)
+        return;
+    const int64_t ns = nsteps[o];
+    for (int rho = 0; rho < R; ++rho)
+        for (int col = 0; col < 6; ++col) {
+            double acc = 0.0;
+            if (ns > 4) {
+                for (int e = 0; e < 33; ++e)
+                    acc += lam[(rho * 33 + e) * n + o] * Zd[(col * 33 + e) * n + o];
+            } else {
+                for (int e = 0; e < 6; ++e) {
+                    const double d = ns == 0 ? (e == col ? 1.0 : 0.0) : RVd[(col * 6 + e) * n + o];
+                    acc += seeds[(rho * 6 + e) * n + o] * d;
+                }
+            }
+            out[(o * R + rho) * 6 + col] = acc;
+        }
+}
+
+// ---- launchers ---------------------------------------------------------------------------
+
+void launch_gravity_jac(hipStream_t st, const GravityDev& g, const double* r, int64_t n,
+    double* a, double* G)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(gravity_jac_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, g, r,
+            n, a, G);
+}
+
+void launch_tangent_init(hipStream_t st, const double* r0, const double* v0, const int64_t* steps,
+    int64_t n, const PropState& ps7, int64_t* ns7)
+{
+    hipLaunchKernelGGL(tangent_init_kernel, dim3((unsigned)((ps7.N + 255) / 256)), dim3(256), 0, st,
+        r0, v0, steps, n, ps7, ns7);
+}
+
+void launch_tangent_final(hipStream_t st, const PropState& ps7, int64_t n, double* r_out,
+    double* v_out, double* stm)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(tangent_final_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+            ps7, n, r_out, v_out, stm);
+}
+
+void launch_tangent_split(hipStream_t st, const PropState& ps7, int64_t n, const PropState& pc,
+    int64_t* pc_ns, double* Zd, double* RVd)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(tangent_split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+            ps7, n, pc, pc_ns, Zd, RVd);
+}
+
+void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_checkpoint)
+{
+    if (ps.N > 0)
+        hipLaunchKernelGGL(checkpoint_kernel, dim3((unsigned)((ps.N + 255) / 256)), dim3(256), 0, st,
+            ps, ck, to_checkpoint ? 1 : 0);
+}
+
+void launch_gj8_record(hipStream_t st, const ForceCtx& f, const PropState& ps, int64_t k0,
+    int64_t k1, int ncorr, double* Gbuf)
+{
+    if (ps.N > 0 && k1 > k0)
+        hipLaunchKernelGGL(gj8_record_kernel, dim3((unsigned)((ps.N + 63) / 64)), dim3(64), 0, st, f,
+            ps, k0, k1, ncorr, Gbuf);
+}
+
+void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
+    int64_t k1, int ncorr, const double* Gbuf, int R, const double* seeds, double* lam)
+{
+    if (n > 0 && k1 > k0)
+        hipLaunchKernelGGL(gj8_reverse_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
+            nsteps, n, h, k0, k1, ncorr, Gbuf, R, seeds, lam);
+}
+
+void launch_adjoint_combine(hipStream_t st, const int64_t* nsteps, int64_t n, int R,
+    const double* lam, const double* Zd, const double* RVd, const double* seeds, double* out)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(adjoint_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+            st, nsteps, n, R, lam, Zd, RVd, seeds, out);
+}
+
+int adjoint_max_corrections() { return kMaxCorr; }
+
+} // namespace ogp

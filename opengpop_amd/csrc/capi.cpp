@@ -140,6 +140,11 @@ struct ogp_context {
     std::vector<double> res_out;
     ogp_conj_timing cj_timing {};
 
+    // derivatives
+    DevBuf ad_in, ad_steps, ad_r, ad_v, ad_s, ad_S, ad_hist, ad_ns, ad_Zd, ad_RVd, ad_ck, ad_G,
+        ad_lam, ad_seeds, ad_out;
+    ogp_adjoint_timing ad_timing {};
+
     ~ogp_context()
     {
         if (stream)
@@ -1309,6 +1314,268 @@ int ogp_conj_last_timing(const ogp_context* ctx, ogp_conj_timing* t)
     if (!ctx || !t)
         return fail(nullptr, OGP_ERR_ARGUMENT, "null argument");
     *t = ctx->cj_timing;
+    return OGP_OK;
+}
+
+} // extern "C"
+
+// ---- derivatives ---------------------------------------------------------------------------
+namespace {
+
+void check_derivative_inputs(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double step_s, const int64_t* steps)
+{
+    if (ctx->kind != kGravity)
+        throw ApiError(OGP_ERR_ARGUMENT,
+            "derivatives need the FP64 gravity force model (OGP_FORCE_GRAVITY, OGP_PRECISION_FP64)");
+    arg_check(n > 0 && r0 && v0 && steps, "null array or no objects");
+    arg_check(std::isfinite(step_s) && step_s > 0, "step size must be positive and finite");
+    arg_check(ctx->cfg.n_corrections >= 1 && ctx->cfg.n_corrections <= adjoint_max_corrections(),
+        "derivatives support 1 to " + std::to_string(adjoint_max_corrections()) + " corrections");
+    for (int64_t i = 0; i < n; ++i)
+        arg_check(steps[i] >= 0, "negative step count (object " + std::to_string(i) + ")");
+    check_finite(r0, 3 * n, "positions");
+    check_finite(v0, 3 * n, "velocities");
+}
+
+// Uploads r0, v0 (AoS) and the step counts, and runs the startup on the variational-equation
+// layout (64 ceil(n / 9) lanes). Returns the layout's state; its lane count is ps7.N.
+PropState tangent_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double h, const int64_t* steps, NodeTable** nt_out, ForceCtx* f_out)
+{
+    const int64_t N7 = 64 * ((n + 8) / 9);
+    const int64_t max_steps = *std::max_element(steps, steps + n);
+    ctx->reserve_state(N7);
+    ctx->ad_in.reserve(sizeof(double) * 6 * n, &ctx->dev_bytes);
+    ctx->ad_steps.reserve(sizeof(int64_t) * n, &ctx->dev_bytes);
+    double* d_r0 = ctx->ad_in.as<double>();
+    double* d_v0 = d_r0 + 3 * n;
+    HIPC(hipMemcpyAsync(d_r0, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(d_v0, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->ad_steps.p, steps, sizeof(int64_t) * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    const PropState ps7 = ctx->prop_state(N7);
+    launch_tangent_init(ctx->stream, d_r0, d_v0, ctx->ad_steps.as<int64_t>(), n, ps7,
+        ctx->st_ns.as<int64_t>());
+    ctx->check_launch();
+    NodeTable* nt = ctx->node_table(h, max_steps, 0.0);
+    const ForceCtx f = ctx->force_ctx(nt, h);
+    launch_startup(ctx->stream, kGravityTangent, f, ctx->rk8_variant, ps7, ctx->scratch(), 0, N7,
+        ctx->cfg.n_corrections, nullptr, 0, 1);
+    ctx->check_launch();
+    *nt_out = nt;
+    *f_out = f;
+    return ps7;
+}
+
+// Reverse mode for R cotangent rows seeded by seeds_host[(rho * 6 + e) * n + o]; writes the
+// final state (AoS) and out_host[(o * R + rho) * 6 + col].
+void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double h,
+    const int64_t* steps, int64_t C, int R, const std::vector<double>& seeds_host,
+    double* r_out, double* v_out, double* out_host)
+{
+    Event e0, e1, e2;
+    HIPC(hipEventRecord(e0.e, ctx->stream));
+    NodeTable* nt = nullptr;
+    ForceCtx f;
+    const PropState ps7 = tangent_startup(ctx, r0, v0, n, h, steps, &nt, &f);
+    const int ncorr = ctx->cfg.n_corrections;
+    const int64_t max_steps = *std::max_element(steps, steps + n);
+    const size_t v3 = sizeof(double) * 3 * n;
+    ctx->ad_r.reserve(v3, &ctx->dev_bytes);
+    ctx->ad_v.reserve(v3, &ctx->dev_bytes);
+    ctx->ad_s.reserve(v3, &ctx->dev_bytes);
+    ctx->ad_S.reserve(v3, &ctx->dev_bytes);
+    ctx->ad_hist.reserve(9 * v3, &ctx->dev_bytes);
+    ctx->ad_ns.reserve(sizeof(int64_t) * n, &ctx->dev_bytes);
+    ctx->ad_Zd.reserve(sizeof(double) * 6 * 33 * n, &ctx->dev_bytes);
+    ctx->ad_RVd.reserve(sizeof(double) * 36 * n, &ctx->dev_bytes);
+    PropState pc;
+    pc.r = ctx->ad_r.as<double>();
+    pc.v = ctx->ad_v.as<double>();
+    pc.s = ctx->ad_s.as<double>();
+    pc.S = ctx->ad_S.as<double>();
+    pc.hist = ctx->ad_hist.as<double>();
+    pc.nsteps = ctx->ad_ns.as<int64_t>();
+    pc.N = n;
+    launch_tangent_split(ctx->stream, ps7, n, pc, ctx->ad_ns.as<int64_t>(),
+        ctx->ad_Zd.as<double>(), ctx->ad_RVd.as<double>());
+    ctx->check_launch();
+    // forward pass over the GJ8 steps, (s, S, ring) stored at the start of every segment
+    const int64_t nseg = max_steps >= 5 ? (max_steps - 4 + C - 1) / C : 0;
+    ctx->ad_ck.reserve(sizeof(double) * 33 * n * std::max<int64_t>(nseg, 1), &ctx->dev_bytes);
+    const int W = ctx->waves_for(n);
+    for (int64_t q = 0; q < nseg; ++q) {
+        const int64_t k0 = 5 + q * C, k1 = std::min<int64_t>(k0 + C, max_steps + 1);
+        launch_checkpoint(ctx->stream, pc, ctx->ad_ck.as<double>() + q * 33 * n, true);
+        launch_chunk(ctx->stream, kGravity, f, pc, 0, n, k0, k1, ncorr, nullptr, 1, 0, W);
+        ctx->check_launch();
+    }
+    HIPC(hipEventRecord(e1.e, ctx->stream));
+    std::vector<double> rs(3 * n), vs(3 * n);
+    HIPC(hipMemcpyAsync(rs.data(), pc.r, v3, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(vs.data(), pc.v, v3, hipMemcpyDeviceToHost, ctx->stream));
+    // reverse pass, segment by segment from the last
+    ctx->ad_lam.reserve(sizeof(double) * R * 33 * n, &ctx->dev_bytes);
+    ctx->ad_seeds.reserve(sizeof(double) * R * 6 * n, &ctx->dev_bytes);
+    ctx->ad_out.reserve(sizeof(double) * R * 6 * n, &ctx->dev_bytes);
+    ctx->ad_G.reserve(sizeof(double) * C * ncorr * 6 * n, &ctx->dev_bytes);
+    HIPC(hipMemsetAsync(ctx->ad_lam.p, 0, sizeof(double) * R * 33 * n, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->ad_seeds.p, seeds_host.data(), sizeof(double) * R * 6 * n,
+        hipMemcpyHostToDevice, ctx->stream));
+    for (int64_t q = nseg - 1; q >= 0; --q) {
+        const int64_t k0 = 5 + q * C, k1 = std::min<int64_t>(k0 + C, max_steps + 1);
+        launch_checkpoint(ctx->stream, pc, ctx->ad_ck.as<double>() + q * 33 * n, false);
+        launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>());
+        launch_gj8_reverse(ctx->stream, pc.nsteps, n, h, k0, k1, ncorr, ctx->ad_G.as<double>(), R,
+            ctx->ad_seeds.as<double>(), ctx->ad_lam.as<double>());
+        ctx->check_launch();
+    }
+    launch_adjoint_combine(ctx->stream, pc.nsteps, n, R, ctx->ad_lam.as<double>(),
+        ctx->ad_Zd.as<double>(), ctx->ad_RVd.as<double>(), ctx->ad_seeds.as<double>(),
+        ctx->ad_out.as<double>());
+    ctx->check_launch();
+    HIPC(hipEventRecord(e2.e, ctx->stream));
+    HIPC(hipMemcpyAsync(out_host, ctx->ad_out.p, sizeof(double) * R * 6 * n, hipMemcpyDeviceToHost,
+        ctx->stream));
+    ctx->sync();
+    float a = 0, b = 0;
+    HIPC(hipEventElapsedTime(&a, e0.e, e1.e));
+    HIPC(hipEventElapsedTime(&b, e1.e, e2.e));
+    ctx->ad_timing.forward_ms = a;
+    ctx->ad_timing.adjoint_ms = b;
+    for (int64_t o = 0; o < n; ++o)
+        for (int c = 0; c < 3; ++c) {
+            if (r_out)
+                r_out[3 * o + c] = rs[c * n + o];
+            if (v_out)
+                v_out[3 * o + c] = vs[c * n + o];
+        }
+}
+
+} // namespace
+
+extern "C" {
+
+int ogp_gravity_jacobian_itrs(ogp_context* ctx, const double* r, int64_t n, double* a, double* G)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        if (ctx->cfg.force != OGP_FORCE_GRAVITY)
+            throw ApiError(OGP_ERR_ARGUMENT, "the context has no gravity model");
+        arg_check(n >= 0 && (n == 0 || (r && a && G)), "null array");
+        if (n == 0)
+            return;
+        check_finite(r, 3 * n, "positions");
+        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->io_b.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->io_c.reserve(sizeof(double) * 9 * n, &ctx->dev_bytes);
+        HIPC(hipMemcpyAsync(ctx->io_a.p, r, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_gravity_jac(ctx->stream, ctx->gdev, ctx->io_a.as<double>(), n, ctx->io_b.as<double>(),
+            ctx->io_c.as<double>());
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(a, ctx->io_b.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(G, ctx->io_c.p, sizeof(double) * 9 * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+        ctx->sync();
+    });
+}
+
+int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
+    const int64_t* steps, double* r_out, double* v_out, double* stm)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        check_derivative_inputs(ctx, r0, v0, n, step_s, steps);
+        arg_check(r_out && v_out && stm, "null output array");
+        Event e0, e1;
+        HIPC(hipEventRecord(e0.e, ctx->stream));
+        NodeTable* nt = nullptr;
+        ForceCtx f;
+        const PropState ps7 = tangent_startup(ctx, r0, v0, n, step_s, steps, &nt, &f);
+        const int64_t max_steps = *std::max_element(steps, steps + n);
+        for (int64_t k0 = 5; k0 <= max_steps; k0 += ctx->chunk) {
+            const int64_t k1 = std::min<int64_t>(k0 + ctx->chunk, max_steps + 1);
+            launch_chunk(ctx->stream, kGravityTangent, f, ps7, 0, ps7.N, k0, k1,
+                ctx->cfg.n_corrections, nullptr, 1, 0, 1);
+            ctx->check_launch();
+        }
+        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->io_b.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        ctx->io_c.reserve(sizeof(double) * 36 * n, &ctx->dev_bytes);
+        launch_tangent_final(ctx->stream, ps7, n, ctx->io_a.as<double>(), ctx->io_b.as<double>(),
+            ctx->io_c.as<double>());
+        ctx->check_launch();
+        HIPC(hipEventRecord(e1.e, ctx->stream));
+        HIPC(hipMemcpyAsync(r_out, ctx->io_a.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(v_out, ctx->io_b.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(stm, ctx->io_c.p, sizeof(double) * 36 * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+        ctx->sync();
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, e0.e, e1.e));
+        ctx->ad_timing.stm_ms = ms;
+    });
+}
+
+int ogp_adjoint_jacobian(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double step_s, const int64_t* steps, int64_t checkpoint_steps, double* r_out, double* v_out,
+    double* jac)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        check_derivative_inputs(ctx, r0, v0, n, step_s, steps);
+        arg_check(jac != nullptr, "null output array");
+        arg_check(checkpoint_steps >= 0, "negative checkpoint interval");
+        const int64_t C = checkpoint_steps > 0 ? checkpoint_steps : 64;
+        std::vector<double> seeds(36 * n, 0.0);
+        for (int rho = 0; rho < 6; ++rho)
+            for (int64_t o = 0; o < n; ++o)
+                seeds[(rho * 6 + rho) * n + o] = 1.0;
+        adjoint_run(ctx, r0, v0, n, step_s, steps, C, 6, seeds, r_out, v_out, jac);
+    });
+}
+
+int ogp_adjoint_vjp(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double step_s, const int64_t* steps, int64_t checkpoint_steps, const double* g_r,
+    const double* g_v, double* g_r0, double* g_v0)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        check_derivative_inputs(ctx, r0, v0, n, step_s, steps);
+        arg_check(g_r && g_v && g_r0 && g_v0, "null array");
+        arg_check(checkpoint_steps >= 0, "negative checkpoint interval");
+        check_finite(g_r, 3 * n, "position gradients");
+        check_finite(g_v, 3 * n, "velocity gradients");
+        const int64_t C = checkpoint_steps > 0 ? checkpoint_steps : 64;
+        std::vector<double> seeds(6 * n), out(6 * n);
+        for (int64_t o = 0; o < n; ++o)
+            for (int c = 0; c < 3; ++c) {
+                seeds[c * n + o] = g_r[3 * o + c];
+                seeds[(3 + c) * n + o] = g_v[3 * o + c];
+            }
+        adjoint_run(ctx, r0, v0, n, step_s, steps, C, 1, seeds, nullptr, nullptr, out.data());
+        for (int64_t o = 0; o < n; ++o)
+            for (int c = 0; c < 3; ++c) {
+                g_r0[3 * o + c] = out[6 * o + c];
+                g_v0[3 * o + c] = out[6 * o + 3 + c];
+            }
+    });
+}
+
+int ogp_adjoint_last_timing(const ogp_context* ctx, ogp_adjoint_timing* t)
+{
+    if (!ctx || !t)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null argument");
+    *t = ctx->ad_timing;
     return OGP_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "coefficients.hpp"
+#include "dual.hpp"
 
 namespace ogp {
 
@@ -35,9 +36,11 @@ struct GravityDev {
 };
 
 // Partial harmonic sums of one column range (gravity_source_device.cu:122-210 restated).
-struct GravSums {
-    double G, H, J, K;
+// V is double, or Dual3 for the Jacobian (dual.hpp).
+template <typename V> struct GravSumsT {
+    V G, H, J, K;
 };
+using GravSums = GravSumsT<double>;
 
 // Fused multiply-add in the precision of the operands (no promotion of FP32 to FP64).
 __device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -45,9 +48,10 @@ __device__ __forceinline__ double fmat(double a, double b, double c) { return __
 
 // One column m of records [k0, k1): k0 is the diagonal n = m (accumulated only), the rest run
 // the recursion P_n = (xi * eps * rho) P_{n-1} - (eta * rho^2) P_{n-2} on the scaled functions.
-template <typename T>
-__device__ __forceinline__ void column_terms(const void* table, int k0, int k1, T er, T rr2,
-    T& P, T& Pp, T& A1, T& A2, T& B1, T& B2, T& E1, T& E2, bool first_is_diag)
+// T: type of the table entries; V: type of the per-object values.
+template <typename T, typename V>
+__device__ __forceinline__ void column_terms(const void* table, int k0, int k1, V er, V rr2,
+    V& P, V& Pp, V& A1, V& A2, V& B1, V& B2, V& E1, V& E2, bool first_is_diag)
 {
     typedef const __attribute__((address_space(4))) T* tptr;
     tptr q = (tptr)table + 8 * (int64_t)k0;
@@ -74,7 +78,7 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
                 rec[b][e] = q[8 * b + e];
 #pragma unroll
         for (int b = 0; b < BATCH; ++b) {
-            const T Pn = fmat(rec[b][0] * er, P, -(rec[b][1] * rr2) * Pp);
+            const V Pn = fmat(rec[b][0] * er, P, -(rec[b][1] * rr2) * Pp);
             Pp = P;
             P = Pn;
             A1 = fmat(P, rec[b][2], A1);
@@ -86,7 +90,7 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
         }
     }
     for (; k < k1; ++k, q += 8) {
-        const T Pn = fmat(q[0] * er, P, -(q[1] * rr2) * Pp);
+        const V Pn = fmat(q[0] * er, P, -(q[1] * rr2) * Pp);
         Pp = P;
         P = Pn;
         A1 = fmat(P, q[2], A1);
@@ -110,27 +114,29 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
 // SPLIT: degrees n <= nsplit in FP64, n > nsplit in FP32; the per-column partial sums are
 // promoted to FP64 before they meet the longitude terms.
 // Geometry shared by all columns of one evaluation point.
-struct GravPoint {
-    double ir, eps, rho, er, rr2, cphi, U1, W1;
+template <typename V> struct GravPointT {
+    V ir, eps, rho, er, rr2, cphi, U1, W1;
 };
+using GravPoint = GravPointT<double>;
 
-__device__ __forceinline__ GravPoint grav_point(double x, double y, double z, double radius)
+template <typename V>
+__device__ __forceinline__ GravPointT<V> grav_point(V x, V y, V z, double radius)
 {
-    GravPoint p;
-    const double pxy2 = __builtin_fma(y, y, x * x);
-    const double r2 = __builtin_fma(z, z, pxy2);
-    const double r = __builtin_sqrt(r2);
+    GravPointT<V> p;
+    const V pxy2 = fmat(y, y, x * x);
+    const V r2 = fmat(z, z, pxy2);
+    const V r = dsqrt(r2);
     p.ir = 1.0 / r;
     p.eps = z * p.ir;
     p.rho = radius * p.ir;
     p.er = p.eps * p.rho;
     p.rr2 = p.rho * p.rho;
-    const double pxy = __builtin_sqrt(pxy2);
+    const V pxy = dsqrt(pxy2);
     p.cphi = pxy * p.ir;
     p.U1 = 1.0;
     p.W1 = 0.0;
-    if (pxy > 0.0) {
-        const double ip = 1.0 / pxy;
+    if (val(pxy) > 0.0) {
+        const V ip = 1.0 / pxy;
         p.U1 = x * ip;
         p.W1 = y * ip;
     }
@@ -138,28 +144,29 @@ __device__ __forceinline__ GravPoint grav_point(double x, double y, double z, do
 }
 
 // gravity_source_device.cu:219-238 on the complete sums.
-__device__ __forceinline__ void grav_assemble(double x, double y, double z, const GravPoint& p,
-    const GravSums& s, double gm, double& ax, double& ay, double& az)
+template <typename V>
+__device__ __forceinline__ void grav_assemble(V x, V y, V z, const GravPointT<V>& p,
+    const GravSumsT<V>& s, double gm, V& ax, V& ay, V& az)
 {
-    const double Lam = (s.G + 1.0) + z * s.H * p.ir;
-    const double coef = -gm * p.ir * p.ir;
-    ax = coef * __builtin_fma(Lam * x, p.ir, -s.J);
-    ay = coef * __builtin_fma(Lam * y, p.ir, -s.K);
-    az = coef * __builtin_fma(Lam * z, p.ir, -s.H);
+    const V Lam = (s.G + 1.0) + z * s.H * p.ir;
+    const V coef = -gm * p.ir * p.ir;
+    ax = coef * fmat(Lam * x, p.ir, -s.J);
+    ay = coef * fmat(Lam * y, p.ir, -s.K);
+    az = coef * fmat(Lam * z, p.ir, -s.H);
 }
 
 // Sums of parts [q0, q1) (q0 < q1, wave-uniform); sink(q, sums) receives the sums of part q, in
 // part order. The longitude terms and the scaled diagonal of the first column are reached by
 // running their recurrences from m = 0 (a few operations per skipped column).
-template <bool SPLIT, class Sink>
-__device__ __forceinline__ void grav_parts(const GravPoint& p, const GravityDev& g, int q0,
+template <bool SPLIT, typename V, class Sink>
+__device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const GravityDev& g, int q0,
     int q1, Sink&& sink)
 {
-    const double er = p.er, rr2 = p.rr2, cphi = p.cphi, rho = p.rho, U1 = p.U1, W1 = p.W1;
-    const float erf_ = (float)er, rr2f = (float)rr2;
-    double Um = 1.0, Wm = 0.0;   // cos(m lambda), sin(m lambda)
-    double Um1 = 1.0, Wm1 = 0.0; // (m - 1)
-    double diagP = 1.0;          // scaled diagonal cos(phi)^(m-1) (R/r)^m Pbar_mm
+    const V er = p.er, rr2 = p.rr2, cphi = p.cphi, rho = p.rho, U1 = p.U1, W1 = p.W1;
+    const float erf_ = tofloat(er), rr2f = tofloat(rr2);
+    V Um = 1.0, Wm = 0.0;   // cos(m lambda), sin(m lambda)
+    V Um1 = 1.0, Wm1 = 0.0; // (m - 1)
+    V diagP = 1.0;          // scaled diagonal cos(phi)^(m-1) (R/r)^m Pbar_mm
     const uniform_dptr diag = (uniform_dptr)g.diag;
     const uniform_iptr parts = (uniform_iptr)g.parts;
     const int L = g.L;
@@ -172,23 +179,23 @@ __device__ __forceinline__ void grav_parts(const GravPoint& p, const GravityDev&
             diagP *= cphi * rho * diag[m];
         Um1 = Um;
         Wm1 = Wm;
-        const double un = __builtin_fma(Um, U1, -Wm * W1);
-        Wm = __builtin_fma(Wm, U1, Um * W1);
+        const V un = fmat(Um, U1, -Wm * W1);
+        Wm = fmat(Wm, U1, Um * W1);
         Um = un;
         kcol += L - m + 1;
     }
     int m = m0;
     for (int q = q0; q < q1; ++q) {
-        double G = 0.0, H = 0.0, J = 0.0, K = 0.0;
+        V G = 0.0, H = 0.0, J = 0.0, K = 0.0;
         for (const int mend = parts[q + 1]; m < mend; ++m) {
             if (m == 1)
                 diagP = rho * diag[1];
             else if (m >= 2)
                 diagP *= cphi * rho * diag[m];
             const int ncol = L - m + 1;
-            double A1, A2, B1, B2, E1, E2;
-            if (!SPLIT) {
-                double P = diagP, Pp = 0.0;
+            V A1, A2, B1, B2, E1, E2;
+            if constexpr (!SPLIT) {
+                V P = diagP, Pp = 0.0;
                 A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
                 column_terms<double>(g.rec, kcol, kcol + ncol, er, rr2, P, Pp, A1, A2, B1, B2, E1,
                     E2, true);
@@ -219,27 +226,27 @@ __device__ __forceinline__ void grav_parts(const GravPoint& p, const GravityDev&
                 G += B1;
             } else {
                 const double dm = (double)m;
-                G += cphi * __builtin_fma(Um, B1, Wm * B2);
-                J += dm * __builtin_fma(Um1, A1, Wm1 * A2);
-                K -= dm * __builtin_fma(Wm1, A1, -Um1 * A2);
-                H += __builtin_fma(Um1, E1, Wm1 * E2);
+                G += cphi * fmat(Um, B1, Wm * B2);
+                J += dm * fmat(Um1, A1, Wm1 * A2);
+                K -= dm * fmat(Wm1, A1, -Um1 * A2);
+                H += fmat(Um1, E1, Wm1 * E2);
             }
             // advance the longitude terms: (U + iW)_{m+1} = (U + iW)_m (U1 + i W1)
             Um1 = Um;
             Wm1 = Wm;
-            const double un = __builtin_fma(Um, U1, -Wm * W1);
-            Wm = __builtin_fma(Wm, U1, Um * W1);
+            const V un = fmat(Um, U1, -Wm * W1);
+            Wm = fmat(Wm, U1, Um * W1);
             Um = un;
             kcol += ncol;
         }
-        sink(q, GravSums { G, H, J, K });
+        sink(q, GravSumsT<V> { G, H, J, K });
     }
 }
 
 // Adds the part sums in part order (the fold every launch shape performs).
-struct PartFold {
-    GravSums t { 0.0, 0.0, 0.0, 0.0 };
-    __device__ __forceinline__ void operator()(int, const GravSums& s)
+template <typename V> struct PartFoldT {
+    GravSumsT<V> t { 0.0, 0.0, 0.0, 0.0 };
+    __device__ __forceinline__ void operator()(int, const GravSumsT<V>& s)
     {
         t.G += s.G;
         t.H += s.H;
@@ -247,6 +254,7 @@ struct PartFold {
         t.K += s.K;
     }
 };
+using PartFold = PartFoldT<double>;
 
 template <bool SPLIT>
 __device__ __forceinline__ void gravity_itrs(
@@ -256,6 +264,26 @@ __device__ __forceinline__ void gravity_itrs(
     PartFold f;
     grav_parts<SPLIT>(p, g, 0, kParts, f);
     grav_assemble(x, y, z, p, f.t, g.gm, ax, ay, az);
+}
+
+// Acceleration and its Jacobian G[i][j] = d a_i / d r_j in the ITRS (FP64, one wave): the same
+// evaluation on dual numbers, so the acceleration is bit-identical to gravity_itrs<false>.
+__device__ __forceinline__ void gravity_itrs_jac(
+    double x, double y, double z, const GravityDev& g, double* a, double (*G)[3])
+{
+    const Dual3 X = dvar(x, 0), Y = dvar(y, 1), Z = dvar(z, 2);
+    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
+    PartFoldT<Dual3> f;
+    grav_parts<false>(p, g, 0, kParts, f);
+    Dual3 A[3];
+    grav_assemble(X, Y, Z, p, f.t, g.gm, A[0], A[1], A[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        a[i] = A[i].v;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            G[i][j] = A[i].d[j];
+    }
 }
 
 // Column-split evaluation for a workgroup of W waves that all hold the same 64 objects: wave w
@@ -317,12 +345,62 @@ __device__ __forceinline__ void gravity_gcrs(const double* Mg, const double* r,
         a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
 }
 
+// Acceleration and Jacobian in the GCRS: a = M g(M^T r), G = M Gi M^T (Gi the ITRS Jacobian).
+// The acceleration is bit-identical to gravity_gcrs<false>.
+__device__ __forceinline__ void gravity_gcrs_jac(const double* Mg, const double* r,
+    const GravityDev& g, double* a, double (*G)[3])
+{
+    const uniform_dptr M = (uniform_dptr)Mg;
+    double ri[3], gi[3], Gi[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
+    gravity_itrs_jac(ri[0], ri[1], ri[2], g, gi, Gi);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
+    // column-major M: M(i, j) = M[j * 3 + i]; T = Gi M^T, G = M T
+    double T[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            T[i][j] = Gi[i][0] * M[j] + Gi[i][1] * M[3 + j] + Gi[i][2] * M[6 + j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            G[i][j] = M[i] * T[0][j] + M[3 + i] * T[1][j] + M[6 + i] * T[2][j];
+}
+
+// Force of the variational-equation lanes (kGravityTangent) at node matrix Mg.
+__device__ __forceinline__ void tangent_force(const double* Mg, const double* x,
+    const GravityDev& g, double* out)
+{
+    const int lane = threadIdx.x % 64;
+    const int src = lane == 63 ? 63 : lane - lane % 7;
+    double rp[3], a[3], G[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        rp[c] = __shfl(x[c], src, 64);
+    gravity_gcrs_jac(Mg, rp, g, a, G);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        out[i] = lane == src ? a[i] : G[i][0] * x[0] + G[i][1] * x[1] + G[i][2] * x[2];
+}
+
 // ------------------------------------------------------------------------------------------
 // Analytic test forces (unit_tests/include/gpu_test_helpers.cuh) and their time mapping
 // (unit_tests/include/mock_force_model.cuh:69-73 and :99)
 // ------------------------------------------------------------------------------------------
+// kGravityTangent: the variational equations. Lanes 7g .. 7g+6 of a wave hold one object: lane
+// 7g its state, lanes 7g+1 .. 7g+6 six tangent states (derivatives of the state with respect to
+// the initial x, y, z, vx, vy, vz). The state lane receives the acceleration g(r); a tangent lane
+// receives G(r) dr, with G = dg/dr at the state lane's position. The integrator is linear in
+// states and accelerations, so the same startup and GJ8 code integrates the tangents exactly as
+// the discrete scheme's derivative (lane 63 is unused).
 enum ForceKind { kGravity = 0, kNone = 1, kConst = 2, kPoly7 = 3, kPoly8 = 4, kPoly9 = 5,
-    kExp = 6, kDamped = 7, kGravitySplit = 100 };
+    kExp = 6, kDamped = 7, kGravitySplit = 100, kGravityTangent = 101 };
 
 template <int KIND>
 __device__ __forceinline__ void analytic_force(const double* p, const double* v, double t, double* a)
@@ -375,6 +453,9 @@ __device__ __forceinline__ void force_gen(const ForceCtx& f, const double* x, co
     if (KIND == kGravity || KIND == kGravitySplit) {
         const int idx = ((s > 0 ? s - 1 : s) + 4) * 10 + k;
         gravity_gcrs<KIND == kGravitySplit, W>(f.gen_M + 9 * idx, x, f.grav, a, ex.red);
+    } else if (KIND == kGravityTangent) {
+        const int idx = ((s > 0 ? s - 1 : s) + 4) * 10 + k;
+        tangent_force(f.gen_M + 9 * idx, x, f.grav, a);
     } else {
         const int dir = s > 0 ? -1 : 1;
         const double t = (double)(s + dir) * f.h + (double)(-dir) * c_rk8c[k] * f.h;
@@ -389,6 +470,8 @@ __device__ __forceinline__ void force_prop(const ForceCtx& f, const double* x, c
 {
     if (KIND == kGravity || KIND == kGravitySplit) {
         gravity_gcrs<KIND == kGravitySplit, W>(f.prop_M + 9 * node, x, f.grav, a, ex.red);
+    } else if (KIND == kGravityTangent) {
+        tangent_force(f.prop_M + 9 * node, x, f.grav, a);
     } else {
         const double t = (double)(node - 4) * f.h;
         analytic_force<KIND>(x, v, t, a);

@@ -1,0 +1,107 @@
+// Forward-mode dual numbers with three partial derivatives, for the gravity Jacobian
+// G = d a / d r. The value part of every operation is the same IEEE operation, in the same order,
+// as the plain double code it replaces, so the value of a dual evaluation is bit-identical to the
+// plain evaluation; the partials follow the product and chain rules.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ogp {
+
+struct Dual3 {
+    double v;
+    double d[3];
+    Dual3() = default;
+    __device__ Dual3(double x) // a constant
+        : v(x)
+        , d { 0.0, 0.0, 0.0 }
+    {
+    }
+    __device__ Dual3(double x, double d0, double d1, double d2)
+        : v(x)
+        , d { d0, d1, d2 }
+    {
+    }
+};
+
+__device__ __forceinline__ Dual3 dvar(double x, int k) // the k-th independent variable
+{
+    return Dual3(x, k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0);
+}
+__device__ __forceinline__ double val(double x) { return x; }
+__device__ __forceinline__ double val(const Dual3& x) { return x.v; }
+
+__device__ __forceinline__ Dual3 operator+(const Dual3& a, const Dual3& b)
+{
+    return Dual3(a.v + b.v, a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]);
+}
+__device__ __forceinline__ Dual3 operator-(const Dual3& a, const Dual3& b)
+{
+    return Dual3(a.v - b.v, a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]);
+}
+__device__ __forceinline__ Dual3 operator-(const Dual3& a)
+{
+    return Dual3(-a.v, -a.d[0], -a.d[1], -a.d[2]);
+}
+__device__ __forceinline__ Dual3& operator+=(Dual3& a, const Dual3& b) { return a = a + b; }
+__device__ __forceinline__ Dual3& operator-=(Dual3& a, const Dual3& b) { return a = a - b; }
+__device__ __forceinline__ Dual3 operator*(const Dual3& a, const Dual3& b)
+{
+    Dual3 r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        r.d[k] = __builtin_fma(a.v, b.d[k], a.d[k] * b.v);
+    return r;
+}
+__device__ __forceinline__ Dual3 operator*(const Dual3& a, double s)
+{
+    return Dual3(a.v * s, a.d[0] * s, a.d[1] * s, a.d[2] * s);
+}
+__device__ __forceinline__ Dual3 operator*(double s, const Dual3& a)
+{
+    return Dual3(s * a.v, s * a.d[0], s * a.d[1], s * a.d[2]);
+}
+__device__ __forceinline__ Dual3& operator*=(Dual3& a, const Dual3& b) { return a = a * b; }
+__device__ __forceinline__ Dual3 operator/(double s, const Dual3& a) // s / a
+{
+    Dual3 r;
+    r.v = s / a.v;
+    const double q = -r.v / a.v;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        r.d[k] = q * a.d[k];
+    return r;
+}
+__device__ __forceinline__ Dual3 fmat(const Dual3& a, const Dual3& b, const Dual3& c)
+{
+    Dual3 r;
+    r.v = __builtin_fma(a.v, b.v, c.v);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        r.d[k] = __builtin_fma(a.v, b.d[k], __builtin_fma(a.d[k], b.v, c.d[k]));
+    return r;
+}
+__device__ __forceinline__ Dual3 fmat(const Dual3& a, double b, const Dual3& c)
+{
+    Dual3 r;
+    r.v = __builtin_fma(a.v, b, c.v);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        r.d[k] = __builtin_fma(a.d[k], b, c.d[k]);
+    return r;
+}
+__device__ __forceinline__ Dual3 dsqrt(const Dual3& a)
+{
+    Dual3 r;
+    r.v = __builtin_sqrt(a.v);
+    const double q = 0.5 / r.v;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        r.d[k] = a.d[k] * q;
+    return r;
+}
+__device__ __forceinline__ double dsqrt(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float tofloat(double a) { return (float)a; }
+__device__ __forceinline__ float tofloat(const Dual3& a) { return (float)a.v; }
+
+} // namespace ogp

@@ -79,4 +79,24 @@ void launch_permute(hipStream_t st, int mode, const double* src, const int64_t* 
 void launch_aos_to_soa(hipStream_t st, const double* aos, double* soa, int64_t n);
 void launch_soa_to_aos(hipStream_t st, const double* soa, double* aos, int64_t n);
 
+// Derivatives (adjoint.hip). The variational-equation layout puts object o in lanes
+// 64 (o / 9) + 7 (o % 9) + t, t = 0 the state and t = 1..6 its tangents; it needs
+// 64 ceil(n / 9) lanes.
+void launch_gravity_jac(hipStream_t st, const GravityDev& g, const double* r, int64_t n,
+    double* a, double* G);
+void launch_tangent_init(hipStream_t st, const double* r0, const double* v0, const int64_t* steps,
+    int64_t n, const PropState& ps7, int64_t* ns7);
+void launch_tangent_final(hipStream_t st, const PropState& ps7, int64_t n, double* r_out,
+    double* v_out, double* stm);
+void launch_tangent_split(hipStream_t st, const PropState& ps7, int64_t n, const PropState& pc,
+    int64_t* pc_ns, double* Zd, double* RVd);
+void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_checkpoint);
+void launch_gj8_record(hipStream_t st, const ForceCtx& f, const PropState& ps, int64_t k0,
+    int64_t k1, int ncorr, double* Gbuf);
+void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
+    int64_t k1, int ncorr, const double* Gbuf, int R, const double* seeds, double* lam);
+void launch_adjoint_combine(hipStream_t st, const int64_t* nsteps, int64_t n, int R,
+    const double* lam, const double* Zd, const double* RVd, const double* seeds, double* out);
+int adjoint_max_corrections();
+
 } // namespace ogp

@@ -6,6 +6,7 @@
 // (device.hpp, gravity_itrs_split); everything else - the integrator arithmetic, its global
 // loads and stores - runs redundantly and bit-identically in all W waves, so no wave ever reads
 // a value written by another wave through global memory.
+#include "gj8.hpp"
 #include "kernels.hpp"
 
 #include <math.h>
@@ -21,8 +22,6 @@ struct Slots {
     int64_t N, i;
     __device__ double& operator()(int slot, int c) const { return base[(slot * 3 + c) * N + i]; }
 };
-
-__device__ __forceinline__ int ring_slot(int64_t j) { return (int)(((j % 9) + 9) % 9); }
 
 // RK8::step (gpu_startup.cu:108-144, reset_temp_vectors.cu, device_functions.cu,
 // copy_prev_state.cu): stage states from the previous node, stage accelerations into Kv,
@@ -227,7 +226,6 @@ gj8_chunk_kernel(ForceCtx f, PropState ps, int64_t obj0,
     if (ns < k0)
         return;
     const double h = f.h, hh = h * h;
-    const double sa88 = c_sa[8][8], gj88 = c_gj[8][8];
     double r[3], v[3], s[3], S[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -239,41 +237,14 @@ gj8_chunk_kernel(ForceCtx f, PropState ps, int64_t obj0,
     for (int64_t k = k0; k < k1; ++k) {
         if (k > ns)
             continue; // this object's span has ended; its workgroup continues for the others
-        double Pv[3] = { 0, 0, 0 }, Pr[3] = { 0, 0, 0 }, T1[3] = { 0, 0, 0 }, T2[3] = { 0, 0, 0 };
-        double alast[3];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            const int slot = ring_slot(k - 9 + q);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double a = ps.hist[(slot * 3 + c) * N + i];
-                Pv[c] = fma(c_sa[9][q], a, Pv[c]);
-                Pr[c] = fma(c_gj[9][q], a, Pr[c]);
-                if (q >= 1) {
-                    T1[c] = fma(c_sa[8][q - 1], a, T1[c]);
-                    T2[c] = fma(c_gj[8][q - 1], a, T2[c]);
-                }
-                if (q == 8)
-                    alast[c] = a;
-            }
-        }
+        GJ8Sums q;
+        gj8_sums(ps.hist, N, i, k, q);
         double sprev[3], anew[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            S[c] += s[c] + alast[c] / 2;
-            v[c] = ((Pv[c] + s[c]) + alast[c] / 2) * h;
-            r[c] = ((Pr[c] + S[c]) * h) * h;
-            sprev[c] = s[c];
-        }
+        gj8_predict(q, h, s, S, r, v, sprev);
         // P E (E C)^ncorr: the predictor's evaluation is the first corrector evaluation.
         for (int it = 0; it < ncorr; ++it) {
             force_prop<KIND, W>(f, r, v, k + 4, anew, ex);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                s[c] = sprev[c] + (anew[c] + alast[c]) / 2;
-                v[c] = h * ((T1[c] + anew[c] * sa88) + s[c]);
-                r[c] = hh * ((T2[c] + anew[c] * gj88) + S[c]);
-            }
+            gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         }
         const int wslot = ring_slot(k);
 #pragma unroll
@@ -309,6 +280,8 @@ gj8_chunk_kernel(ForceCtx f, PropState ps, int64_t obj0,
             case 8: if (split_) LAUNCH(kGravitySplit, 8); else LAUNCH(kGravity, 8); break;         \
             default: if (split_) LAUNCH(kGravitySplit, 1); else LAUNCH(kGravity, 1); break;        \
             }                                                                                      \
+        } else if (kind == kGravityTangent) {                                                      \
+            LAUNCH(kGravityTangent, 1);                                                            \
         } else {                                                                                   \
             switch (kind) {                                                                        \
             case kNone: LAUNCH(kNone, 1); break;                                                   \

@@ -118,6 +118,61 @@ class Context:
         check(self.h, lib().ogp_last_timing(self.h, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in C.OgpTiming._fields_}
 
+    # -- derivatives ------------------------------------------------------------------
+    def gravity_jacobian_itrs(self, r):
+        """(a [n][3], G [n][3][3]) with G[i][j] = d a_i / d r_j, ITRS."""
+        r = f64(r).reshape(-1, 3)
+        a = np.zeros_like(r)
+        G = np.zeros((len(r), 3, 3))
+        check(self.h, lib().ogp_gravity_jacobian_itrs(self.h, ptr(r), len(r), ptr(a), ptr(G)))
+        return a, G
+
+    def _steps(self, n, steps):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(steps, np.int64), (n,)))
+
+    def stm(self, r0, v0, step, steps):
+        """Final state and the state-transition matrix [n][6][6] from the variational
+        equations."""
+        r0 = f64(r0).reshape(-1, 3)
+        v0 = f64(v0).reshape(-1, 3)
+        n = len(r0)
+        ns = self._steps(n, steps)
+        r, v, m = np.zeros_like(r0), np.zeros_like(v0), np.zeros((n, 6, 6))
+        check(self.h, lib().ogp_stm(self.h, ptr(r0), ptr(v0), n, float(step), ptr(ns), ptr(r),
+                                    ptr(v), ptr(m)))
+        return r, v, m
+
+    def adjoint_jacobian(self, r0, v0, step, steps, checkpoint_steps=0):
+        """Final state and d(r, v)_final / d(r, v)_initial [n][6][6] by the discrete adjoint."""
+        r0 = f64(r0).reshape(-1, 3)
+        v0 = f64(v0).reshape(-1, 3)
+        n = len(r0)
+        ns = self._steps(n, steps)
+        r, v, J = np.zeros_like(r0), np.zeros_like(v0), np.zeros((n, 6, 6))
+        check(self.h, lib().ogp_adjoint_jacobian(self.h, ptr(r0), ptr(v0), n, float(step), ptr(ns),
+                                                 checkpoint_steps, ptr(r), ptr(v), ptr(J)))
+        return r, v, J
+
+    def adjoint_vjp(self, r0, v0, step, steps, g_r, g_v, checkpoint_steps=0):
+        """Gradients w.r.t. (r0, v0) of a scalar whose gradients w.r.t. the final state are
+        (g_r, g_v)."""
+        r0 = f64(r0).reshape(-1, 3)
+        v0 = f64(v0).reshape(-1, 3)
+        n = len(r0)
+        ns = self._steps(n, steps)
+        g_r = f64(g_r).reshape(n, 3)
+        g_v = f64(g_v).reshape(n, 3)
+        gr0, gv0 = np.zeros_like(r0), np.zeros_like(v0)
+        check(self.h, lib().ogp_adjoint_vjp(self.h, ptr(r0), ptr(v0), n, float(step), ptr(ns),
+                                            checkpoint_steps, ptr(g_r), ptr(g_v), ptr(gr0),
+                                            ptr(gv0)))
+        return gr0, gv0
+
+    def adjoint_timing(self):
+        t = C.OgpAdjointTiming()
+        check(self.h, lib().ogp_adjoint_last_timing(self.h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in C.OgpAdjointTiming._fields_}
+
     def startup(self, r0, v0, step):
         """GpuStartup::Startup: node matrices, s, S and accelerations, each [n][9][3]."""
         r0 = f64(r0).reshape(-1, 3)
