@@ -1,6 +1,7 @@
 """Benchmark of the propagation and conjunction hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs fp64,split,pairs,cpu,refpoint]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--legs fp64,split,pairs,adjoint,cpu,refpoint,copy]
 
 Prints ONE JSON line. The headline `value` is FP64 object-steps/s of the mixed-regime workload:
 100,000 objects (80% LEO at 30 s, 15% MEO and 5% GEO at 300 s, spans U(6, 24) h), JGM3
@@ -11,10 +12,14 @@ complete propagation of that population. Side legs (same line):
             and against the oracle for a sample)
   pairs     20,000 objects with 1,000 forced encounters, set 0 = first 2,000, 3 days at 60 s,
             degree 20: propagation, screen (10 km) and refinement; pair-steps/s of the screen
+  adjoint   10,000 LEO, 60 s, 24 h, degree 20: the 6x6 d(final)/d(initial) per object by the
+            discrete adjoint and by the variational equations; agreement and cost vs the forward
+            propagation (and central differences of the oracle for two objects)
   cpu       the FP64 host oracle on a stratified 1,000-object sample of the headline workload,
             timed on this host's cores; also the GPU-vs-oracle difference on that sample
   refpoint  10,000 LEO, 30 s, 720 steps, degree 70, end to end from host arrays (the workload
             of the reference's published H100 figure)
+  copy      a 1 GiB device copy, the calibration dispatch of the HBM byte counters (not default)
 """
 import argparse
 import json
@@ -66,7 +71,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--legs", default="fp64,split,pairs,cpu,refpoint")
+    p.add_argument("--legs", default="fp64,split,pairs,adjoint,cpu,refpoint")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     return p.parse_args()
 
@@ -250,6 +255,52 @@ def leg_refpoint(ogp, scenarios):
                 published_h100_object_steps_per_s=3.99e5)
 
 
+def leg_adjoint(ogp, scenarios, with_oracle):
+    """Differentiation workload: 10,000 LEO, 60 s, 1,440 steps (24 h), JGM3 20x20, checkpoints
+    every 64 steps: the 6x6 d(final)/d(initial) per object by the adjoint and by the variational
+    equations, their agreement, and their cost against the plain propagation."""
+    c3 = scenarios.config3()
+    r0, v0, h, steps = c3["r"], c3["v"], 60.0, c3["steps"]
+    with ogp.Context(degree=20) as ctx:
+        ctx.propagate(r0[:640], v0[:640], h, steps[:640])  # node tables, wave tuning
+        ctx.propagate(r0, v0, h, steps)
+        fwd_ms = ctx.timing()["total_ms"]
+        _, _, J = ctx.adjoint_jacobian(r0, v0, h, steps, checkpoint_steps=64)
+        ta = ctx.adjoint_timing()
+        rng = np.random.default_rng(3)
+        ctx.adjoint_vjp(r0, v0, h, steps, rng.normal(size=(len(r0), 3)),
+                        rng.normal(size=(len(r0), 3)), checkpoint_steps=64)
+        tv = ctx.adjoint_timing()
+        _, _, Phi = ctx.stm(r0, v0, h, steps)
+        stm_ms = ctx.adjoint_timing()["stm_ms"]
+    rel = np.linalg.norm(J - Phi, axis=(1, 2)) / np.linalg.norm(Phi, axis=(1, 2))
+    out = dict(workload="10,000 LEO, 60 s, 1,440 steps (24 h), JGM3 20x20, checkpoints every 64 "
+                        "steps, 7 corrector evaluations per step",
+               forward_ms=fwd_ms,
+               jacobian_ms=ta["forward_ms"] + ta["adjoint_ms"],
+               jacobian_forward_ms=ta["forward_ms"], jacobian_adjoint_ms=ta["adjoint_ms"],
+               vjp_ms=tv["forward_ms"] + tv["adjoint_ms"], stm_ms=stm_ms,
+               jacobian_cost_vs_forward=(ta["forward_ms"] + ta["adjoint_ms"]) / fwd_ms,
+               vjp_cost_vs_forward=(tv["forward_ms"] + tv["adjoint_ms"]) / fwd_ms,
+               stm_cost_vs_forward=stm_ms / fwd_ms,
+               adjoint_vs_stm_rel=dict(max=float(rel.max()), median=float(np.median(rel))))
+    if with_oracle:
+        from oracle import pyoracle as po
+        o = po.Oracle(degree=20)
+        k = [0, 5000]
+        fd = np.zeros((len(k), 6, 6))
+        x0 = np.hstack([r0[k], v0[k]])
+        for col in range(6):
+            d = np.zeros((len(k), 6))
+            d[:, col] = 1e-3 if col < 3 else 1e-6
+            rp, vp = o.propagate(po.GRAVITY, (x0 + d)[:, :3], (x0 + d)[:, 3:], h, steps[k])
+            rm, vm = o.propagate(po.GRAVITY, (x0 - d)[:, :3], (x0 - d)[:, 3:], h, steps[k])
+            fd[:, :, col] = np.hstack([rp - rm, vp - vm]) / (2 * d[0, col])
+        e = np.linalg.norm(J[k] - fd, axis=(1, 2)) / np.linalg.norm(fd, axis=(1, 2))
+        out["adjoint_vs_oracle_central_differences_rel"] = dict(objects=len(k), max=float(e.max()))
+    return out
+
+
 def leg_copy(torch):
     """A plain device copy of known size (1 GiB read + 1 GiB written): the calibration dispatch
     for the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of the profiler runs."""
@@ -311,6 +362,9 @@ def main():
     if "cpu" in legs and sample is not None:
         line["cpu_baseline"], line["parity_sample"] = leg_cpu(sample, threads)
         log(f"cpu leg: {line['cpu_baseline']['value']:.3e} object-steps/s on {threads} threads")
+    if "adjoint" in legs:
+        line["adjoint"] = leg_adjoint(ogp, scenarios, "cpu" in legs)
+        log(f"adjoint leg: jacobian {line['adjoint']['jacobian_cost_vs_forward']:.1f}x forward")
     if "refpoint" in legs:
         line["reference_point"] = leg_refpoint(ogp, scenarios)
     if "copy" in legs:
