@@ -190,3 +190,20 @@ def test_identical_objects_give_identical_results(oracle70):
     v0 = np.tile([-795.304, 573.772, 7518.179], (4, 1))
     r, v = oracle70.propagate(0, r0, v0, 30.0, 10, epoch=(2015, 5, 4, 22, 22, 2.0))
     assert all(np.array_equal(r[0], r[i]) and np.array_equal(v[0], v[i]) for i in range(4))
+
+
+def test_startup_fixed_point_absorbs_the_reference_quirks(po):
+    # The startup corrector iterates to a fixed point that does not depend on the RK8 guess:
+    # with the reference's 7 corrections, RK8 row 7 (-234 vs -231) and the mirrored backward
+    # stage times leave the 24 h state bit-identical; with one correction they do not.
+    from opengpop_amd import scenarios
+    oracle20 = po.Oracle(degree=20)
+    d = scenarios.config0()
+    r, v = d["r"][:16], d["v"][:16]
+    ref = oracle20.propagate(po.GRAVITY, r, v, 60.0, 1440, n_corr=7, quirks=3)
+    for q in (0, 1, 2):
+        alt = oracle20.propagate(po.GRAVITY, r, v, 60.0, 1440, n_corr=7, quirks=q)
+        assert np.array_equal(ref[0], alt[0]) and np.array_equal(ref[1], alt[1])
+    one = oracle20.propagate(po.GRAVITY, r, v, 60.0, 1440, n_corr=1, quirks=3)
+    alt = oracle20.propagate(po.GRAVITY, r, v, 60.0, 1440, n_corr=1, quirks=2)
+    assert np.abs(one[0] - alt[0]).max() > 1e-2
