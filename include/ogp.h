@@ -120,6 +120,8 @@ typedef struct ogp_timing {
     double step_launches;  /* number of GJ8 chunk launches */
     double object_steps;   /* sum over objects of GJ8 + startup-copy steps */
     double gj_object_steps; /* object-steps executed by the GJ8 chunk kernels */
+    double launch_ms_sum;   /* sum of the GJ8 chunk launches' own durations; launches of
+                               concurrent object slices overlap, so this exceeds steps_ms */
 } ogp_timing;
 int ogp_last_timing(const ogp_context* ctx, ogp_timing* t);
 
@@ -132,6 +134,12 @@ int ogp_last_timing(const ogp_context* ctx, ogp_timing* t);
  * n objects. */
 int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps);
 int ogp_get_tuning(const ogp_context* ctx, int64_t n, int* waves, int64_t* chunk_steps);
+
+/* Object slices per step-size class (no counterpart in the reference): each class is cut into
+ * `slices` contiguous slices whose GJ8 chunk sequences run concurrently on streams of their own,
+ * so the end of one launch overlaps the launches of other slices. 0 = automatic (2 for a class
+ * of at least 4096 objects, else 1), at most 16. Results are bit-identical for every choice. */
+int ogp_set_slices(ogp_context* ctx, int slices);
 
 /* Stepwise GpuPropagator (gpu_propagator.cuh:50-58): init with n objects and one step size;
  * each ogp_prop_step performs one GpuPropagator::Step (startup on the first call, startup

@@ -34,7 +34,7 @@ class OgpConfig(ctypes.Structure):
 class OgpTiming(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("total_ms", "nodes_ms", "startup_ms", "steps_ms", "h2d_ms", "d2h_ms",
-                 "step_launches", "object_steps", "gj_object_steps")]
+                 "step_launches", "object_steps", "gj_object_steps", "launch_ms_sum")]
 
 
 class OgpConjTiming(ctypes.Structure):
@@ -76,6 +76,7 @@ def lib():
         "ogp_propagate_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P]),
         "ogp_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(OgpTiming)]),
         "ogp_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, _I64]),
+        "ogp_set_slices": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_get_tuning": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(_I64)]),
         "ogp_prop_init": (ctypes.c_int, [_P, _P, _P, _I64, _D, _I64]),

@@ -145,8 +145,16 @@ struct ogp_context {
         ad_lam, ad_seeds, ad_out;
     ogp_adjoint_timing ad_timing {};
 
+    // Streams of the concurrent GJ8 chunk sequences (object slices), default priority.
+    static constexpr int kLaneStreams = 4;
+    hipStream_t lane_streams[kLaneStreams] = {};
+    int slices_override = 0; // object slices per class; 0 = chosen from the class size
+
     ~ogp_context()
     {
+        for (hipStream_t s : lane_streams)
+            if (s)
+                (void)hipStreamDestroy(s);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -334,39 +342,95 @@ struct ogp_context {
         }
         HIPC(hipEventRecord(e2.e, stream));
         double launches = 0;
+
+        // GJ8 chunks. Each class is cut into contiguous object slices whose chunk sequences run
+        // on streams of their own: slices do not depend on each other, so the end of one slice's
+        // launch, where few waves remain on the chip, overlaps launches of the other slices and
+        // classes. The host issues the chunks of all slices of a class in lockstep (one k0 per
+        // class); the waves per group follow the class's whole active count at k0.
+        struct Lane {
+            size_t q;
+            int64_t first, count;
+            hipStream_t st;
+        };
+        std::vector<Lane> lanes;
         for (size_t q = 0; q < groups.size(); ++q) {
             const Group& g = groups[q];
-            const ForceCtx f = force_ctx(tabs[q], g.h);
-            auto active_at = [&](int64_t k0) {
-                if (!ns)
-                    return g.count;
-                const int64_t* b = ns + g.first;
-                return (int64_t)(std::partition_point(b, b + g.count,
-                                     [k0](int64_t s) { return s >= k0; }) - b);
-            };
-            auto launch = [&](int64_t k0, int64_t active, int W) {
-                const int64_t k1 = std::min<int64_t>(k0 + chunk, g.max_steps + 1);
-                launch_chunk(stream, kind, f, ps, g.first, active, k0, k1, cfg.n_corrections,
-                    traj, 1, traj_steps, W);
-                check_launch();
-                launches += 1;
-                return k1 - k0;
-            };
-            static const int kTrial[4] = { 1, 2, 4, 8 };
-            int64_t k0 = 5;
-            while (k0 <= g.max_steps) {
-                const int64_t active = active_at(k0);
-                if (needs_tuning(active) && g.max_steps - k0 + 1 >= 4 * chunk) {
-                    // time the next four chunks, one per candidate, and keep the fastest per
+            const int64_t ngroups = (g.count + 63) / 64;
+            int S = slices_override > 0 ? slices_override : (g.count >= 4096 ? 2 : 1);
+            S = (int)std::max<int64_t>(1, std::min<int64_t>(S, ngroups));
+            for (int s = 0; s < S; ++s) {
+                const int64_t a = 64 * (ngroups * s / S), b = std::min(g.count, 64 * (ngroups * (s + 1) / S));
+                lanes.push_back(Lane { q, g.first + a, b - a,
+                    lane_streams[lanes.size() % kLaneStreams] });
+            }
+        }
+        std::vector<ForceCtx> fctx;
+        for (size_t q = 0; q < groups.size(); ++q)
+            fctx.push_back(force_ctx(tabs[q], groups[q].h));
+        auto active_in = [&](int64_t first, int64_t count, int64_t k0) {
+            if (!ns)
+                return count;
+            const int64_t* b = ns + first;
+            return (int64_t)(std::partition_point(b, b + count,
+                                 [k0](int64_t s) { return s >= k0; }) - b);
+        };
+        // every launch is bracketed by events on its own stream, for its own duration
+        std::vector<std::pair<std::unique_ptr<Event>, std::unique_ptr<Event>>> spans;
+        auto launch = [&](hipStream_t st, size_t q, int64_t first, int64_t active, int64_t k0,
+                          int W) {
+            const int64_t k1 = std::min<int64_t>(k0 + chunk, groups[q].max_steps + 1);
+            spans.emplace_back(std::make_unique<Event>(), std::make_unique<Event>());
+            HIPC(hipEventRecord(spans.back().first->e, st));
+            launch_chunk(st, kind, fctx[q], ps, first, active, k0, k1, cfg.n_corrections, traj, 1,
+                traj_steps, W);
+            check_launch();
+            HIPC(hipEventRecord(spans.back().second->e, st));
+            launches += 1;
+            return k1 - k0;
+        };
+        // join: `to` waits for the work issued so far on every stream of `from` (the events live
+        // until the end of the call)
+        std::vector<std::unique_ptr<Event>> marks;
+        auto join = [&](hipStream_t to, const std::vector<hipStream_t>& from) {
+            for (hipStream_t s : from) {
+                if (s == to)
+                    continue;
+                marks.push_back(std::make_unique<Event>());
+                HIPC(hipEventRecord(marks.back()->e, s));
+                HIPC(hipStreamWaitEvent(to, marks.back()->e, 0));
+            }
+        };
+        std::vector<hipStream_t> used;
+        for (const Lane& l : lanes)
+            if (std::find(used.begin(), used.end(), l.st) == used.end())
+                used.push_back(l.st);
+        for (hipStream_t s : used)
+            join(s, { stream });
+
+        static const int kTrial[4] = { 1, 2, 4, 8 };
+        std::vector<int64_t> k0(groups.size(), 5);
+        for (bool more = true; more;) {
+            more = false;
+            for (size_t q = 0; q < groups.size(); ++q) {
+                const Group& g = groups[q];
+                if (k0[q] > g.max_steps)
+                    continue;
+                more = true;
+                const int64_t active = active_in(g.first, g.count, k0[q]);
+                if (needs_tuning(active) && g.max_steps - k0[q] + 1 >= 4 * chunk) {
+                    // time the next four chunks of the whole class on the context's stream, one
+                    // per candidate, after all slices have drained, and keep the fastest per
                     // object-step; the chunks do real work (results do not depend on W)
+                    join(stream, used);
                     Event t[5];
                     double work[4];
                     HIPC(hipEventRecord(t[0].e, stream));
                     for (int c = 0; c < 4; ++c) {
-                        const int64_t a = active_at(k0);
-                        const int64_t steps = launch(k0, a, kTrial[c]);
+                        const int64_t a = active_in(g.first, g.count, k0[q]);
+                        const int64_t steps = launch(stream, q, g.first, a, k0[q], kTrial[c]);
                         work[c] = (double)std::max<int64_t>(a, 1) * (double)steps;
-                        k0 += steps;
+                        k0[q] += steps;
                         HIPC(hipEventRecord(t[c + 1].e, stream));
                     }
                     sync();
@@ -382,11 +446,23 @@ struct ogp_context {
                         }
                     }
                     tuned_waves[group_bucket(active)] = best;
+                    for (hipStream_t s : used)
+                        join(s, { stream });
                     continue;
                 }
-                k0 += launch(k0, active, waves_for(active));
+                const int W = waves_for(active);
+                int64_t steps = 0;
+                for (const Lane& l : lanes) {
+                    if (l.q != q)
+                        continue;
+                    const int64_t a = active_in(l.first, l.count, k0[q]);
+                    if (a > 0)
+                        steps = launch(l.st, q, l.first, a, k0[q], W);
+                }
+                k0[q] += steps > 0 ? steps : chunk;
             }
         }
+        join(stream, used);
         HIPC(hipEventRecord(e3.e, stream));
         sync();
         float ms0 = 0, ms1 = 0, ms2 = 0;
@@ -398,6 +474,13 @@ struct ogp_context {
         timing.steps_ms = ms2;
         timing.total_ms = (double)ms0 + ms1 + ms2;
         timing.step_launches = launches;
+        double sum = 0.0;
+        for (const auto& sp : spans) {
+            float ms = 0;
+            HIPC(hipEventElapsedTime(&ms, sp.first->e, sp.second->e));
+            sum += ms;
+        }
+        timing.launch_ms_sum = sum;
     }
 };
 
@@ -488,6 +571,8 @@ int ogp_create(const ogp_config* cfg, ogp_context** out)
             throw ApiError(OGP_ERR_DEVICE, "no HIP device");
         HIPC(hipSetDevice(0));
         HIPC(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        for (hipStream_t& s : c.lane_streams)
+            HIPC(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         c.rk8_variant = (cfg->quirks & OGP_QUIRK_RK8_ROW7) ? 0 : 1;
         if (cfg->force == OGP_FORCE_GRAVITY) {
             c.kind = cfg->precision == OGP_PRECISION_SPLIT ? kGravitySplit : kGravity;
@@ -795,6 +880,16 @@ int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps)
     ctx->waves_override = waves;
     ctx->chunk = chunk_steps > 0 ? chunk_steps : 64;
     ctx->tuned_waves.clear();
+    return OGP_OK;
+}
+
+int ogp_set_slices(ogp_context* ctx, int slices)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    if (slices < 0 || slices > 16)
+        return fail(ctx, OGP_ERR_ARGUMENT, "slices must be 0 (automatic) .. 16");
+    ctx->slices_override = slices;
     return OGP_OK;
 }
 

@@ -107,6 +107,10 @@ class Context:
     def set_tuning(self, waves=0, chunk_steps=0):
         check(self.h, lib().ogp_set_tuning(self.h, waves, chunk_steps))
 
+    def set_slices(self, slices=0):
+        """Object slices per step-size class run on concurrent streams (0 = automatic)."""
+        check(self.h, lib().ogp_set_slices(self.h, slices))
+
     def get_tuning(self, n):
         w = ctypes.c_int()
         ch = C._I64()

@@ -225,6 +225,34 @@ def test_launch_shape_does_not_change_results(precision):
             assert np.array_equal(r, out[0][0]) and np.array_equal(v, out[0][1])
 
 
+def test_concurrent_slices_do_not_change_results():
+    # 900 objects in two step-size classes (720 at 30 s, 180 at 300 s) with uneven spans, cut
+    # into 1..7 slices per class on concurrent streams; also with the autotuner's trial chunks
+    # (which join the slices) and with the stored trajectory of the conjunction search
+    d = scenarios.mixed(900, seed=13)
+    steps = np.minimum(d["steps"], 150)
+    with ogp.Context(degree=20) as c:
+        c.set_tuning(1, 16)
+        c.set_slices(1)
+        r1, v1 = c.propagate(d["r"], d["v"], d["step"], steps)
+        for s, w in ((2, 2), (3, 4), (7, 8), (3, 0)):
+            c.set_slices(s)
+            c.set_tuning(w, 16)
+            r, v = c.propagate(d["r"], d["v"], d["step"], steps)
+            assert np.array_equal(r, r1) and np.array_equal(v, v1), (s, w)
+        traj = []
+        for s in (1, 3):
+            c.set_slices(s)
+            ca = ogp.ClosestApproach(c, 30.0)
+            ca.initialize(d["r"][:700], d["v"][:700], np.arange(700) >= 350)
+            ca.propagate(40)
+            ca.propagate(30)
+            traj.append(np.array(ca.trajectory()))
+        assert traj[0].shape == traj[1].shape and np.array_equal(traj[0], traj[1])
+        with pytest.raises(ogp.OgpError):
+            c.set_slices(17)
+
+
 def test_autotuner_keeps_results_and_records_choice():
     d = scenarios.config0()
     r0, v0 = np.tile(d["r"], (10, 1)), np.tile(d["v"], (10, 1))  # 640 objects, 10 groups
