@@ -418,48 +418,50 @@ struct ogp_context {
                     continue;
                 more = true;
                 const int64_t active = active_in(g.first, g.count, k0[q]);
-                if (needs_tuning(active) && g.max_steps - k0[q] + 1 >= 4 * chunk) {
-                    // time the next four chunks of the whole class on the context's stream, one
-                    // per candidate, after all slices have drained, and keep the fastest per
-                    // object-step; the chunks do real work (results do not depend on W)
-                    join(stream, used);
-                    Event t[5];
-                    double work[4];
-                    HIPC(hipEventRecord(t[0].e, stream));
-                    for (int c = 0; c < 4; ++c) {
-                        const int64_t a = active_in(g.first, g.count, k0[q]);
-                        const int64_t steps = launch(stream, q, g.first, a, k0[q], kTrial[c]);
-                        work[c] = (double)std::max<int64_t>(a, 1) * (double)steps;
-                        k0[q] += steps;
-                        HIPC(hipEventRecord(t[c + 1].e, stream));
+                // one chunk of every slice of the class with W waves per group; returns the
+                // steps done and appends (launch index, object-steps) of each launch to `out`
+                auto class_chunk = [&](int W, std::vector<std::pair<size_t, double>>* out) {
+                    int64_t steps = 0;
+                    for (const Lane& l : lanes) {
+                        if (l.q != q)
+                            continue;
+                        const int64_t a = active_in(l.first, l.count, k0[q]);
+                        if (a > 0) {
+                            steps = launch(l.st, q, l.first, a, k0[q], W);
+                            if (out)
+                                out->emplace_back(spans.size() - 1, (double)a * (double)steps);
+                        }
                     }
-                    sync();
+                    k0[q] += steps > 0 ? steps : chunk;
+                };
+                if (needs_tuning(active) && g.max_steps - k0[q] + 1 >= 4 * chunk) {
+                    // the next four chunks of the class, one per candidate, run as every other
+                    // chunk does (the slices concurrently on their streams); a candidate's rate
+                    // is the sum over slices of object-steps / launch duration, and the fastest
+                    // is kept. The chunks do real work (results do not depend on W).
+                    std::vector<std::pair<size_t, double>> trial[4];
+                    for (int c = 0; c < 4; ++c)
+                        class_chunk(kTrial[c], &trial[c]);
                     int best = 0;
                     double best_rate = 0.0;
                     for (int c = 0; c < 4; ++c) {
-                        float ms = 0;
-                        HIPC(hipEventElapsedTime(&ms, t[c].e, t[c + 1].e));
-                        const double rate = work[c] / std::max(1e-3, (double)ms);
+                        double rate = 0.0;
+                        for (const auto& t : trial[c]) {
+                            const auto& sp = spans[t.first];
+                            HIPC(hipEventSynchronize(sp.second->e));
+                            float ms = 0;
+                            HIPC(hipEventElapsedTime(&ms, sp.first->e, sp.second->e));
+                            rate += t.second / std::max(1e-3, (double)ms);
+                        }
                         if (rate > best_rate) {
                             best_rate = rate;
                             best = kTrial[c];
                         }
                     }
                     tuned_waves[group_bucket(active)] = best;
-                    for (hipStream_t s : used)
-                        join(s, { stream });
                     continue;
                 }
-                const int W = waves_for(active);
-                int64_t steps = 0;
-                for (const Lane& l : lanes) {
-                    if (l.q != q)
-                        continue;
-                    const int64_t a = active_in(l.first, l.count, k0[q]);
-                    if (a > 0)
-                        steps = launch(l.st, q, l.first, a, k0[q], W);
-                }
-                k0[q] += steps > 0 ? steps : chunk;
+                class_chunk(waves_for(active), nullptr);
             }
         }
         join(stream, used);
