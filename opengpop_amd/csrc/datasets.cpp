@@ -96,18 +96,23 @@ GravityModel load_gravity(const std::string& path, int degree)
         return m == 0 ? std::sqrt(n * (n + 1.0) / 2.0) : std::sqrt((n - m) * (n + m + 1.0));
     };
     g.records.clear();
-    g.records.reserve(8 * tri(N, 0));
+    g.records.reserve(kRecordLen * tri(N, 0));
+    std::vector<double> scale(N + 1); // c_n of the current column
     for (int m = 0; m <= g.L; ++m) {
         for (int n = m; n <= g.L; ++n) {
-            double xi = 0, eta = 0;
-            if (n == m + 1) {
-                xi = std::sqrt(2.0 * m + 3.0);
-            } else if (n >= m + 2) {
-                double dn = n, dm = m;
-                xi = std::sqrt((2.0 * dn - 1.0) * (2.0 * dn + 1.0) / ((dn - dm) * (dn + dm)));
-                eta = std::sqrt((dn + dm - 1.0) * (2.0 * dn + 1.0) * (dn - dm - 1.0)
+            double xi = 0;
+            if (n <= m + 1) {
+                scale[n] = 1.0;
+                xi = n == m + 1 ? std::sqrt(2.0 * m + 3.0) : 0.0;
+            } else {
+                const double dn = n, dm = m;
+                const double x = std::sqrt((2.0 * dn - 1.0) * (2.0 * dn + 1.0) / ((dn - dm) * (dn + dm)));
+                const double eta = std::sqrt((dn + dm - 1.0) * (2.0 * dn + 1.0) * (dn - dm - 1.0)
                     / ((dn + dm) * (dn - dm) * (2.0 * dn - 3.0)));
+                scale[n] = eta * scale[n - 2];
+                xi = x * scale[n - 1] / scale[n];
             }
+            const double cn = scale[n];
             double c = 0, s = 0, kc = 0, ks = 0, zc = 0, zs = 0;
             if (n >= 2) {
                 c = g.C[tri(n, m)];
@@ -120,8 +125,9 @@ GravityModel load_gravity(const std::string& path, int degree)
                     zs = z * g.S[tri(n, m - 1)];
                 }
             }
-            const double rec[8] = { xi, eta, c, s, kc, ks, zc, zs };
-            g.records.insert(g.records.end(), rec, rec + 8);
+            const double rec[kRecordLen] = { xi, cn * c, cn * s, cn * kc, cn * ks, cn * zc,
+                cn * zs };
+            g.records.insert(g.records.end(), rec, rec + kRecordLen);
         }
     }
     g.records_f.assign(g.records.begin(), g.records.end());

@@ -27,23 +27,28 @@ struct FrameData {
 };
 
 // One record per (n, m) visited by the device's column recursion, column-major (m outer,
-// n = m..L inner). Eight coefficients:
-//   xi, eta : recursion Pbar_n^m = xi * eps * Pbar_{n-1}^m - eta * Pbar_{n-2}^m
-//             (for n = m + 1: xi = sqrt(2m + 3), eta = 0; unused on the diagonal n = m)
-//   c, s    : Cbar_nm, Sbar_nm                       (J and K sums)
-//   kc, ks  : (1 + n + m) Cbar_nm, (1 + n + m) Sbar_nm (Gamma sum)
-//   zc, zs  : zeta_{n,m-1} Cbar_{n,m-1}, zeta_{n,m-1} Sbar_{n,m-1} (H sum carried by column m)
+// n = m..L inner). The column recursion Pbar_n = xi_n eps Pbar_{n-1} - eta_n Pbar_{n-2} runs on
+// the rescaled functions U_n = Pbar_n / c_n, with c_m = c_{m+1} = 1 and c_n = eta_n c_{n-2}, so
+// that U_n = xi'_n eps U_{n-1} - U_{n-2} with xi'_n = xi_n c_{n-1} / c_n (one multiplication
+// less per degree; c_n stays within [0.46, 1.13] up to degree 70). The c_n go into the
+// coefficients. Seven values per record:
+//   xi'     : xi'_n (unused on the diagonal n = m)
+//   c, s    : c_n Cbar_nm, c_n Sbar_nm                       (J and K sums)
+//   kc, ks  : c_n (1 + n + m) Cbar_nm, c_n (1 + n + m) Sbar_nm (Gamma sum)
+//   zc, zs  : c_n zeta_{n,m-1} Cbar_{n,m-1}, c_n zeta_{n,m-1} Sbar_{n,m-1} (H sum carried by
+//             column m)
 // Coefficients of degrees n < 2 are zero: the central term is added separately
 // (gravity_source_device.cu:146-152, :219).
+constexpr int kRecordLen = 7;
 struct GravityModel {
     int L = 0;         // degree/order used
     int file_max = 0;  // max_degree of the file
     double gm = 0, radius = 0;
     std::vector<double> C, S;       // triangular, index n(n+1)/2 + m
-    std::vector<double> records;    // 8 doubles per record
+    std::vector<double> records;    // kRecordLen doubles per record
     std::vector<float> records_f;   // same in FP32
     std::vector<double> diag_ratio; // Pbar_mm / Pbar_{m-1,m-1} (m >= 2), [1] = sqrt(3)
-    size_t num_records() const { return records.size() / 8; }
+    size_t num_records() const { return records.size() / kRecordLen; }
 };
 
 // Throws std::runtime_error with a message naming the file on any failure.

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "coefficients.hpp"
+#include "datasets.hpp"
 #include "dual.hpp"
 
 namespace ogp {
@@ -26,7 +27,7 @@ constexpr int kMaxWaves = 8; // waves sharing one object group (column split)
 constexpr int kParts = 8;
 
 struct GravityDev {
-    const double* rec;  // 8 doubles per record, see datasets.hpp (GravityModel::records)
+    const double* rec;  // kRecordLen doubles per record, see datasets.hpp (GravityModel::records)
     const float* recf;  // the same records in FP32
     const double* diag; // Pbar_mm / Pbar_{m-1,m-1}
     const int* parts;   // kParts + 1 column bounds: part q = columns [parts[q], parts[q + 1])
@@ -47,58 +48,59 @@ __device__ __forceinline__ float fmat(float a, float b, float c) { return __buil
 __device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // One column m of records [k0, k1): k0 is the diagonal n = m (accumulated only), the rest run
-// the recursion P_n = (xi * eps * rho) P_{n-1} - (eta * rho^2) P_{n-2} on the scaled functions.
+// the recursion U_n = (xi'_n * eps * rho) U_{n-1} - rho^2 U_{n-2} on the scaled, rescaled
+// functions (the rescaling constants are folded into the coefficients, see GravityModel).
+// Nine instructions per record: two multiplications and seven FMAs.
 // T: type of the table entries; V: type of the per-object values.
 template <typename T, typename V>
 __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, V er, V rr2,
     V& P, V& Pp, V& A1, V& A2, V& B1, V& B2, V& E1, V& E2, bool first_is_diag)
 {
+    constexpr int R = kRecordLen;
     typedef const __attribute__((address_space(4))) T* tptr;
-    tptr q = (tptr)table + 8 * (int64_t)k0;
+    tptr q = (tptr)table + R * (int64_t)k0;
     int k = k0;
     if (first_is_diag) {
-        A1 = fmat(P, q[2], A1);
-        A2 = fmat(P, q[3], A2);
-        B1 = fmat(P, q[4], B1);
-        B2 = fmat(P, q[5], B2);
-        E1 = fmat(P, q[6], E1);
-        E2 = fmat(P, q[7], E2);
-        q += 8;
+        A1 = fmat(P, q[1], A1);
+        A2 = fmat(P, q[2], A2);
+        B1 = fmat(P, q[3], B1);
+        B2 = fmat(P, q[4], B2);
+        E1 = fmat(P, q[5], E1);
+        E2 = fmat(P, q[6], E2);
+        q += R;
         ++k;
     }
+    auto step = [&](const T* r) {
+        const V Pn = fmat(r[0] * er, P, -(rr2 * Pp));
+        Pp = P;
+        P = Pn;
+        A1 = fmat(P, r[1], A1);
+        A2 = fmat(P, r[2], A2);
+        B1 = fmat(P, r[3], B1);
+        B2 = fmat(P, r[4], B2);
+        E1 = fmat(P, r[5], E1);
+        E2 = fmat(P, r[6], E2);
+    };
     // Records are fetched BATCH at a time into scalar registers and then consumed: one wait for
     // the scalar loads per batch instead of one per record.
     constexpr int BATCH = 4;
-    for (; k + BATCH <= k1; k += BATCH, q += 8 * BATCH) {
-        T rec[BATCH][8];
+    for (; k + BATCH <= k1; k += BATCH, q += R * BATCH) {
+        T rec[BATCH][R];
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-                rec[b][e] = q[8 * b + e];
+            for (int e = 0; e < R; ++e)
+                rec[b][e] = q[R * b + e];
 #pragma unroll
-        for (int b = 0; b < BATCH; ++b) {
-            const V Pn = fmat(rec[b][0] * er, P, -(rec[b][1] * rr2) * Pp);
-            Pp = P;
-            P = Pn;
-            A1 = fmat(P, rec[b][2], A1);
-            A2 = fmat(P, rec[b][3], A2);
-            B1 = fmat(P, rec[b][4], B1);
-            B2 = fmat(P, rec[b][5], B2);
-            E1 = fmat(P, rec[b][6], E1);
-            E2 = fmat(P, rec[b][7], E2);
-        }
+        for (int b = 0; b < BATCH; ++b)
+            step(rec[b]);
     }
-    for (; k < k1; ++k, q += 8) {
-        const V Pn = fmat(q[0] * er, P, -(q[1] * rr2) * Pp);
-        Pp = P;
-        P = Pn;
-        A1 = fmat(P, q[2], A1);
-        A2 = fmat(P, q[3], A2);
-        B1 = fmat(P, q[4], B1);
-        B2 = fmat(P, q[5], B2);
-        E1 = fmat(P, q[6], E1);
-        E2 = fmat(P, q[7], E2);
+    for (; k < k1; ++k, q += R) {
+        T rec[R];
+#pragma unroll
+        for (int e = 0; e < R; ++e)
+            rec[e] = q[e];
+        step(rec);
     }
 }
 

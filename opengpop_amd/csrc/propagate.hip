@@ -335,11 +335,11 @@ void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& 
 
 void column_parts(int L, int* parts)
 {
-    // Contiguous column ranges of equal cost: about 10 vector operations per record (column m
+    // Contiguous column ranges of equal cost: about 9 vector operations per record (column m
     // has L - m + 1 records) plus about 25 per column for its set-up and epilogue.
     std::vector<double> C(L + 2, 0.0);
     for (int m = 0; m <= L; ++m)
-        C[m + 1] = C[m] + 10.0 * (L - m + 1) + 25.0;
+        C[m + 1] = C[m] + 9.0 * (L - m + 1) + 25.0;
     parts[0] = 0;
     for (int q = 1; q < kParts; ++q) {
         const double target = C[L + 1] * q / kParts;
