@@ -47,6 +47,10 @@ using GravSums = GravSumsT<double>;
 __device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+#ifndef OGP_RECORD_BATCH
+#define OGP_RECORD_BATCH 4 // gravity records per scalar-load wait
+#endif
+
 // One column m of records [k0, k1): k0 is the diagonal n = m (accumulated only), the rest run
 // the recursion U_n = (xi'_n * eps * rho) U_{n-1} - rho^2 U_{n-2} on the scaled, rescaled
 // functions (the rescaling constants are folded into the coefficients, see GravityModel).
@@ -83,7 +87,7 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
     };
     // Records are fetched BATCH at a time into scalar registers and then consumed: one wait for
     // the scalar loads per batch instead of one per record.
-    constexpr int BATCH = 4;
+    constexpr int BATCH = OGP_RECORD_BATCH;
     for (; k + BATCH <= k1; k += BATCH, q += R * BATCH) {
         T rec[BATCH][R];
 #pragma unroll
