@@ -1,7 +1,7 @@
 """Benchmark of the propagation and conjunction hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--legs fp64,split,pairs,adjoint,cpu,refpoint,copy]
+                    [--legs fp64,split,pairs,adjoint,cpu,refpoint,perturb,copy]
 
 Prints ONE JSON line. The headline `value` is FP64 object-steps/s of the mixed-regime workload:
 100,000 objects (80% LEO at 30 s, 15% MEO and 5% GEO at 300 s, spans U(6, 24) h), JGM3
@@ -19,6 +19,8 @@ complete propagation of that population. Side legs (same line):
             timed on this host's cores; also the GPU-vs-oracle difference on that sample
   refpoint  10,000 LEO, 30 s, 720 steps, degree 70, end to end from host arrays (the workload
             of the reference's published H100 figure)
+  perturb   the Monte-Carlo population generator: 10^8 satellites written to HBM (satellites/s
+            and the write bandwidth against the HBM peak)
   copy      a 1 GiB device copy, the calibration dispatch of the HBM byte counters (not default)
 """
 import argparse
@@ -71,7 +73,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--legs", default="fp64,split,pairs,adjoint,cpu,refpoint")
+    p.add_argument("--legs", default="fp64,split,pairs,adjoint,cpu,refpoint,perturb")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     return p.parse_args()
 
@@ -312,6 +314,28 @@ def leg_adjoint(ogp, scenarios, with_oracle):
     return out
 
 
+def leg_perturb(ogp, torch):
+    """Monte-Carlo population in HBM: 10^8 satellites around one LEO state (SoA, 48 B written per
+    satellite, nothing read), timed around the synchronous device entry point."""
+    n = 100_000_000
+    r0 = [-3884872.0, -5749111.0, 14723.0]
+    v0 = [-795.304, 573.772, 7518.179]
+    rd = torch.empty((3, n), dtype=torch.float64, device="cuda:0")
+    vd = torch.empty_like(rd)
+    with ogp.Context(force=ogp.FORCE_NONE) as ctx:
+        ctx.perturbed_satellites_device(r0, v0, n, 1.0, rd, vd)  # warm-up
+        best = 1e9
+        for _ in range(5):
+            t0 = time.perf_counter()
+            ctx.perturbed_satellites_device(r0, v0, n, 1.0, rd, vd)
+            best = min(best, time.perf_counter() - t0)
+    gbs = 48.0 * n / best / 1e9
+    return dict(workload="10^8 satellites, Philox4x32-10 + Box-Muller normals, SoA output in HBM",
+                satellites_per_s=n / best, ms=best * 1e3,
+                roofline=dict(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s",
+                              frac=gbs / PEAK_HBM_GBS, bytes_per_satellite=48))
+
+
 def leg_copy(torch):
     """A plain device copy of known size (1 GiB read + 1 GiB written): the calibration dispatch
     for the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of the profiler runs."""
@@ -378,6 +402,9 @@ def main():
         log(f"adjoint leg: jacobian {line['adjoint']['jacobian_cost_vs_forward']:.1f}x forward")
     if "refpoint" in legs:
         line["reference_point"] = leg_refpoint(ogp, scenarios)
+    if "perturb" in legs:
+        line["perturb"] = leg_perturb(ogp, torch)
+        log(f"perturb leg: {line['perturb']['satellites_per_s']:.3e} satellites/s")
     if "copy" in legs:
         line["copy_calibration"] = leg_copy(torch)
     print(json.dumps(line), flush=True)

@@ -110,6 +110,24 @@ int ogp_propagate(ogp_context* ctx, const double* r0, const double* v0, int64_t 
 int ogp_propagate_device(ogp_context* ctx, const double* d_r0_soa, const double* d_v0_soa,
     int64_t n, const double* step_s, const int64_t* steps, double* d_r_soa, double* d_v_soa);
 
+/* PerturbationGenerator::generatePerturbedSatellites (perturbation_generator.cu:40-79,
+ * perturbation_generator.cuh:24-43): n satellites at the epoch position r0 with velocity
+ * v0 + strength * N(0, 1) per component, followed by (obj_r, obj_v) as entry n when both are
+ * non-null (n + 1 entries). The normals come from Philox4x32-10 keyed by `seed`, with the
+ * satellite index as counter, through the Box-Muller transform; the reference draws them with
+ * cuRAND (seed 1234), so the two streams differ while the distribution is the same. Entry i
+ * depends only on (seed, i). Host AoS arrays of n (+ 1) x 3 doubles. */
+int ogp_perturbed_satellites(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double strength, uint64_t seed, const double* obj_r, const double* obj_v, double* r_out,
+    double* v_out);
+
+/* The same into device-resident SoA arrays: component c of entry i at [c * stride + i]
+ * (stride >= n + 1 with the appended object, else >= n), the layout ogp_propagate_device
+ * takes with stride = number of objects. */
+int ogp_perturbed_satellites_device(ogp_context* ctx, const double* r0, const double* v0,
+    int64_t n, double strength, uint64_t seed, const double* obj_r, const double* obj_v,
+    double* d_r_soa, double* d_v_soa, int64_t stride);
+
 /* Timing of the last propagate call, from HIP events on the context stream. */
 typedef struct ogp_timing {
     double total_ms;       /* all device work of the call (nodes + startup + steps) */

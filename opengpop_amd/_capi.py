@@ -74,6 +74,11 @@ def lib():
         "ogp_time_nodes": (ctypes.c_int, [_P, _D, ctypes.c_int, _P, _I64, _P, _P]),
         "ogp_propagate": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P]),
         "ogp_propagate_device": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P]),
+        "ogp_perturbed_satellites": (ctypes.c_int, [_P, _P, _P, _I64, ctypes.c_double,
+                                                    ctypes.c_uint64, _P, _P, _P, _P]),
+        "ogp_perturbed_satellites_device": (ctypes.c_int, [_P, _P, _P, _I64, ctypes.c_double,
+                                                           ctypes.c_uint64, _P, _P, _P, _P,
+                                                           _I64]),
         "ogp_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(OgpTiming)]),
         "ogp_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, _I64]),
         "ogp_set_slices": (ctypes.c_int, [_P, ctypes.c_int]),

@@ -22,7 +22,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", f"--offload-arch={ARCH}",
           "-I", os.path.join(ROOT, "include")]
 
 SOURCES = ["datasets.cpp", "kernels.hip", "propagate.hip", "conjunction.hip", "adjoint.hip",
-           "capi.cpp"]
+           "perturb.hip", "capi.cpp"]
 
 
 def _run(cmd):

@@ -873,6 +873,84 @@ int ogp_propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v
     });
 }
 
+// ---- Monte-Carlo populations -----------------------------------------------------------
+namespace {
+
+PerturbArgs perturb_args(const double* r0, const double* v0, int64_t n, double strength,
+    uint64_t seed, const double* obj_r, const double* obj_v)
+{
+    arg_check(r0 && v0, "null epoch state");
+    arg_check(n >= 0, "n must be >= 0");
+    arg_check(std::isfinite(strength), "strength must be finite");
+    PerturbArgs a {};
+    for (int c = 0; c < 3; ++c) {
+        a.r0[c] = r0[c];
+        a.v0[c] = v0[c];
+    }
+    // generatePerturbedSatellites appends the object only when both of its arrays are given
+    a.append = obj_r && obj_v ? 1 : 0;
+    if (a.append)
+        for (int c = 0; c < 3; ++c) {
+            a.obj_r[c] = obj_r[c];
+            a.obj_v[c] = obj_v[c];
+        }
+    a.strength = strength;
+    a.k0 = (uint32_t)seed;
+    a.k1 = (uint32_t)(seed >> 32);
+    a.n = n;
+    return a;
+}
+
+} // namespace
+
+int ogp_perturbed_satellites(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double strength, uint64_t seed, const double* obj_r, const double* obj_v, double* r_out,
+    double* v_out)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        PerturbArgs a = perturb_args(r0, v0, n, strength, seed, obj_r, obj_v);
+        const int64_t count = n + a.append;
+        arg_check(count == 0 || (r_out && v_out), "null output array");
+        if (count == 0)
+            return;
+        const size_t bytes = sizeof(double) * 3 * count;
+        ctx->io_a.reserve(bytes, &ctx->dev_bytes);
+        ctx->io_b.reserve(bytes, &ctx->dev_bytes);
+        a.r = ctx->io_a.as<double>();
+        a.v = ctx->io_b.as<double>();
+        a.stride = 0;
+        launch_perturb(ctx->stream, a);
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(r_out, a.r, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(v_out, a.v, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    });
+}
+
+int ogp_perturbed_satellites_device(ogp_context* ctx, const double* r0, const double* v0,
+    int64_t n, double strength, uint64_t seed, const double* obj_r, const double* obj_v,
+    double* d_r_soa, double* d_v_soa, int64_t stride)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        PerturbArgs a = perturb_args(r0, v0, n, strength, seed, obj_r, obj_v);
+        const int64_t count = n + a.append;
+        arg_check(count == 0 || (d_r_soa && d_v_soa), "null output array");
+        arg_check(stride >= count && stride >= 1, "stride must be at least the number of entries");
+        if (count == 0)
+            return;
+        a.r = d_r_soa;
+        a.v = d_v_soa;
+        a.stride = stride;
+        launch_perturb(ctx->stream, a);
+        ctx->check_launch();
+        ctx->sync();
+    });
+}
+
 int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps)
 {
     if (!ctx)

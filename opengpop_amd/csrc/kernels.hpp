@@ -41,6 +41,22 @@ struct StartupScratch {
     double* Kv;     // 10 x 3
 };
 
+// Monte-Carlo population (perturb.hip): n satellites at r0 with velocity v0 + strength * N(0, 1)
+// per component, normals from Philox4x32-10 keyed by (k0, k1) with the satellite index as
+// counter; with `append`, (obj_r, obj_v) is written as entry n. AoS output ([i][3]) when
+// stride == 0, else SoA with component c of entry i at [c * stride + i] (stride >= n + append).
+struct PerturbArgs {
+    double r0[3], v0[3], obj_r[3], obj_v[3];
+    double strength;
+    uint32_t k0, k1;
+    int64_t n;
+    int append;
+    double* r;
+    double* v;
+    int64_t stride;
+};
+void launch_perturb(hipStream_t st, const PerturbArgs& a);
+
 // Kernel variants: force kind (ForceKind; kGravitySplit for the split-precision gravity).
 void launch_time_nodes(hipStream_t st, const FrameDev& fd, const int* ymdhm, double sec_shift,
     const double* d_offsets, int64_t len, double* d_M, double* d_mjd);

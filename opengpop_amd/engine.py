@@ -104,6 +104,38 @@ class Context:
         check(self.h, lib().ogp_propagate_device(self.h, addr[0], addr[1], n, ptr(h), ptr(ns),
                                                  addr[2], addr[3]))
 
+    def perturbed_satellites(self, r0, v0, n, strength, seed=1234, obj_r=None, obj_v=None):
+        """PerturbationGenerator::generatePerturbedSatellites: (r, v) host arrays [n (+1)][3]
+        of n satellites at r0 with velocity v0 + strength * N(0, 1) per component, plus
+        (obj_r, obj_v) as the last entry when both are given."""
+        r0 = f64(r0).reshape(3)
+        v0 = f64(v0).reshape(3)
+        append = obj_r is not None and obj_v is not None
+        orr = f64(obj_r).reshape(3) if append else None
+        ov = f64(obj_v).reshape(3) if append else None
+        count = max(n + (1 if append else 0), 0)  # a negative n is reported by the library
+        r = np.zeros((count, 3))
+        v = np.zeros((count, 3))
+        check(self.h, lib().ogp_perturbed_satellites(self.h, ptr(r0), ptr(v0), n, strength, seed,
+                                                     ptr(orr), ptr(ov), ptr(r), ptr(v)))
+        return r, v
+
+    def perturbed_satellites_device(self, r0, v0, n, strength, r_soa, v_soa, stride=None,
+                                    seed=1234, obj_r=None, obj_v=None):
+        """ogp_perturbed_satellites_device into device SoA arrays (device addresses: ints or
+        objects with data_ptr()); stride defaults to the number of entries."""
+        r0 = f64(r0).reshape(3)
+        v0 = f64(v0).reshape(3)
+        append = obj_r is not None and obj_v is not None
+        orr = f64(obj_r).reshape(3) if append else None
+        ov = f64(obj_v).reshape(3) if append else None
+        if stride is None:
+            stride = n + (1 if append else 0)
+        addr = [a.data_ptr() if hasattr(a, "data_ptr") else int(a) for a in (r_soa, v_soa)]
+        check(self.h, lib().ogp_perturbed_satellites_device(
+            self.h, ptr(r0), ptr(v0), n, strength, seed, ptr(orr), ptr(ov), addr[0], addr[1],
+            stride))
+
     def set_tuning(self, waves=0, chunk_steps=0):
         check(self.h, lib().ogp_set_tuning(self.h, waves, chunk_steps))
 

@@ -1556,4 +1556,60 @@ int orc_refine(const double* traj, int nobj, int nsteps, double h, unsigned a, u
     return 0;
 }
 
+
+// Philox4x32-10: round r maps (x0, x1, x2, x3) with key (k0, k1) to
+// (hi(B*x2) ^ x1 ^ k0, lo(B*x2), hi(A*x0) ^ x3 ^ k1, lo(A*x0)), A = 0xD2511F53, B = 0xCD9E8D57;
+// the key grows by (0x9E3779B9, 0xBB67AE85) after every round.
+void orc_philox4x32_10(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4)
+{
+    uint32_t x[4] = { ctr4[0], ctr4[1], ctr4[2], ctr4[3] };
+    uint32_t k[2] = { key2[0], key2[1] };
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t pa = static_cast<uint64_t>(0xD2511F53u) * x[0];
+        const uint64_t pb = static_cast<uint64_t>(0xCD9E8D57u) * x[2];
+        const uint32_t y0 = static_cast<uint32_t>(pb >> 32) ^ x[1] ^ k[0];
+        const uint32_t y1 = static_cast<uint32_t>(pb & 0xffffffffu);
+        const uint32_t y2 = static_cast<uint32_t>(pa >> 32) ^ x[3] ^ k[1];
+        const uint32_t y3 = static_cast<uint32_t>(pa & 0xffffffffu);
+        x[0] = y0;
+        x[1] = y1;
+        x[2] = y2;
+        x[3] = y3;
+        k[0] += 0x9E3779B9u;
+        k[1] += 0xBB67AE85u;
+    }
+    for (int j = 0; j < 4; ++j)
+        out4[j] = x[j];
+}
+
+void orc_perturbed(const double* r0, const double* v0, long n, double strength, uint64_t seed,
+    double* r, double* v)
+{
+    const uint32_t key[2] = { static_cast<uint32_t>(seed & 0xffffffffu),
+        static_cast<uint32_t>(seed >> 32) };
+    // 53 bits of a word pair -> (0, 1), never 0 or 1
+    auto unit = [](uint32_t hi, uint32_t lo) {
+        const uint64_t m = (static_cast<uint64_t>(hi) << 21) | (lo >> 11);
+        return (static_cast<double>(m) + 0.5) / 9007199254740992.0;
+    };
+    for (long i = 0; i < n; ++i) {
+        const uint64_t idx = static_cast<uint64_t>(i);
+        double z[4];
+        for (uint32_t blk = 0; blk < 2; ++blk) {
+            const uint32_t ctr[4] = { static_cast<uint32_t>(idx & 0xffffffffu),
+                static_cast<uint32_t>(idx >> 32), blk, 0u };
+            uint32_t w[4];
+            orc_philox4x32_10(ctr, key, w);
+            const double rad = std::sqrt(-2.0 * std::log(unit(w[0], w[1])));
+            const double ang = (2.0 * kPi) * unit(w[2], w[3]);
+            z[2 * blk] = rad * std::cos(ang);
+            z[2 * blk + 1] = rad * std::sin(ang);
+        }
+        for (int c = 0; c < 3; ++c) {
+            r[3 * i + c] = r0[c];
+            v[3 * i + c] = v0[c] + z[c] * strength;
+        }
+    }
+}
+
 } // extern "C"

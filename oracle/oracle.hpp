@@ -14,6 +14,7 @@
 //   lib/gpu_startup/src/gpu_startup.cu + src/device/*.cu (RK8 and startup corrector)
 //   lib/gpu_integrator/src/*.cu, lib/gpu_propagator/src/*.cu (GJ8 predictor / corrector)
 //   lib/closest_approach/** (partition, screening, Hermite + secant refinement)
+//   lib/perturbation_generator/src/perturbation_generator.cu (Monte-Carlo population)
 // Every summation that the reference hands to cuBLAS or Thrust is done here sequentially in
 // index order; the reference's own GPU reduction order is not reproducible.
 //
@@ -48,6 +49,15 @@ double orc_leap_seconds(double mjd_utc);
 // times: node offsets in seconds from the epoch (already shifted by the caller).
 int orc_time_nodes(const int* ymdhm, double sec, const double* times, int len, double* M,
     double* mjd);
+
+// ---- Monte-Carlo population (perturbation_generator.cu:21-79) ----------------------------
+// The reference's normals come from cuRAND; the product uses Philox4x32-10 (Salmon, Moraes,
+// Dror and Shaw, SC 2011) with Box-Muller, restated here. orc_philox4x32_10 is one 10-round
+// block (pinned by the published known-answer vectors); orc_perturbed writes n satellites (AoS)
+// at r0 with velocity v0 + strength * z, z = the normals of counter blocks (i, 0) and (i, 1).
+void orc_philox4x32_10(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);
+void orc_perturbed(const double* r0, const double* v0, long n, double strength, uint64_t seed,
+    double* r, double* v);
 
 // ---- gravity (gravity_source*.cu) ------------------------------------------------------
 int orc_gravity_itrs(const double* r, long n, double* a);

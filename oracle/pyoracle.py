@@ -59,6 +59,8 @@ class Oracle:
         L.orc_time_nodes.argtypes = [_P, _D, _P, _I, _P, _P]
         L.orc_gravity_itrs.argtypes = [_P, ctypes.c_long, _P]
         L.orc_potential_itrs.restype = _D
+        L.orc_philox4x32_10.argtypes = [_P, _P, _P]
+        L.orc_perturbed.argtypes = [_P, _P, ctypes.c_long, _D, ctypes.c_uint64, _P, _P]
         L.orc_potential_itrs.argtypes = [_P]
         L.orc_gj_predict.argtypes = [_P, _P, _P, _D, _P, _P]
         L.orc_gj_finalize_correction.argtypes = [_P, _P, _P, _P, _P, _D]
@@ -174,6 +176,23 @@ class Oracle:
         return self.L.orc_potential_itrs(_ptr(r))
 
     # -- integrator pieces -----------------------------------------------------------------
+    def philox4x32_10(self, ctr, key):
+        """One Philox4x32-10 block: four uint32 counter words and two key words."""
+        c = np.ascontiguousarray(ctr, dtype=np.uint32)
+        k = np.ascontiguousarray(key, dtype=np.uint32)
+        out = np.zeros(4, np.uint32)
+        self.L.orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
+        return out
+
+    def perturbed(self, r0, v0, n, strength, seed=1234):
+        """(r, v) [n][3] of the Monte-Carlo population (generator's normals restated)."""
+        r0 = np.ascontiguousarray(r0, dtype=np.float64)
+        v0 = np.ascontiguousarray(v0, dtype=np.float64)
+        r = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        self.L.orc_perturbed(_ptr(r0), _ptr(v0), n, strength, seed, _ptr(r), _ptr(v))
+        return r, v
+
     def gj_predict(self, acc27, s3, S3, h):
         acc = np.ascontiguousarray(acc27, dtype=np.float64)
         s = np.ascontiguousarray(s3, dtype=np.float64)
