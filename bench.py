@@ -1,7 +1,7 @@
 """Benchmark of the propagation and conjunction hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--legs fp64,split,pairs,adjoint,cpu,refpoint,perturb,copy]
+                    [--legs fp64,split,pairs,adjoint,cpu,refpoint,sources,perturb,copy]
 
 Prints ONE JSON line. The headline `value` is FP64 object-steps/s of the mixed-regime workload:
 100,000 objects (80% LEO at 30 s, 15% MEO and 5% GEO at 300 s, spans U(6, 24) h), JGM3
@@ -19,6 +19,8 @@ complete propagation of that population. Side legs (same line):
             timed on this host's cores; also the GPU-vs-oracle difference on that sample
   refpoint  10,000 LEO, 30 s, 720 steps, degree 70, end to end from host arrays (the workload
             of the reference's published H100 figure)
+  sources   the headline workload with the Sun, the Moon and drag added to gravity (the
+            acceleration-source extension point): object-steps/s and cost vs gravity alone
   perturb   the Monte-Carlo population generator: 10^8 satellites written to HBM (satellites/s
             and the write bandwidth against the HBM peak)
   copy      a 1 GiB device copy, the calibration dispatch of the HBM byte counters (not default)
@@ -73,7 +75,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--legs", default="fp64,split,pairs,adjoint,cpu,refpoint,perturb")
+    p.add_argument("--legs", default="fp64,split,pairs,adjoint,cpu,refpoint,sources,perturb")
     p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     return p.parse_args()
 
@@ -314,6 +316,32 @@ def leg_adjoint(ogp, scenarios, with_oracle):
     return out
 
 
+def leg_sources(ogp, scenarios, torch, gravity_only_ms):
+    """The headline workload with the Sun, the Moon and drag (C_D A / m = 0.01 m^2/kg) added to
+    the degree-70 gravity inside the integrator kernels: object-steps/s and the cost relative to
+    gravity alone."""
+    c1 = scenarios.config1()
+    n = len(c1["r"])
+    dev = torch.device("cuda:0")
+    r0 = torch.tensor(np.ascontiguousarray(c1["r"].T), device=dev)
+    v0 = torch.tensor(np.ascontiguousarray(c1["v"].T), device=dev)
+    r = torch.empty_like(r0)
+    v = torch.empty_like(v0)
+    with ogp.Context(degree=70) as ctx:
+        ctx.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON | ogp.SOURCE_DRAG, 0.01)
+        ctx.propagate_device(r0, v0, n, c1["step"], c1["steps"], r, v)  # tuning, tables
+        best = 1e30
+        for _ in range(2):
+            ctx.propagate_device(r0, v0, n, c1["step"], c1["steps"], r, v)
+            best = min(best, ctx.timing()["total_ms"])
+    rate = float(c1["steps"].sum()) / (best * 1e-3)
+    out = dict(workload="the headline workload with Sun + Moon + drag (C_D A/m = 0.01 m^2/kg)",
+               object_steps_per_s=rate, device_ms=best)
+    if gravity_only_ms:
+        out["cost_vs_gravity_only"] = best / gravity_only_ms
+    return out
+
+
 def leg_perturb(ogp, torch):
     """Monte-Carlo population in HBM: 10^8 satellites around one LEO state (SoA, 48 B written per
     satellite, nothing read), timed around the synchronous device entry point."""
@@ -402,6 +430,10 @@ def main():
         log(f"adjoint leg: jacobian {line['adjoint']['jacobian_cost_vs_forward']:.1f}x forward")
     if "refpoint" in legs:
         line["reference_point"] = leg_refpoint(ogp, scenarios)
+    if "sources" in legs:
+        line["sources"] = leg_sources(ogp, scenarios, torch,
+                                      line["device"]["total_ms"] if "device" in line else None)
+        log(f"sources leg: {line['sources']['object_steps_per_s']:.3e} object-steps/s")
     if "perturb" in legs:
         line["perturb"] = leg_perturb(ogp, torch)
         log(f"perturb leg: {line['perturb']['satellites_per_s']:.3e} satellites/s")

@@ -86,10 +86,36 @@ int ogp_gravity_itrs(ogp_context* ctx, const double* r, int64_t n, double* a);
 
 /* ForceModel::ComputePropagationAcceleration (force_model.cu:91-109): GCRS positions at
  * propagation node `node` of a run with step `step_s` (node time (node - 4) * step_s relative
- * to the epoch) -> GCRS accelerations. out_stride = 3 * reference outStride (doubles between
- * consecutive objects' outputs). Host arrays. */
+ * to the epoch) -> GCRS accelerations: gravity plus the enabled sources (ogp_set_sources).
+ * out_stride = 3 * reference outStride (doubles between consecutive objects' outputs).
+ * v (AoS velocities) may be null unless drag is enabled. Host arrays. */
 int ogp_accel(ogp_context* ctx, const double* r, const double* v, int64_t n, double step_s,
     int64_t node, double* a, int64_t out_stride);
+
+/* Acceleration sources added to gravity: the reference's IAccelerationSource extension point
+ * (iacceleration_source.cuh:19-33), whose outputs ForceModel sums (force_model.cu:73-86); the
+ * reference implements gravity only and names drag and third bodies as further sources. Here
+ * the sources run inside the integrator kernels after gravity, added in the order Sun, Moon,
+ * drag, in every propagation, conjunction and ogp_accel call of the context:
+ *   OGP_SOURCE_SUN, OGP_SOURCE_MOON: point masses (GM 1.32712440018e20 and 4.9028e12 m^3/s^2)
+ *     at low-precision analytic positions (Montenbruck and Gill 2000, eq. 3.43-3.49) computed
+ *     once per time node, a += GM ((s - r)/|s - r|^3 - s/|s|^3);
+ *   OGP_SOURCE_DRAG: a cannonball of ballistic coefficient drag_bc = C_D A / m (m^2/kg) in an
+ *     exponential atmosphere (Vallado, Table 8-4) rotating with the Earth,
+ *     a += -1/2 drag_bc rho |v_r| v_r, v_r = v - omega_E z x r.
+ * mask = 0 (the default) is the reference's gravity-only model, bit for bit. The derivative
+ * entry points (ogp_stm, ogp_adjoint_*) require mask = 0. Gravity contexts only. */
+enum {
+    OGP_SOURCE_SUN = 1,
+    OGP_SOURCE_MOON = 2,
+    OGP_SOURCE_DRAG = 4
+};
+typedef struct ogp_sources {
+    int mask;       /* OGP_SOURCE_* bits */
+    double drag_bc; /* C_D A / m of every object, m^2/kg */
+} ogp_sources;
+int ogp_set_sources(ogp_context* ctx, const ogp_sources* sources);
+int ogp_get_sources(const ogp_context* ctx, ogp_sources* sources);
 
 /* FMTimePoints::CalculateParameters + GetParametersDev (time_points.cu:126-207): the
  * ITRS->GCRS matrix (column-major, 9 per node) and MJD(UTC) of `len` nodes at offsets

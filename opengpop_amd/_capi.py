@@ -37,6 +37,13 @@ class OgpTiming(ctypes.Structure):
                  "step_launches", "object_steps", "gj_object_steps", "launch_ms_sum")]
 
 
+class OgpSources(ctypes.Structure):
+    _fields_ = [("mask", ctypes.c_int), ("drag_bc", ctypes.c_double)]
+
+
+SOURCE_SUN, SOURCE_MOON, SOURCE_DRAG = 1, 2, 4
+
+
 class OgpConjTiming(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("propagate_ms", "screen_ms", "refine_ms", "pair_steps", "pairs", "candidates")]
@@ -82,6 +89,8 @@ def lib():
         "ogp_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(OgpTiming)]),
         "ogp_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, _I64]),
         "ogp_set_slices": (ctypes.c_int, [_P, ctypes.c_int]),
+        "ogp_set_sources": (ctypes.c_int, [_P, ctypes.POINTER(OgpSources)]),
+        "ogp_get_sources": (ctypes.c_int, [_P, ctypes.POINTER(OgpSources)]),
         "ogp_get_tuning": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(_I64)]),
         "ogp_prop_init": (ctypes.c_int, [_P, _P, _P, _I64, _D, _I64]),

@@ -84,6 +84,7 @@ struct DevBuf {
 // Time-node tables of one step-size class (FMTimePoints, time_points.cu:126-207).
 struct NodeTable {
     DevBuf gen_M, prop_M, prop_mjd, gen_mjd, offsets;
+    DevBuf gen_tb, prop_tb; // Sun and Moon GCRS positions per node (6 doubles)
     int64_t n_prop = 0;
 };
 
@@ -203,12 +204,34 @@ struct ogp_context {
             && tuned_waves.find(group_bucket(n)) == tuned_waves.end();
     }
 
+    // Acceleration sources added to gravity (ogp_set_sources).
+    ogp_sources sources {};
+    SourcesDev sources_dev() const
+    {
+        SourcesDev d {};
+        if (kind == kGravity || kind == kGravitySplit) {
+            d.mask = sources.mask;
+            d.drag_bc = sources.drag_bc;
+        }
+        return d;
+    }
+    // The derivative paths linearise gravity alone.
+    void require_gravity_only(const char* what) const
+    {
+        if (sources.mask != 0)
+            throw ApiError(OGP_ERR_STATE,
+                std::string(what) + " covers gravity only; clear the sources (ogp_set_sources)");
+    }
+
     ForceCtx force_ctx(const NodeTable* nt, double h) const
     {
-        ForceCtx f;
+        ForceCtx f {};
         f.grav = gdev;
         f.gen_M = nt ? nt->gen_M.as<double>() : nullptr;
         f.prop_M = nt ? nt->prop_M.as<double>() : nullptr;
+        f.gen_tb = nt ? nt->gen_tb.as<double>() : nullptr;
+        f.prop_tb = nt ? nt->prop_tb.as<double>() : nullptr;
+        f.src = nt ? sources_dev() : SourcesDev {};
         f.h = h;
         return f;
     }
@@ -251,10 +274,11 @@ struct ogp_context {
         nt.offsets.reserve(sizeof(double) * len, &dev_bytes);
         nt.gen_M.reserve(sizeof(double) * 9 * 80, &dev_bytes);
         nt.gen_mjd.reserve(sizeof(double) * 80, &dev_bytes);
+        nt.gen_tb.reserve(sizeof(double) * 6 * 80, &dev_bytes);
         HIPC(hipMemcpyAsync(nt.offsets.p, off.data(), sizeof(double) * 80, hipMemcpyHostToDevice,
             stream));
         launch_time_nodes(stream, fdev, ymdhm, shifted, nt.offsets.as<double>(), 80,
-            nt.gen_M.as<double>(), nt.gen_mjd.as<double>());
+            nt.gen_M.as<double>(), nt.gen_mjd.as<double>(), nt.gen_tb.as<double>());
         check_launch();
         sync(); // offsets buffer is reused below
         off.resize(need);
@@ -262,10 +286,11 @@ struct ogp_context {
             off[i] = (double)i * h;
         nt.prop_M.reserve(sizeof(double) * 9 * need, &dev_bytes);
         nt.prop_mjd.reserve(sizeof(double) * need, &dev_bytes);
+        nt.prop_tb.reserve(sizeof(double) * 6 * need, &dev_bytes);
         HIPC(hipMemcpyAsync(nt.offsets.p, off.data(), sizeof(double) * need, hipMemcpyHostToDevice,
             stream));
         launch_time_nodes(stream, fdev, ymdhm, shifted, nt.offsets.as<double>(), need,
-            nt.prop_M.as<double>(), nt.prop_mjd.as<double>());
+            nt.prop_M.as<double>(), nt.prop_mjd.as<double>(), nt.prop_tb.as<double>());
         check_launch();
         sync();
         nt.n_prop = need;
@@ -673,14 +698,17 @@ int ogp_accel(ogp_context* ctx, const double* r, const double* v, int64_t n, dou
 {
     if (!ctx)
         return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
-    (void)v;
     return guarded(ctx, [&] {
         arg_check(n >= 0 && step_s > 0 && node >= 0 && out_stride >= 3, "invalid accel arguments");
         if (ctx->cfg.force != OGP_FORCE_GRAVITY)
             throw ApiError(OGP_ERR_STATE, "context was created without the gravity field");
+        const SourcesDev src = ctx->sources_dev();
+        arg_check(v || !(src.mask & kSourceDrag), "drag needs the velocities");
         if (n == 0)
             return;
         check_finite(r, 3 * n, "positions");
+        if (v)
+            check_finite(v, 3 * n, "velocities");
         NodeTable* nt = ctx->node_table(step_s, node);
         const int64_t out_len = out_stride * (n - 1) + 3;
         ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
@@ -689,9 +717,15 @@ int ogp_accel(ogp_context* ctx, const double* r, const double* v, int64_t n, dou
             ctx->stream));
         HIPC(hipMemcpyAsync(ctx->io_b.p, a, sizeof(double) * out_len, hipMemcpyHostToDevice,
             ctx->stream));
+        if (v) {
+            ctx->io_c.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+            HIPC(hipMemcpyAsync(ctx->io_c.p, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                ctx->stream));
+        }
         launch_gravity_aos(ctx->stream, ctx->kind == kGravitySplit, ctx->gdev,
             nt->prop_M.as<double>() + 9 * node, ctx->io_a.as<double>(), n, ctx->io_b.as<double>(),
-            out_stride, 64);
+            out_stride, 64, src, nt->prop_tb.as<double>() + 6 * node,
+            v ? ctx->io_c.as<double>() : nullptr);
         ctx->check_launch();
         HIPC(hipMemcpyAsync(a, ctx->io_b.p, sizeof(double) * out_len, hipMemcpyDeviceToHost,
             ctx->stream));
@@ -970,6 +1004,29 @@ int ogp_set_slices(ogp_context* ctx, int slices)
     if (slices < 0 || slices > 16)
         return fail(ctx, OGP_ERR_ARGUMENT, "slices must be 0 (automatic) .. 16");
     ctx->slices_override = slices;
+    return OGP_OK;
+}
+
+int ogp_set_sources(ogp_context* ctx, const ogp_sources* sources)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(sources != nullptr, "null sources");
+        arg_check((sources->mask & ~kSourceMask) == 0, "unknown source bit");
+        arg_check(std::isfinite(sources->drag_bc) && sources->drag_bc >= 0.0,
+            "drag_bc must be finite and >= 0");
+        if (sources->mask != 0 && ctx->cfg.force != OGP_FORCE_GRAVITY)
+            throw ApiError(OGP_ERR_STATE, "sources are added to the gravity force model only");
+        ctx->sources = *sources;
+    });
+}
+
+int ogp_get_sources(const ogp_context* ctx, ogp_sources* sources)
+{
+    if (!ctx || !sources)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null argument");
+    *sources = ctx->sources;
     return OGP_OK;
 }
 
@@ -1503,6 +1560,7 @@ void check_derivative_inputs(ogp_context* ctx, const double* r0, const double* v
     if (ctx->kind != kGravity)
         throw ApiError(OGP_ERR_ARGUMENT,
             "derivatives need the FP64 gravity force model (OGP_FORCE_GRAVITY, OGP_PRECISION_FP64)");
+    ctx->require_gravity_only("the derivative");
     arg_check(n > 0 && r0 && v0 && steps, "null array or no objects");
     arg_check(std::isfinite(step_s) && step_s > 0, "step size must be positive and finite");
     arg_check(ctx->cfg.n_corrections >= 1 && ctx->cfg.n_corrections <= adjoint_max_corrections(),

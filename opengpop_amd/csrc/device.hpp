@@ -438,10 +438,106 @@ __device__ __forceinline__ void analytic_force(const double* p, const double* v,
 }
 
 // Everything a force evaluation needs for one step-size class.
+// ------------------------------------------------------------------------------------------
+// Acceleration sources added to gravity. The reference's ForceModel sums a list of
+// IAccelerationSource outputs with a matrix-vector product (iacceleration_source.cuh:19-33,
+// force_model.cu:73-86); only gravity is implemented there. Here every source is a device
+// function of the state and of its per-node data, evaluated inside the integrator kernels after
+// gravity and added in a fixed order (Sun, Moon, drag), so no per-source buffer exists. A new
+// source is one more function called from add_sources with its own mask bit and node data.
+// ------------------------------------------------------------------------------------------
+enum SourceBit { kSourceSun = 1, kSourceMoon = 2, kSourceDrag = 4 };
+constexpr int kSourceMask = kSourceSun | kSourceMoon | kSourceDrag;
+constexpr double kGmSun = 1.32712440018e20; // m^3 / s^2
+constexpr double kGmMoon = 4.9028e12;       // m^3 / s^2
+constexpr double kOmegaEarth = 7.292115e-5; // rad / s, rotation of the atmosphere about GCRS z
+constexpr double kAtmRadius = 6378137.0;    // m, altitude reference of the atmosphere
+constexpr int kAtmBands = 28;
+
+// Exponential atmosphere (Vallado, Fundamentals of Astrodynamics and Applications, 4th ed.,
+// Table 8-4): base altitude [km], base density [kg/m^3], scale height [km] per band.
+static __constant__ const double kAtmosphere[kAtmBands][3] = {
+    { 0, 1.225, 7.249 }, { 25, 3.899e-2, 6.349 }, { 30, 1.774e-2, 6.682 },
+    { 40, 3.972e-3, 7.554 }, { 50, 1.057e-3, 8.382 }, { 60, 3.206e-4, 7.714 },
+    { 70, 8.770e-5, 6.549 }, { 80, 1.905e-5, 5.799 }, { 90, 3.396e-6, 5.382 },
+    { 100, 5.297e-7, 5.877 }, { 110, 9.661e-8, 7.263 }, { 120, 2.438e-8, 9.473 },
+    { 130, 8.484e-9, 12.636 }, { 140, 3.845e-9, 16.149 }, { 150, 2.070e-9, 22.523 },
+    { 180, 5.464e-10, 29.740 }, { 200, 2.789e-10, 37.105 }, { 250, 7.248e-11, 45.546 },
+    { 300, 2.418e-11, 53.628 }, { 350, 9.518e-12, 53.298 }, { 400, 3.725e-12, 58.515 },
+    { 450, 1.585e-12, 60.828 }, { 500, 6.967e-13, 63.822 }, { 600, 1.454e-13, 71.835 },
+    { 700, 3.614e-14, 88.667 }, { 800, 1.170e-14, 124.64 }, { 900, 5.245e-15, 181.05 },
+    { 1000, 3.019e-15, 268.00 }
+};
+
+struct SourcesDev {
+    int mask;       // SourceBit set; 0 = gravity only
+    double drag_bc; // C_D A / m, m^2 / kg
+};
+
+// Point mass at s (GCRS, m): a += GM ((s - r) / |s - r|^3 - s / |s|^3).
+__device__ __forceinline__ void third_body(double gm, const double* s, const double* r, double* a)
+{
+    const double d0 = s[0] - r[0], d1 = s[1] - r[1], d2 = s[2] - r[2];
+    const double dd = fma(d2, d2, fma(d1, d1, d0 * d0));
+    const double ss = fma(s[2], s[2], fma(s[1], s[1], s[0] * s[0]));
+    const double kd = gm / (dd * sqrt(dd)), ks = gm / (ss * sqrt(ss));
+    a[0] += fma(kd, d0, -ks * s[0]);
+    a[1] += fma(kd, d1, -ks * s[1]);
+    a[2] += fma(kd, d2, -ks * s[2]);
+}
+
+// Density at altitude h (m) above the reference sphere: the band whose base is the highest one
+// at or below h (the first band below the surface, the last one above 1000 km).
+__device__ __forceinline__ double atm_density(double h)
+{
+    const double km = h * 1e-3;
+    int b = 0;
+    for (int q = 1; q < kAtmBands; ++q)
+        b = km >= kAtmosphere[q][0] ? q : b;
+    return kAtmosphere[b][1] * exp((kAtmosphere[b][0] - km) / kAtmosphere[b][2]);
+}
+
+// Drag on a cannonball in an atmosphere rotating with the Earth:
+// a += -1/2 (C_D A / m) rho |v_r| v_r with v_r = v - omega x r, omega = (0, 0, omega_E).
+__device__ __forceinline__ void drag(double bc, const double* r, const double* v, double* a)
+{
+    const double rr = sqrt(fma(r[2], r[2], fma(r[1], r[1], r[0] * r[0])));
+    const double vr0 = fma(kOmegaEarth, r[1], v[0]), vr1 = fma(-kOmegaEarth, r[0], v[1]),
+                 vr2 = v[2];
+    const double vn = sqrt(fma(vr2, vr2, fma(vr1, vr1, vr0 * vr0)));
+    const double k = -0.5 * bc * atm_density(rr - kAtmRadius) * vn;
+    a[0] = fma(k, vr0, a[0]);
+    a[1] = fma(k, vr1, a[1]);
+    a[2] = fma(k, vr2, a[2]);
+}
+
+// The sources' epilogue at one node: tb = that node's Sun and Moon GCRS positions (6 doubles,
+// wave-uniform).
+__device__ __forceinline__ void add_sources(const SourcesDev& src, const double* tb_node,
+    const double* r, const double* v, double* a)
+{
+    if (src.mask == 0)
+        return;
+    const uniform_dptr tb = (uniform_dptr)tb_node;
+    if (src.mask & kSourceSun) {
+        const double s[3] = { tb[0], tb[1], tb[2] };
+        third_body(kGmSun, s, r, a);
+    }
+    if (src.mask & kSourceMoon) {
+        const double s[3] = { tb[3], tb[4], tb[5] };
+        third_body(kGmMoon, s, r, a);
+    }
+    if (src.mask & kSourceDrag)
+        drag(src.drag_bc, r, v, a);
+}
+
 struct ForceCtx {
     GravityDev grav;
     const double* gen_M;  // 80 RK8 stage nodes x 9
     const double* prop_M; // propagation nodes x 9 (node j at time (j - 4) h + t0)
+    const double* gen_tb; // Sun and Moon GCRS positions of the same nodes, 6 per node
+    const double* prop_tb;
+    SourcesDev src;
     double h;
 };
 
@@ -459,6 +555,7 @@ __device__ __forceinline__ void force_gen(const ForceCtx& f, const double* x, co
     if (KIND == kGravity || KIND == kGravitySplit) {
         const int idx = ((s > 0 ? s - 1 : s) + 4) * 10 + k;
         gravity_gcrs<KIND == kGravitySplit, W>(f.gen_M + 9 * idx, x, f.grav, a, ex.red);
+        add_sources(f.src, f.gen_tb + 6 * idx, x, v, a);
     } else if (KIND == kGravityTangent) {
         const int idx = ((s > 0 ? s - 1 : s) + 4) * 10 + k;
         tangent_force(f.gen_M + 9 * idx, x, f.grav, a);
@@ -476,6 +573,7 @@ __device__ __forceinline__ void force_prop(const ForceCtx& f, const double* x, c
 {
     if (KIND == kGravity || KIND == kGravitySplit) {
         gravity_gcrs<KIND == kGravitySplit, W>(f.prop_M + 9 * node, x, f.grav, a, ex.red);
+        add_sources(f.src, f.prop_tb + 6 * node, x, v, a);
     } else if (KIND == kGravityTangent) {
         tangent_force(f.prop_M + 9 * node, x, f.grav, a);
     } else {

@@ -189,10 +189,57 @@ __device__ double series(const double* terms, int n, const double* F, int lane)
     return acc;
 }
 
+// Sun and Moon positions (m) in the mean equator and equinox of J2000 (taken as the GCRS, whose
+// axes differ by the 23 mas frame bias) at T Julian centuries of TT from J2000: the low-precision
+// series of Montenbruck and Gill, Satellite Orbits (2000), eq. 3.43-3.49, referred to the J2000
+// ecliptic (about 0.1 deg and 0.1% in distance; the third-body accelerations they feed are
+// 1e-7 of gravity).
+__device__ void sun_moon(double T, double* out6)
+{
+    constexpr double D = kPi / 180.0, AS = D / 3600.0;
+    constexpr double kEps = 23.43929111 * D; // obliquity of the J2000 ecliptic
+    const double ce = cos(kEps), se = sin(kEps);
+    auto put = [&](double lam, double beta, double r, double* o) {
+        const double cb = cos(beta);
+        const double x = r * cb * cos(lam), y = r * cb * sin(lam), z = r * sin(beta);
+        o[0] = x;
+        o[1] = ce * y - se * z;
+        o[2] = se * y + ce * z;
+    };
+    // Sun
+    const double Ms = D * (357.5256 + 35999.049 * T);
+    const double lam_s = D * 282.94 + Ms + AS * (6892.0 * sin(Ms) + 72.0 * sin(2.0 * Ms));
+    const double r_s = (149.619 - 2.499 * cos(Ms) - 0.021 * cos(2.0 * Ms)) * 1e9;
+    put(lam_s, 0.0, r_s, out6);
+    // Moon
+    const double L0 = D * (218.31617 + 481267.88088 * T - 1.3972 * T);
+    const double l = D * (134.96292 + 477198.86753 * T);
+    const double lp = D * (357.52543 + 35999.04944 * T);
+    const double F = D * (93.27283 + 483202.01873 * T);
+    const double Dm = D * (297.85027 + 445267.11135 * T);
+    const double lam_m = L0
+        + AS * (22640.0 * sin(l) + 769.0 * sin(2.0 * l) - 4586.0 * sin(l - 2.0 * Dm)
+            + 2370.0 * sin(2.0 * Dm) - 668.0 * sin(lp) - 412.0 * sin(2.0 * F)
+            - 212.0 * sin(2.0 * l - 2.0 * Dm) - 206.0 * sin(l + lp - 2.0 * Dm)
+            + 192.0 * sin(l + 2.0 * Dm) - 165.0 * sin(lp - 2.0 * Dm) + 148.0 * sin(l - lp)
+            - 125.0 * sin(Dm) - 110.0 * sin(l + lp) - 55.0 * sin(2.0 * F - 2.0 * Dm));
+    const double beta_m = AS
+        * (18520.0 * sin(F + lam_m - L0 + AS * (412.0 * sin(2.0 * F) + 541.0 * sin(lp)))
+            - 526.0 * sin(F - 2.0 * Dm) + 44.0 * sin(l + F - 2.0 * Dm)
+            - 31.0 * sin(-l + F - 2.0 * Dm) - 25.0 * sin(-2.0 * l + F)
+            - 23.0 * sin(lp + F - 2.0 * Dm) + 21.0 * sin(-l + F) + 11.0 * sin(-lp + F - 2.0 * Dm));
+    const double r_m = 1e3
+        * (385000.0 - 20905.0 * cos(l) - 3699.0 * cos(2.0 * Dm - l) - 2956.0 * cos(2.0 * Dm)
+            - 570.0 * cos(2.0 * l) + 246.0 * cos(2.0 * l - 2.0 * Dm) - 205.0 * cos(lp - 2.0 * Dm)
+            - 171.0 * cos(l + 2.0 * Dm) - 152.0 * cos(l + lp - 2.0 * Dm));
+    put(lam_m, beta_m, r_m, out6 + 3);
+}
+
 } // namespace frames
 
 __global__ void __launch_bounds__(64) time_nodes_kernel(FrameDev fd, int y, int mo, int d, int hh,
-    int mi, double sec_shift, const double* offsets, int64_t len, double* M, double* mjd_out)
+    int mi, double sec_shift, const double* offsets, int64_t len, double* M, double* mjd_out,
+    double* tb)
 {
     using namespace frames;
     const int64_t node = blockIdx.x;
@@ -236,15 +283,21 @@ __global__ void __launch_bounds__(64) time_nodes_kernel(FrameDev fd, int y, int 
     for (int i = 0; i < 9; ++i)
         M[node * 9 + i] = B[i];
     mjd_out[node] = mjd;
+    if (tb) {
+        double o[6];
+        sun_moon(t, o);
+        for (int i = 0; i < 6; ++i)
+            tb[node * 6 + i] = o[i];
+    }
 }
 
 void launch_time_nodes(hipStream_t st, const FrameDev& fd, const int* ymdhm, double sec_shift,
-    const double* d_offsets, int64_t len, double* d_M, double* d_mjd)
+    const double* d_offsets, int64_t len, double* d_M, double* d_mjd, double* d_tb)
 {
     if (len <= 0)
         return;
     hipLaunchKernelGGL(time_nodes_kernel, dim3((unsigned)len), dim3(64), 0, st, fd, ymdhm[0],
-        ymdhm[1], ymdhm[2], ymdhm[3], ymdhm[4], sec_shift, d_offsets, len, d_M, d_mjd);
+        ymdhm[1], ymdhm[2], ymdhm[3], ymdhm[4], sec_shift, d_offsets, len, d_M, d_mjd, d_tb);
 }
 
 // =========================================================================================
@@ -252,34 +305,38 @@ void launch_time_nodes(hipStream_t st, const FrameDev& fd, const int* ymdhm, dou
 // =========================================================================================
 template <bool SPLIT>
 __global__ void __launch_bounds__(256) gravity_aos_kernel(GravityDev g, const double* M, const double* r, int64_t n,
-    double* a, int64_t out_stride)
+    double* a, int64_t out_stride, SourcesDev src, const double* tb, const double* v)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const double p[3] = { r[3 * i], r[3 * i + 1], r[3 * i + 2] };
     double o[3];
-    if (M)
+    if (M) {
         gravity_gcrs<SPLIT>(M, p, g, o);
-    else
+        const double w[3] = { v ? v[3 * i] : 0.0, v ? v[3 * i + 1] : 0.0, v ? v[3 * i + 2] : 0.0 };
+        add_sources(src, tb, p, w, o);
+    } else {
         gravity_itrs<SPLIT>(p[0], p[1], p[2], g, o[0], o[1], o[2]);
+    }
     a[i * out_stride] = o[0];
     a[i * out_stride + 1] = o[1];
     a[i * out_stride + 2] = o[2];
 }
 
 void launch_gravity_aos(hipStream_t st, bool split, const GravityDev& g, const double* d_M,
-    const double* d_r, int64_t n, double* d_a, int64_t out_stride, int block)
+    const double* d_r, int64_t n, double* d_a, int64_t out_stride, int block,
+    const SourcesDev& src, const double* d_tb, const double* d_v)
 {
     if (n <= 0)
         return;
     const dim3 grid((unsigned)((n + block - 1) / block));
     if (split)
         hipLaunchKernelGGL(gravity_aos_kernel<true>, grid, dim3(block), 0, st, g, d_M, d_r, n,
-            d_a, out_stride);
+            d_a, out_stride, src, d_tb, d_v);
     else
         hipLaunchKernelGGL(gravity_aos_kernel<false>, grid, dim3(block), 0, st, g, d_M, d_r, n,
-            d_a, out_stride);
+            d_a, out_stride, src, d_tb, d_v);
 }
 
 // =========================================================================================

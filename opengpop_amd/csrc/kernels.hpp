@@ -58,11 +58,18 @@ struct PerturbArgs {
 void launch_perturb(hipStream_t st, const PerturbArgs& a);
 
 // Kernel variants: force kind (ForceKind; kGravitySplit for the split-precision gravity).
+// Per node: the ITRS->GCRS matrix (9, column-major), MJD(UTC) and, when d_tb is not null, the
+// Sun and Moon GCRS positions (6) of the third-body sources.
 void launch_time_nodes(hipStream_t st, const FrameDev& fd, const int* ymdhm, double sec_shift,
-    const double* d_offsets, int64_t len, double* d_M, double* d_mjd);
+    const double* d_offsets, int64_t len, double* d_M, double* d_mjd, double* d_tb = nullptr);
 
+// Gravity of n AoS positions: in the ITRS (d_M null), or in the GCRS at the node of matrix d_M
+// plus the sources of `src` with that node's Sun/Moon positions d_tb and AoS velocities d_v
+// (null = zero).
 void launch_gravity_aos(hipStream_t st, bool split, const GravityDev& g, const double* d_M,
-    const double* d_r, int64_t n, double* d_a, int64_t out_stride, int block);
+    const double* d_r, int64_t n, double* d_a, int64_t out_stride, int block,
+    const SourcesDev& src = SourcesDev {}, const double* d_tb = nullptr,
+    const double* d_v = nullptr);
 
 // Startup and GJ8 kernels (propagate.hip). `waves` = W waves per group of 64 objects
 // (1, 2, 4 or 8; analytic test forces always use 1). Results do not depend on W.

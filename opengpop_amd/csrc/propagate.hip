@@ -147,7 +147,9 @@ template <int W> struct ExchangeArea {
 // acceleration matrix and ncorr corrections; then the hand-over to the GJ8 steps
 // (gpu_propagator.cu:31-53, copy_s.cu:19-30).
 template <int KIND, int W>
-__global__ void __launch_bounds__(64 * W) startup_kernel(ForceCtx f, int rk8_variant, PropState ps,
+__global__ void __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(2))) // at most 256 VGPRs + AGPRs: 2 resident waves per SIMD
+startup_kernel(ForceCtx f, int rk8_variant, PropState ps,
     StartupScratch sc, int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps)
 {
     __shared__ double red[ExchangeArea<W>::size];

@@ -65,11 +65,15 @@ class Context:
         check(self.h, lib().ogp_gravity_itrs(self.h, ptr(r), len(r), ptr(a)))
         return a
 
-    def accel(self, r, step, node, out_stride=3):
+    def accel(self, r, step, node, out_stride=3, v=None):
+        """ForceModel::ComputePropagationAcceleration at propagation node `node`: gravity plus the
+        enabled sources (v, AoS velocities, is needed for drag)."""
         r = f64(r).reshape(-1, 3)
         n = len(r)
+        vv = None if v is None else f64(v).reshape(-1, 3)
         out = np.zeros(max(out_stride * (n - 1) + 3, 3))
-        check(self.h, lib().ogp_accel(self.h, ptr(r), None, n, step, node, ptr(out), out_stride))
+        check(self.h, lib().ogp_accel(self.h, ptr(r), ptr(vv), n, step, node, ptr(out),
+                                      out_stride))
         return out
 
     def time_nodes(self, step, before, offsets):
@@ -138,6 +142,17 @@ class Context:
 
     def set_tuning(self, waves=0, chunk_steps=0):
         check(self.h, lib().ogp_set_tuning(self.h, waves, chunk_steps))
+
+    def set_sources(self, mask=0, drag_bc=0.0):
+        """Acceleration sources added to gravity: SOURCE_SUN | SOURCE_MOON | SOURCE_DRAG bits,
+        drag_bc = C_D A / m (m^2/kg). mask 0 = gravity only."""
+        s = C.OgpSources(mask, drag_bc)
+        check(self.h, lib().ogp_set_sources(self.h, ctypes.byref(s)))
+
+    def get_sources(self):
+        s = C.OgpSources()
+        check(self.h, lib().ogp_get_sources(self.h, ctypes.byref(s)))
+        return s.mask, s.drag_bc
 
     def set_slices(self, slices=0):
         """Object slices per step-size class run on concurrent streams (0 = automatic)."""

@@ -691,11 +691,126 @@ void gravity_gcrs(const Data& d, const double* M, const double* r, double* a, st
 // Force models: the gravity force model (force_model.cu:47-109) and the analytic test
 // forces (unit_tests/include/gpu_test_helpers.cuh, mock_force_model.cuh:69-73, :99)
 // ---------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------
+// Acceleration sources beyond gravity (the extension point of iacceleration_source.cuh:19-33,
+// summed by force_model.cu:73-86): Sun and Moon point masses at the low-precision positions of
+// Montenbruck and Gill (2000) eq. 3.43-3.49 (J2000 ecliptic, rotated to the J2000 equator), and
+// cannonball drag in an exponential atmosphere (Vallado, Table 8-4) rotating with the Earth.
+// Added after gravity in the order Sun, Moon, drag.
+// ---------------------------------------------------------------------------------------
+int g_src_mask = 0;
+double g_src_bc = 0.0;
+constexpr double kGmSun = 1.32712440018e20, kGmMoon = 4.9028e12, kOmegaE = 7.292115e-5;
+constexpr double kAtmR = 6378137.0;
+const double kAtm[28][3] = { { 0, 1.225, 7.249 }, { 25, 3.899e-2, 6.349 },
+    { 30, 1.774e-2, 6.682 }, { 40, 3.972e-3, 7.554 }, { 50, 1.057e-3, 8.382 },
+    { 60, 3.206e-4, 7.714 }, { 70, 8.770e-5, 6.549 }, { 80, 1.905e-5, 5.799 },
+    { 90, 3.396e-6, 5.382 }, { 100, 5.297e-7, 5.877 }, { 110, 9.661e-8, 7.263 },
+    { 120, 2.438e-8, 9.473 }, { 130, 8.484e-9, 12.636 }, { 140, 3.845e-9, 16.149 },
+    { 150, 2.070e-9, 22.523 }, { 180, 5.464e-10, 29.740 }, { 200, 2.789e-10, 37.105 },
+    { 250, 7.248e-11, 45.546 }, { 300, 2.418e-11, 53.628 }, { 350, 9.518e-12, 53.298 },
+    { 400, 3.725e-12, 58.515 }, { 450, 1.585e-12, 60.828 }, { 500, 6.967e-13, 63.822 },
+    { 600, 1.454e-13, 71.835 }, { 700, 3.614e-14, 88.667 }, { 800, 1.170e-14, 124.64 },
+    { 900, 5.245e-15, 181.05 }, { 1000, 3.019e-15, 268.00 } };
+
+void ecliptic_to_equator(double lon, double lat, double dist, double* o)
+{
+    const double eps = 23.43929111 * kPi / 180.0;
+    const double xe = dist * std::cos(lat) * std::cos(lon);
+    const double ye = dist * std::cos(lat) * std::sin(lon);
+    const double ze = dist * std::sin(lat);
+    o[0] = xe;
+    o[1] = std::cos(eps) * ye - std::sin(eps) * ze;
+    o[2] = std::sin(eps) * ye + std::cos(eps) * ze;
+}
+
+// T: Julian centuries of TT since J2000. out: Sun xyz, Moon xyz (m).
+void sun_moon(double T, double* out)
+{
+    const double deg = kPi / 180.0, sec = deg / 3600.0;
+    const double M = deg * (357.5256 + 35999.049 * T);
+    const double lsun = deg * 282.94 + M + sec * (6892.0 * std::sin(M) + 72.0 * std::sin(2.0 * M));
+    const double rsun = 1e9 * (149.619 - 2.499 * std::cos(M) - 0.021 * std::cos(2.0 * M));
+    ecliptic_to_equator(lsun, 0.0, rsun, out);
+    const double L0 = deg * (218.31617 + 481267.88088 * T - 1.3972 * T);
+    const double l = deg * (134.96292 + 477198.86753 * T);
+    const double lp = deg * (357.52543 + 35999.04944 * T);
+    const double F = deg * (93.27283 + 483202.01873 * T);
+    const double D = deg * (297.85027 + 445267.11135 * T);
+    struct Term {
+        double amp, cl, clp, cF, cD;
+    };
+    // longitude perturbations (arcsec) and their arguments l, l', F, D
+    const Term lon[14] = { { 22640, 1, 0, 0, 0 }, { 769, 2, 0, 0, 0 }, { -4586, 1, 0, 0, -2 },
+        { 2370, 0, 0, 0, 2 }, { -668, 0, 1, 0, 0 }, { -412, 0, 0, 2, 0 }, { -212, 2, 0, 0, -2 },
+        { -206, 1, 1, 0, -2 }, { 192, 1, 0, 0, 2 }, { -165, 0, 1, 0, -2 }, { 148, 1, -1, 0, 0 },
+        { -125, 0, 0, 0, 1 }, { -110, 1, 1, 0, 0 }, { -55, 0, 0, 2, -2 } };
+    double dl = 0.0;
+    for (const Term& t : lon)
+        dl += t.amp * std::sin(t.cl * l + t.clp * lp + t.cF * F + t.cD * D);
+    const double lmoon = L0 + sec * dl;
+    const Term lat[7] = { { -526, 0, 0, 1, -2 }, { 44, 1, 0, 1, -2 }, { -31, -1, 0, 1, -2 },
+        { -25, -2, 0, 1, 0 }, { -23, 0, 1, 1, -2 }, { 21, -1, 0, 1, 0 }, { 11, 0, -1, 1, -2 } };
+    double db = 18520.0 * std::sin(F + lmoon - L0 + sec * (412.0 * std::sin(2.0 * F) + 541.0 * std::sin(lp)));
+    for (const Term& t : lat)
+        db += t.amp * std::sin(t.cl * l + t.clp * lp + t.cF * F + t.cD * D);
+    const double bmoon = sec * db;
+    const Term dist[8] = { { -20905, 1, 0, 0, 0 }, { -3699, -1, 0, 0, 2 }, { -2956, 0, 0, 0, 2 },
+        { -570, 2, 0, 0, 0 }, { 246, 2, 0, 0, -2 }, { -205, 0, 1, 0, -2 }, { -171, 1, 0, 0, 2 },
+        { -152, 1, 1, 0, -2 } };
+    double rk = 385000.0;
+    for (const Term& t : dist)
+        rk += t.amp * std::cos(t.cl * l + t.clp * lp + t.cF * F + t.cD * D);
+    ecliptic_to_equator(lmoon, bmoon, 1e3 * rk, out + 3);
+}
+
+double density(double h)
+{
+    const double km = h / 1000.0;
+    int b = 0;
+    for (int q = 1; q < 28; ++q)
+        if (km >= kAtm[q][0])
+            b = q;
+    return kAtm[b][1] * std::exp((kAtm[b][0] - km) / kAtm[b][2]);
+}
+
+void point_mass(double gm, const double* s, const double* r, double* a)
+{
+    double d[3], dn = 0.0, sn = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        d[i] = s[i] - r[i];
+        dn += d[i] * d[i];
+        sn += s[i] * s[i];
+    }
+    const double d3 = dn * std::sqrt(dn), s3 = sn * std::sqrt(sn);
+    for (int i = 0; i < 3; ++i)
+        a[i] += gm * (d[i] / d3 - s[i] / s3);
+}
+
+void add_sources(int mask, double bc, const double* tb, const double* r, const double* v, double* a)
+{
+    if (mask & 1)
+        point_mass(kGmSun, tb, r, a);
+    if (mask & 2)
+        point_mass(kGmMoon, tb + 3, r, a);
+    if (mask & 4) {
+        const double vr[3] = { v[0] + kOmegaE * r[1], v[1] - kOmegaE * r[0], v[2] };
+        const double speed = std::sqrt(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+        const double rn = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const double k = -0.5 * bc * density(rn - kAtmR) * speed;
+        for (int i = 0; i < 3; ++i)
+            a[i] += k * vr[i];
+    }
+}
+
 struct Force {
     int kind = 1;
     double h = 1;
     const Data* d = nullptr;
     std::shared_ptr<std::vector<double>> genM, propM; // 9 per node, shared between objects
+    std::shared_ptr<std::vector<double>> genTB, propTB; // Sun and Moon per node (6)
+    int src_mask = 0;
+    double src_bc = 0.0;
     std::vector<double> P;                            // per-thread scratch
 
     static void analytic(int kind, const double* p, const double* v, double t, double* a)
@@ -746,6 +861,8 @@ struct Force {
             int s = stepNum > 0 ? stepNum - 1 : stepNum;
             int idx = (s + 4) * 10 + k;
             gravity_gcrs(*d, &(*genM)[idx * 9], x, a, P);
+            if (src_mask)
+                add_sources(src_mask, src_bc, &(*genTB)[idx * 6], x, v, a);
             return;
         }
         int dir = stepNum > 0 ? -1 : 1;
@@ -757,6 +874,8 @@ struct Force {
     {
         if (kind == 0) {
             gravity_gcrs(*d, &(*propM)[(size_t)node * 9], x, a, P);
+            if (src_mask)
+                add_sources(src_mask, src_bc, &(*propTB)[(size_t)node * 6], x, v, a);
             return;
         }
         double t = (double)(node - 4) * h;
@@ -794,13 +913,26 @@ void build_force(Force& f, int kind, int quirks, const int* ymdhm, double sec, d
     double shifted = sec - (double)4 * h; // Sec - beforeEpoch * stepSize
     std::vector<double> go = gen_offsets(h, quirks);
     f.genM = std::make_shared<std::vector<double>>(80 * 9);
+    f.genTB = std::make_shared<std::vector<double>>(80 * 6);
+    f.src_mask = g_src_mask;
+    f.src_bc = g_src_bc;
+    // Sun and Moon at the node's TT (the time-node kernel's T)
+    auto tb = [&](double mjd_utc, double* out) {
+        const double mjd_tt = mjd_utc + (leap_get(f.d->ls_mjd, f.d->ls_val, mjd_utc) + kTtTai) / 86400.0;
+        sun_moon((mjd_tt - kMjdJ2000) / 36525.0, out);
+    };
     double mjd;
-    for (int i = 0; i < 80; ++i)
+    for (int i = 0; i < 80; ++i) {
         time_node(*f.d, ymdhm, shifted, go[i], &(*f.genM)[i * 9], &mjd);
+        tb(mjd, &(*f.genTB)[i * 6]);
+    }
     int len = 4 + maxSteps + 1;
     f.propM = std::make_shared<std::vector<double>>((size_t)len * 9);
-    for (int i = 0; i < len; ++i)
+    f.propTB = std::make_shared<std::vector<double>>((size_t)len * 6);
+    for (int i = 0; i < len; ++i) {
         time_node(*f.d, ymdhm, shifted, i * h, &(*f.propM)[(size_t)i * 9], &mjd);
+        tb(mjd, &(*f.propTB)[(size_t)i * 6]);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1610,6 +1742,23 @@ void orc_perturbed(const double* r0, const double* v0, long n, double strength, 
             v[3 * i + c] = v0[c] + z[c] * strength;
         }
     }
+}
+
+
+void orc_set_sources(int mask, double drag_bc)
+{
+    g_src_mask = mask;
+    g_src_bc = drag_bc;
+}
+
+void orc_sun_moon(double mjd_tt, double* out6) { sun_moon((mjd_tt - kMjdJ2000) / 36525.0, out6); }
+
+double orc_density(double h_m) { return density(h_m); }
+
+void orc_sources_accel(int mask, double drag_bc, const double* tb6, const double* r, const double* v,
+    double* a)
+{
+    add_sources(mask, drag_bc, tb6, r, v, a);
 }
 
 } // extern "C"

@@ -59,6 +59,18 @@ void orc_philox4x32_10(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out
 void orc_perturbed(const double* r0, const double* v0, long n, double strength, uint64_t seed,
     double* r, double* v);
 
+// ---- acceleration sources beyond gravity (iacceleration_source.cuh:19-33) ------------------
+// Sun and Moon point masses and cannonball drag, added after gravity in every force evaluation
+// of the propagations that follow orc_set_sources (mask bits 1 Sun, 2 Moon, 4 drag; C_D A / m
+// in m^2/kg). orc_sun_moon: Sun and Moon GCRS positions (m) at an MJD of TT; orc_density: the
+// exponential atmosphere (kg/m^3) at an altitude (m); orc_sources_accel adds the sources'
+// accelerations at one point to a[3] (tb6 = that time's Sun and Moon positions).
+void orc_set_sources(int mask, double drag_bc);
+void orc_sun_moon(double mjd_tt, double* out6);
+double orc_density(double h_m);
+void orc_sources_accel(int mask, double drag_bc, const double* tb6, const double* r, const double* v,
+    double* a);
+
 // ---- gravity (gravity_source*.cu) ------------------------------------------------------
 int orc_gravity_itrs(const double* r, long n, double* a);
 // Non-central potential U - GM/r (degrees 2..L) at an ITRS point by an independent textbook recursion

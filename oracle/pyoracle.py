@@ -60,6 +60,11 @@ class Oracle:
         L.orc_gravity_itrs.argtypes = [_P, ctypes.c_long, _P]
         L.orc_potential_itrs.restype = _D
         L.orc_philox4x32_10.argtypes = [_P, _P, _P]
+        L.orc_set_sources.argtypes = [_I, _D]
+        L.orc_sun_moon.argtypes = [_D, _P]
+        L.orc_density.restype = _D
+        L.orc_density.argtypes = [_D]
+        L.orc_sources_accel.argtypes = [_I, _D, _P, _P, _P, _P]
         L.orc_perturbed.argtypes = [_P, _P, ctypes.c_long, _D, ctypes.c_uint64, _P, _P]
         L.orc_potential_itrs.argtypes = [_P]
         L.orc_gj_predict.argtypes = [_P, _P, _P, _D, _P, _P]
@@ -176,6 +181,26 @@ class Oracle:
         return self.L.orc_potential_itrs(_ptr(r))
 
     # -- integrator pieces -----------------------------------------------------------------
+    def set_sources(self, mask=0, drag_bc=0.0):
+        """Sources added to gravity in the propagations that follow (bits 1 Sun, 2 Moon, 4 drag)."""
+        self.L.orc_set_sources(mask, drag_bc)
+
+    def sun_moon(self, mjd_tt):
+        out = np.zeros(6)
+        self.L.orc_sun_moon(mjd_tt, _ptr(out))
+        return out[:3], out[3:]
+
+    def density(self, h_m):
+        return self.L.orc_density(h_m)
+
+    def sources_accel(self, mask, drag_bc, tb6, r, v):
+        tb6 = np.ascontiguousarray(tb6, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        a = np.zeros(3)
+        self.L.orc_sources_accel(mask, drag_bc, _ptr(tb6), _ptr(r), _ptr(v), _ptr(a))
+        return a
+
     def philox4x32_10(self, ctr, key):
         """One Philox4x32-10 block: four uint32 counter words and two key words."""
         c = np.ascontiguousarray(ctr, dtype=np.uint32)
