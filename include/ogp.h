@@ -118,6 +118,14 @@ typedef struct ogp_sources {
 int ogp_set_sources(ogp_context* ctx, const ogp_sources* sources);
 int ogp_get_sources(const ogp_context* ctx, ogp_sources* sources);
 
+/* Per-object ballistic coefficients for OGP_SOURCE_DRAG (no counterpart in the reference):
+ * bc[i] = C_D A / m (m^2/kg) of object i in the caller's order. While drag is enabled they replace
+ * ogp_sources.drag_bc in every later call of the context on n objects (propagation, stepwise
+ * propagator and closest-approach search from their init calls, ogp_startup, ogp_accel); a call
+ * on another number of objects is then an error. bc = NULL or n = 0 returns to the single
+ * coefficient. */
+int ogp_set_drag_coefficients(ogp_context* ctx, const double* bc, int64_t n);
+
 /* FMTimePoints::CalculateParameters + GetParametersDev (time_points.cu:126-207): the
  * ITRS->GCRS matrix (column-major, 9 per node) and MJD(UTC) of `len` nodes at offsets
  * offsets_s (seconds) from (epoch - before * step_s). */

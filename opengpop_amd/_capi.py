@@ -91,6 +91,7 @@ def lib():
         "ogp_set_slices": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_set_sources": (ctypes.c_int, [_P, ctypes.POINTER(OgpSources)]),
         "ogp_get_sources": (ctypes.c_int, [_P, ctypes.POINTER(OgpSources)]),
+        "ogp_set_drag_coefficients": (ctypes.c_int, [_P, _P, _I64]),
         "ogp_get_tuning": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(_I64)]),
         "ogp_prop_init": (ctypes.c_int, [_P, _P, _P, _I64, _D, _I64]),

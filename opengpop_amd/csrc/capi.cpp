@@ -204,16 +204,44 @@ struct ogp_context {
             && tuned_waves.find(group_bucket(n)) == tuned_waves.end();
     }
 
-    // Acceleration sources added to gravity (ogp_set_sources).
+    // Acceleration sources added to gravity (ogp_set_sources) and the per-object drag
+    // coefficients (ogp_set_drag_coefficients, caller order). bc_obj: device copy in state order
+    // for the launches of the current call (null = drag_bc for every object); the bulk, stepwise
+    // and conjunction states keep their own copies.
     ogp_sources sources {};
+    std::vector<double> drag_bc_obj;
+    DevBuf d_bc, sw_bc, cj_bc;
+    const double* bc_obj = nullptr;
+    const double* sw_bc_ptr = nullptr;
+    const double* cj_bc_ptr = nullptr;
     SourcesDev sources_dev() const
     {
         SourcesDev d {};
         if (kind == kGravity || kind == kGravitySplit) {
             d.mask = sources.mask;
             d.drag_bc = sources.drag_bc;
+            d.bc_obj = (sources.mask & kSourceDrag) ? bc_obj : nullptr;
         }
         return d;
+    }
+    // Device copy of the per-object drag coefficients for a call on n objects, in state order
+    // (perm[p] = caller index at state position p; null = the caller's order), or null when none
+    // are set or drag is off. A different object count is an error while drag is on.
+    const double* stage_bc(DevBuf& buf, int64_t n, const int64_t* perm)
+    {
+        if (drag_bc_obj.empty() || !(sources.mask & kSourceDrag) || n <= 0
+            || (kind != kGravity && kind != kGravitySplit))
+            return nullptr;
+        if ((int64_t)drag_bc_obj.size() != n)
+            throw ApiError(OGP_ERR_ARGUMENT, "per-object drag coefficients were set for "
+                    + std::to_string(drag_bc_obj.size()) + " objects; this call has "
+                    + std::to_string(n) + " (ogp_set_drag_coefficients)");
+        std::vector<double> h(n);
+        for (int64_t p = 0; p < n; ++p)
+            h[p] = drag_bc_obj[perm ? perm[p] : p];
+        buf.reserve(sizeof(double) * n, &dev_bytes);
+        HIPC(hipMemcpy(buf.p, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        return buf.as<double>();
     }
     // The derivative paths linearise gravity alone.
     void require_gravity_only(const char* what) const
@@ -702,10 +730,11 @@ int ogp_accel(ogp_context* ctx, const double* r, const double* v, int64_t n, dou
         arg_check(n >= 0 && step_s > 0 && node >= 0 && out_stride >= 3, "invalid accel arguments");
         if (ctx->cfg.force != OGP_FORCE_GRAVITY)
             throw ApiError(OGP_ERR_STATE, "context was created without the gravity field");
-        const SourcesDev src = ctx->sources_dev();
-        arg_check(v || !(src.mask & kSourceDrag), "drag needs the velocities");
+        arg_check(v || !(ctx->sources.mask & kSourceDrag), "drag needs the velocities");
         if (n == 0)
             return;
+        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, nullptr);
+        const SourcesDev src = ctx->sources_dev();
         check_finite(r, 3 * n, "positions");
         if (v)
             check_finite(v, 3 * n, "velocities");
@@ -821,6 +850,7 @@ int ogp_propagate(ogp_context* ctx, const double* r0, const double* v0, int64_t 
         std::vector<int64_t> ns(n);
         for (int64_t p = 0; p < n; ++p)
             ns[p] = steps[perm[p]];
+        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, perm.data());
         ctx->reserve_state(n);
         ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
         ctx->io_b.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
@@ -883,6 +913,7 @@ int ogp_propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v
         std::vector<int64_t> ns(n);
         for (int64_t p = 0; p < n; ++p)
             ns[p] = steps[perm[p]];
+        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, perm.data());
         ctx->reserve_state(n);
         HIPC(hipMemcpyAsync(ctx->perm.p, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
             ctx->stream));
@@ -1022,6 +1053,23 @@ int ogp_set_sources(ogp_context* ctx, const ogp_sources* sources)
     });
 }
 
+int ogp_set_drag_coefficients(ogp_context* ctx, const double* bc, int64_t n)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n >= 0, "negative object count");
+        if (!bc || n == 0) {
+            ctx->drag_bc_obj.clear();
+            return;
+        }
+        for (int64_t i = 0; i < n; ++i)
+            arg_check(std::isfinite(bc[i]) && bc[i] >= 0.0,
+                "drag coefficients must be finite and >= 0 (object " + std::to_string(i) + ")");
+        ctx->drag_bc_obj.assign(bc, bc + n);
+    });
+}
+
 int ogp_get_sources(const ogp_context* ctx, ogp_sources* sources)
 {
     if (!ctx || !sources)
@@ -1076,6 +1124,7 @@ int ogp_prop_init(ogp_context* ctx, const double* r0, const double* v0, int64_t 
         ctx->sw_traj.reserve(sizeof(double) * 9 * 4 * n, &ctx->dev_bytes);
         ctx->check_launch();
         ctx->sync();
+        ctx->sw_bc_ptr = ctx->stage_bc(ctx->sw_bc, n, nullptr);
         ctx->sw_n = n;
         ctx->sw_step = 0;
         ctx->sw_max = max_steps;
@@ -1096,6 +1145,7 @@ int ogp_prop_step(ogp_context* ctx, double* r_out, double* v_out, double* a_out)
             throw ApiError(OGP_ERR_STATE, "step beyond max_steps of ogp_prop_init");
         NodeTable* nt = ctx->node_table(ctx->sw_h, ctx->sw_max);
         const int W = ctx->waves_for(n);
+        ctx->bc_obj = ctx->sw_bc_ptr;
         const ForceCtx f = ctx->force_ctx(nt, ctx->sw_h);
         const PropState ps = ctx->prop_state(n);
         double* traj = ctx->sw_traj.as<double>();
@@ -1152,6 +1202,7 @@ int ogp_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
             ctx->stream));
         NodeTable* nt = ctx->node_table(step_s, 4);
         const int W = ctx->waves_for(n);
+        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, nullptr);
         launch_startup(ctx->stream, ctx->kind, ctx->force_ctx(nt, step_s), ctx->rk8_variant,
             ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0, W);
         ctx->check_launch();
@@ -1267,6 +1318,7 @@ int ogp_conj_init(ogp_context* ctx, const double* r, const double* v, const uint
         launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->cj_v.as<double>(), n);
         ctx->check_launch();
         ctx->sync();
+        ctx->cj_bc_ptr = ctx->stage_bc(ctx->cj_bc, n, nullptr);
         ctx->cj_n = n;
         ctx->cj_h = step_s;
         ctx->cj_steps = 0;
@@ -1314,6 +1366,7 @@ int ogp_conj_propagate(ogp_context* ctx, int64_t n_steps)
         const double t0 = (double)ctx->cj_steps * ctx->cj_h;
         std::vector<ogp_context::Group> g { { ctx->cj_h, 0, n, n_steps } };
         double* traj = ctx->cj_traj.as<double>() + 9 * n * ctx->cj_steps;
+        ctx->bc_obj = ctx->cj_bc_ptr;
         ctx->run_groups(g, n, t0, traj, n_steps);
         HIPC(hipMemcpyAsync(ctx->cj_r.p, ctx->st_r.p, sizeof(double) * 3 * n,
             hipMemcpyDeviceToDevice, ctx->stream));

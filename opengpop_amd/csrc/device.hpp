@@ -470,9 +470,11 @@ static __constant__ const double kAtmosphere[kAtmBands][3] = {
 };
 
 struct SourcesDev {
-    int mask;       // SourceBit set; 0 = gravity only
-    double drag_bc; // C_D A / m, m^2 / kg
+    int mask;              // SourceBit set; 0 = gravity only
+    double drag_bc;        // C_D A / m, m^2 / kg (per object inside the integrator kernels)
+    const double* bc_obj;  // per-object C_D A / m in the state's order, or null = drag_bc for all
 };
+
 
 // Point mass at s (GCRS, m): a += GM ((s - r) / |s - r|^3 - s / |s|^3).
 __device__ __forceinline__ void third_body(double gm, const double* s, const double* r, double* a)
@@ -540,6 +542,16 @@ struct ForceCtx {
     SourcesDev src;
     double h;
 };
+
+// The force context of object i (index into the state): drag with that object's coefficient
+// when per-object coefficients are given.
+__device__ __forceinline__ ForceCtx object_ctx(const ForceCtx& f, int64_t i)
+{
+    ForceCtx o = f;
+    if (f.src.bc_obj)
+        o.src.drag_bc = f.src.bc_obj[i];
+    return o;
+}
 
 // Per-workgroup exchange area of the column-split evaluation (unused when W == 1).
 struct Exchange {

@@ -315,7 +315,10 @@ __global__ void __launch_bounds__(256) gravity_aos_kernel(GravityDev g, const do
     if (M) {
         gravity_gcrs<SPLIT>(M, p, g, o);
         const double w[3] = { v ? v[3 * i] : 0.0, v ? v[3 * i + 1] : 0.0, v ? v[3 * i + 2] : 0.0 };
-        add_sources(src, tb, p, w, o);
+        SourcesDev so = src;
+        if (src.bc_obj)
+            so.drag_bc = src.bc_obj[i];
+        add_sources(so, tb, p, w, o);
     } else {
         gravity_itrs<SPLIT>(p[0], p[1], p[2], g, o[0], o[1], o[2]);
     }

@@ -149,7 +149,7 @@ template <int W> struct ExchangeArea {
 template <int KIND, int W>
 __global__ void __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(2))) // at most 256 VGPRs + AGPRs: 2 resident waves per SIMD
-startup_kernel(ForceCtx f, int rk8_variant, PropState ps,
+startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     StartupScratch sc, int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps)
 {
     __shared__ double red[ExchangeArea<W>::size];
@@ -158,6 +158,7 @@ startup_kernel(ForceCtx f, int rk8_variant, PropState ps,
     if (t >= nobj)
         return;
     const int64_t i = obj0 + t;
+    const ForceCtx f = object_ctx(f0, i);
     const int64_t N = ps.N;
     const int64_t ns = ps.nsteps[i];
     if (ns <= 0)
@@ -213,7 +214,7 @@ startup_kernel(ForceCtx f, int rk8_variant, PropState ps,
 template <int KIND, int W>
 __global__ void __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(OGP_CHUNK_WAVES_PER_EU)))
-gj8_chunk_kernel(ForceCtx f, PropState ps, int64_t obj0,
+gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj, int64_t traj_first,
     int64_t traj_steps)
 {
@@ -223,6 +224,7 @@ gj8_chunk_kernel(ForceCtx f, PropState ps, int64_t obj0,
     if (t >= nobj)
         return;
     const int64_t i = obj0 + t;
+    const ForceCtx f = object_ctx(f0, i);
     const int64_t N = ps.N;
     const int64_t ns = ps.nsteps[i];
     if (ns < k0)

@@ -149,6 +149,15 @@ class Context:
         s = C.OgpSources(mask, drag_bc)
         check(self.h, lib().ogp_set_sources(self.h, ctypes.byref(s)))
 
+    def set_drag_coefficients(self, bc=None):
+        """Per-object C_D A / m (m^2/kg) for drag, in object order; None returns to the single
+        coefficient of set_sources."""
+        if bc is None:
+            check(self.h, lib().ogp_set_drag_coefficients(self.h, None, 0))
+            return
+        b = f64(bc).reshape(-1)
+        check(self.h, lib().ogp_set_drag_coefficients(self.h, ptr(b), len(b)))
+
     def get_sources(self):
         s = C.OgpSources()
         check(self.h, lib().ogp_get_sources(self.h, ctypes.byref(s)))

@@ -96,6 +96,35 @@ def test_drag_lowers_the_orbit(ctx20):
     assert np.linalg.norm(rd - rg) > 100.0
 
 
+def test_per_object_drag_coefficients(ctx20):
+    # a batch with per-object C_D A / m equals, object by object, the same population propagated
+    # with each coefficient as the single value (an object's arithmetic does not depend on the
+    # others, whatever the launch shape); also through ogp_accel
+    r0, v0, step = _population(seed=12, n=48)
+    bc = np.array([0.004, 0.02, 0.0])[np.arange(48) % 3]
+    try:
+        ctx20.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_DRAG, 0.5)
+        ctx20.set_drag_coefficients(bc)
+        r, v = ctx20.propagate(r0, v0, step, 90)
+        a = ctx20.accel(r0, 60.0, 6, v=v0).reshape(-1, 3)
+        ctx20.set_drag_coefficients(None)
+        for value in (0.004, 0.02, 0.0):
+            ctx20.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_DRAG, value)
+            rs, vs = ctx20.propagate(r0, v0, step, 90)
+            as_ = ctx20.accel(r0, 60.0, 6, v=v0).reshape(-1, 3)
+            sel = bc == value
+            assert np.array_equal(r[sel], rs[sel]) and np.array_equal(v[sel], vs[sel]), value
+            assert np.array_equal(a[sel], as_[sel]), value
+        ctx20.set_drag_coefficients(bc)
+        with pytest.raises(ogp.OgpError):
+            ctx20.propagate(r0[:10], v0[:10], step[:10], 20)  # another object count
+        with pytest.raises(ogp.OgpError):
+            ctx20.set_drag_coefficients(-bc)
+    finally:
+        ctx20.set_drag_coefficients(None)
+        ctx20.set_sources(0, 0.0)
+
+
 def test_split_precision_with_sources(po):
     r0, v0, step = _population(seed=10, n=16)
     out = {}
