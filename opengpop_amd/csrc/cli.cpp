@@ -7,7 +7,8 @@
 //
 // Extensions (all optional; defaults reproduce the reference): --step, --degree,
 // --corrections, --precision fp64|split, --data-dir, --threshold (closest-approach: report only
-// pairs whose sampled minimum distance is below the threshold, in metres).
+// pairs whose sampled minimum distance is below the threshold, in metres), and the acceleration
+// sources added to gravity: --sun, --moon, --drag C_D*A/m (m^2/kg).
 #include "ogp.h"
 
 #include <array>
@@ -342,6 +343,8 @@ struct Options {
     double step = 30.0; // IdealStepSize, runner.cuh:6
     int degree = -1, corrections = 7;
     double threshold = 0.0;
+    int sources = 0;      // OGP_SOURCE_* bits
+    double drag_bc = 0.0; // C_D A / m with --drag
 };
 
 [[noreturn]] void usage(const char* msg)
@@ -350,7 +353,7 @@ struct Options {
         std::cerr << "error: " << msg << "\n";
     std::cerr << "usage: opengpop {propagate|closest-approach} -i INPUT [-o OUTPUT]\n"
                  "       [--step S] [--degree L] [--corrections K] [--precision fp64|split]\n"
-                 "       [--data-dir DIR] [--threshold METRES]\n";
+                 "       [--data-dir DIR] [--threshold METRES] [--sun] [--moon] [--drag CDA_M]\n";
     std::exit(2);
 }
 
@@ -431,6 +434,16 @@ ogp_context* make_context(const Options& opt, const Common& c)
     ogp_context* ctx = nullptr;
     if (ogp_create(&cfg, &ctx) != OGP_OK)
         throw std::runtime_error(ogp_last_error(nullptr));
+    if (opt.sources) {
+        ogp_sources s {};
+        s.mask = opt.sources;
+        s.drag_bc = opt.drag_bc;
+        if (ogp_set_sources(ctx, &s) != OGP_OK) {
+            const std::string msg = ogp_last_error(ctx);
+            ogp_destroy(ctx);
+            throw std::runtime_error(msg);
+        }
+    }
     return ctx;
 }
 
@@ -550,7 +563,14 @@ int main(int argc, char** argv)
             opt.data_dir = next();
         else if (a == "--threshold")
             opt.threshold = std::atof(next().c_str());
-        else
+        else if (a == "--sun")
+            opt.sources |= OGP_SOURCE_SUN;
+        else if (a == "--moon")
+            opt.sources |= OGP_SOURCE_MOON;
+        else if (a == "--drag") {
+            opt.sources |= OGP_SOURCE_DRAG;
+            opt.drag_bc = std::atof(next().c_str());
+        } else
             usage(("unknown option " + a).c_str());
     }
     if (opt.input.empty())
@@ -559,6 +579,8 @@ int main(int argc, char** argv)
         usage("--step must be positive");
     if (opt.precision != "fp64" && opt.precision != "split")
         usage("--precision must be fp64 or split");
+    if ((opt.sources & OGP_SOURCE_DRAG) && !(opt.drag_bc >= 0))
+        usage("--drag needs C_D A / m >= 0 (m^2/kg)");
     {
         std::ifstream probe(opt.input);
         if (!probe.is_open())

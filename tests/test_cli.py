@@ -71,6 +71,8 @@ def test_argument_errors_exit_with_usage():
     assert r.returncode == 2 and "subcommand" in r.stderr
     r = _run(["propagate", "-i", "x", "--precision", "fp16"])
     assert r.returncode == 2
+    r = _run(["propagate", "-i", os.path.join(DATA, "sat_data.json"), "--drag", "-1"])
+    assert r.returncode == 2 and "--drag" in r.stderr
 
 
 def _check_dump4_layout(text):
@@ -139,3 +141,26 @@ def test_closest_approach_records_every_cross_set_pair(tmp_path, po):
         ref = o.refine(traj, 30.0, a, b, int(mink[idx]), float(mind[idx]))
         assert abs(rec["distance"] - ref["distance"]) < 1.0
         assert abs(rec["time"] - ref["time"]) < 1e-3
+
+
+@pytest.mark.gpu
+def test_propagate_with_sources_matches_oracle(tmp_path, po):
+    # --sun --moon --drag: the acceleration sources added to gravity, against the oracle's
+    out = tmp_path / "out.json"
+    r = _run(["propagate", "-i", os.path.join(DATA, "sat_data.json"), "-o", str(out), "--sun",
+              "--moon", "--drag", "0.01"])
+    assert r.returncode == 0, r.stderr
+    obj = json.loads(out.read_text())
+    inp = json.load(open(os.path.join(DATA, "sat_data.json")))
+    n = 46800 // 30
+    st = inp["propagationStart"]
+    epoch = (st["Year"], st["Month"], st["Day"], st["Hour"], st["Min"], st["Sec"])
+    o = po.Oracle()
+    try:
+        o.set_sources(7, 0.01)
+        ro, vo = o.propagate(po.GRAVITY, inp["positions"], inp["velocities"], 46800 / n, n,
+                             epoch=epoch)
+    finally:
+        o.set_sources(0, 0.0)
+    assert np.abs(np.array(obj["positionsEnd"]) - ro).max() < 1e-3
+    assert np.abs(np.array(obj["velocitiesEnd"]) - vo).max() < 1e-6
