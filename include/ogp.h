@@ -257,9 +257,12 @@ int ogp_gravity_jacobian_itrs(ogp_context* ctx, const double* r, int64_t n, doub
 int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
     const int64_t* steps, double* r_out, double* v_out, double* stm);
 
-/* d(r, v)_final / d(r, v)_initial by the discrete adjoint of the GJ8 steps (reverse mode, the
- * state checkpointed every checkpoint_steps steps, 0 = 64) composed with the forward-mode
- * derivative of the startup. Also returns the final state. */
+/* d(r, v)_final / d(r, v)_initial by the discrete adjoint of the GJ8 steps (reverse mode)
+ * composed with the forward-mode derivative of the startup. Also returns the final state.
+ * checkpoint_steps = 0: the forward pass records the gravity Jacobians of the whole span when
+ * they fit in 16 GiB (6 doubles per object, step and corrector iterate), else checkpoints every
+ * 64 steps; checkpoint_steps > 0: checkpoints every that many steps, each segment recorded again
+ * in the reverse sweep. Results do not depend on the choice. */
 int ogp_adjoint_jacobian(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
     double step_s, const int64_t* steps, int64_t checkpoint_steps, double* r_out, double* v_out,
     double* jac);

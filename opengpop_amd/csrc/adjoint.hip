@@ -6,11 +6,13 @@
 //    kernel runs on the variational-equation layout (kGravityTangent, device.hpp), giving the
 //    derivatives of the post-startup state Z = (s, S, a_{-4..4}) w.r.t. the six initial
 //    components;
-//  * the GJ8 steps are differentiated backward: a forward pass stores the state (s, S and the
-//    acceleration ring, 33 doubles per object) every C steps; per segment, last to first, a
-//    recording pass re-runs the steps from the checkpoint and stores the gravity Jacobian at
-//    every corrector iterate, and the reverse pass carries the cotangents of (s, S, ring) back
-//    through the segment. Cotangents of Z times dZ/dX give the Jacobian rows.
+//  * the GJ8 steps are differentiated backward. When the gravity Jacobians of every corrector
+//    iterate of the span fit in HBM, the forward pass itself records them (gj8_record_kernel,
+//    bit-identical trajectory); otherwise the forward pass stores the state (s, S and the
+//    acceleration ring, 33 doubles per object) every C steps and, per segment, last to first, a
+//    recording pass re-runs the steps from the checkpoint. The reverse pass carries the
+//    cotangents of (s, S, ring) back through each segment. Cotangents of Z times dZ/dX give the
+//    Jacobian rows.
 //
 // Reverse step (transpose of gj8_sums / gj8_predict / gj8_correct, gj8.hpp): with e_it the
 // acceleration at iterate r_it (it = 0 .. K-1), r_{it+1} = hh((T2 + gj88 e_it) + S'),
@@ -155,21 +157,31 @@ __global__ void checkpoint_kernel(PropState ps, double* __restrict__ ck, int to_
 // GJ8 steps k0 .. k1-1 exactly as gj8_chunk_kernel<kGravity> (same arithmetic, gj8.hpp, and a
 // bit-identical acceleration), storing the GCRS gravity Jacobian of every corrector iterate:
 // Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + o], e = xx, xy, xz, yy, yz, zz.
-__global__ void __launch_bounds__(64) gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
+template <int W>
+__global__ void __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(2))) // at most 256 VGPRs + AGPRs: 2 resident waves per SIMD
+gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
     int64_t k1, int ncorr, double* __restrict__ Gbuf)
 {
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    __shared__ double red[W > 1 ? kJacExchange : 1];
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
     const int64_t n = ps.N;
     if (i >= n)
         return;
     const int64_t ns = ps.nsteps[i];
     if (ns < k0)
         return;
+    // the W waves of a workgroup hold the same 64 objects and share the harmonic columns; each
+    // keeps the whole integrator state and writes its own ring slots (which it reads back), and
+    // wave 0 writes the Jacobians and the state at the end
+    const bool writer = threadIdx.x < 64;
     const double h = f.h, hh = h * h;
     double r[3], v[3], s[3], S[3];
     for (int c = 0; c < 3; ++c) {
         s[c] = ps.s[c * n + i];
         S[c] = ps.S[c * n + i];
+        r[c] = ps.r[c * n + i];
+        v[c] = ps.v[c * n + i];
     }
     for (int64_t k = k0; k < k1; ++k) {
         if (k > ns)
@@ -180,16 +192,25 @@ __global__ void __launch_bounds__(64) gj8_record_kernel(ForceCtx f, PropState ps
         gj8_predict(q, h, s, S, r, v, sprev);
         for (int it = 0; it < ncorr; ++it) {
             double G[3][3];
-            gravity_gcrs_jac(f.prop_M + 9 * (k + 4), r, f.grav, anew, G);
+            gravity_gcrs_jac<W>(f.prop_M + 9 * (k + 4), r, f.grav, anew, G, red);
             const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
-            for (int e = 0; e < 6; ++e)
-                Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
+            if (writer)
+                for (int e = 0; e < 6; ++e)
+                    Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
             gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         }
         const int wslot = ring_slot(k);
         for (int c = 0; c < 3; ++c)
             ps.hist[(wslot * 3 + c) * n + i] = anew[c];
     }
+    // the state after the segment: the record pass can then serve as the forward pass
+    if (writer)
+        for (int c = 0; c < 3; ++c) {
+            ps.s[c * n + i] = s[c];
+            ps.S[c * n + i] = S[c];
+            ps.r[c * n + i] = r[c];
+            ps.v[c * n + i] = v[c];
+        }
 }
 
 // Reverse pass over steps k1-1 .. k0 for R cotangent rows. lam[(rho * 33 + e) * n + o] holds the
@@ -342,11 +363,20 @@ void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_
 }
 
 void launch_gj8_record(hipStream_t st, const ForceCtx& f, const PropState& ps, int64_t k0,
-    int64_t k1, int ncorr, double* Gbuf)
+    int64_t k1, int ncorr, double* Gbuf, int waves)
 {
-    if (ps.N > 0 && k1 > k0)
-        hipLaunchKernelGGL(gj8_record_kernel, dim3((unsigned)((ps.N + 63) / 64)), dim3(64), 0, st, f,
-            ps, k0, k1, ncorr, Gbuf);
+    if (ps.N <= 0 || k1 <= k0)
+        return;
+    const dim3 grid((unsigned)((ps.N + 63) / 64));
+#define L_(Wv)                                                                                     \
+    hipLaunchKernelGGL(gj8_record_kernel<Wv>, grid, dim3(64 * (Wv)), 0, st, f, ps, k0, k1, ncorr, \
+        Gbuf)
+    // 4 or 8 waves per group: the 64 KB exchange area leaves a 2-wave shape one wave per SIMD
+    if (waves >= 8)
+        L_(8);
+    else
+        L_(4);
+#undef L_
 }
 
 void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,

@@ -1654,8 +1654,14 @@ PropState tangent_startup(ogp_context* ctx, const double* r0, const double* v0, 
     return ps7;
 }
 
+// Recorded gravity Jacobians of a whole span up to this size are kept in HBM and the record pass
+// is the forward pass; beyond it (or with an explicit checkpoint interval) the forward pass keeps
+// checkpoints and every segment is recorded again in the reverse sweep.
+constexpr double kRecordAllBytes = 16.0 * (1 << 30);
+
 // Reverse mode for R cotangent rows seeded by seeds_host[(rho * 6 + e) * n + o]; writes the
-// final state (AoS) and out_host[(o * R + rho) * 6 + col].
+// final state (AoS) and out_host[(o * R + rho) * 6 + col]. C = checkpoint interval in steps,
+// 0 = automatic.
 void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double h,
     const int64_t* steps, int64_t C, int R, const std::vector<double>& seeds_host,
     double* r_out, double* v_out, double* out_host)
@@ -1687,15 +1693,30 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
     launch_tangent_split(ctx->stream, ps7, n, pc, ctx->ad_ns.as<int64_t>(),
         ctx->ad_Zd.as<double>(), ctx->ad_RVd.as<double>());
     ctx->check_launch();
-    // forward pass over the GJ8 steps, (s, S, ring) stored at the start of every segment
+    const int64_t gj_steps = std::max<int64_t>(max_steps - 4, 0);
+    const bool record_all = C == 0
+        && (double)gj_steps * ncorr * 6 * (double)n * sizeof(double) <= kRecordAllBytes;
+    if (record_all)
+        C = std::max<int64_t>(gj_steps, 1);
+    else if (C == 0)
+        C = 64;
     const int64_t nseg = max_steps >= 5 ? (max_steps - 4 + C - 1) / C : 0;
-    ctx->ad_ck.reserve(sizeof(double) * 33 * n * std::max<int64_t>(nseg, 1), &ctx->dev_bytes);
     const int W = ctx->waves_for(n);
-    for (int64_t q = 0; q < nseg; ++q) {
-        const int64_t k0 = 5 + q * C, k1 = std::min<int64_t>(k0 + C, max_steps + 1);
-        launch_checkpoint(ctx->stream, pc, ctx->ad_ck.as<double>() + q * 33 * n, true);
-        launch_chunk(ctx->stream, kGravity, f, pc, 0, n, k0, k1, ncorr, nullptr, 1, 0, W);
+    ctx->ad_G.reserve(sizeof(double) * C * ncorr * 6 * n, &ctx->dev_bytes);
+    if (record_all) {
+        // forward pass = record pass over the whole span
+        if (nseg > 0)
+            launch_gj8_record(ctx->stream, f, pc, 5, max_steps + 1, ncorr, ctx->ad_G.as<double>(), W);
         ctx->check_launch();
+    } else {
+        // forward pass over the GJ8 steps, (s, S, ring) stored at the start of every segment
+        ctx->ad_ck.reserve(sizeof(double) * 33 * n * std::max<int64_t>(nseg, 1), &ctx->dev_bytes);
+        for (int64_t q = 0; q < nseg; ++q) {
+            const int64_t k0 = 5 + q * C, k1 = std::min<int64_t>(k0 + C, max_steps + 1);
+            launch_checkpoint(ctx->stream, pc, ctx->ad_ck.as<double>() + q * 33 * n, true);
+            launch_chunk(ctx->stream, kGravity, f, pc, 0, n, k0, k1, ncorr, nullptr, 1, 0, W);
+            ctx->check_launch();
+        }
     }
     HIPC(hipEventRecord(e1.e, ctx->stream));
     std::vector<double> rs(3 * n), vs(3 * n);
@@ -1705,14 +1726,15 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
     ctx->ad_lam.reserve(sizeof(double) * R * 33 * n, &ctx->dev_bytes);
     ctx->ad_seeds.reserve(sizeof(double) * R * 6 * n, &ctx->dev_bytes);
     ctx->ad_out.reserve(sizeof(double) * R * 6 * n, &ctx->dev_bytes);
-    ctx->ad_G.reserve(sizeof(double) * C * ncorr * 6 * n, &ctx->dev_bytes);
     HIPC(hipMemsetAsync(ctx->ad_lam.p, 0, sizeof(double) * R * 33 * n, ctx->stream));
     HIPC(hipMemcpyAsync(ctx->ad_seeds.p, seeds_host.data(), sizeof(double) * R * 6 * n,
         hipMemcpyHostToDevice, ctx->stream));
     for (int64_t q = nseg - 1; q >= 0; --q) {
         const int64_t k0 = 5 + q * C, k1 = std::min<int64_t>(k0 + C, max_steps + 1);
-        launch_checkpoint(ctx->stream, pc, ctx->ad_ck.as<double>() + q * 33 * n, false);
-        launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>());
+        if (!record_all) {
+            launch_checkpoint(ctx->stream, pc, ctx->ad_ck.as<double>() + q * 33 * n, false);
+            launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>(), W);
+        }
         launch_gj8_reverse(ctx->stream, pc.nsteps, n, h, k0, k1, ncorr, ctx->ad_G.as<double>(), R,
             ctx->ad_seeds.as<double>(), ctx->ad_lam.as<double>());
         ctx->check_launch();
@@ -1820,7 +1842,7 @@ int ogp_adjoint_jacobian(ogp_context* ctx, const double* r0, const double* v0, i
         check_derivative_inputs(ctx, r0, v0, n, step_s, steps);
         arg_check(jac != nullptr, "null output array");
         arg_check(checkpoint_steps >= 0, "negative checkpoint interval");
-        const int64_t C = checkpoint_steps > 0 ? checkpoint_steps : 64;
+        const int64_t C = checkpoint_steps;
         std::vector<double> seeds(36 * n, 0.0);
         for (int rho = 0; rho < 6; ++rho)
             for (int64_t o = 0; o < n; ++o)
@@ -1841,7 +1863,7 @@ int ogp_adjoint_vjp(ogp_context* ctx, const double* r0, const double* v0, int64_
         arg_check(checkpoint_steps >= 0, "negative checkpoint interval");
         check_finite(g_r, 3 * n, "position gradients");
         check_finite(g_v, 3 * n, "velocity gradients");
-        const int64_t C = checkpoint_steps > 0 ? checkpoint_steps : 64;
+        const int64_t C = checkpoint_steps;
         std::vector<double> seeds(6 * n), out(6 * n);
         for (int64_t o = 0; o < n; ++o)
             for (int c = 0; c < 3; ++c) {

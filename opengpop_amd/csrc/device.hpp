@@ -292,6 +292,59 @@ __device__ __forceinline__ void gravity_itrs_jac(
     }
 }
 
+// Column-split Jacobian evaluation for a workgroup of W waves holding the same 64 objects (the
+// dual-number counterpart of gravity_itrs_split below): each part's Dual3 sums (16 doubles per
+// lane) go to LDS, red holding kParts * 16 * 64 doubles, and every wave folds the kParts slots in
+// part order, so value and Jacobian are bit-identical to gravity_itrs_jac for every W.
+constexpr int kJacExchange = kParts * 16 * 64;
+template <int W>
+__device__ __forceinline__ void gravity_itrs_jac_split(double x, double y, double z,
+    const GravityDev& g, double* red, double* a, double (*G)[3])
+{
+    if (W == 1) {
+        gravity_itrs_jac(x, y, z, g, a, G);
+        return;
+    }
+    static_assert(kParts % W == 0, "waves must divide the part count");
+    const Dual3 X = dvar(x, 0), Y = dvar(y, 1), Z = dvar(z, 2);
+    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+    const int lane = threadIdx.x % 64;
+    auto put = [&](int q, int t, const Dual3& d) {
+        double* o = red + (q * 16 + 4 * t) * 64 + lane;
+        o[0] = d.v;
+        o[64] = d.d[0];
+        o[128] = d.d[1];
+        o[192] = d.d[2];
+    };
+    auto get = [&](int q, int t) {
+        const double* o = red + (q * 16 + 4 * t) * 64 + lane;
+        return Dual3(o[0], o[64], o[128], o[192]);
+    };
+    __syncthreads();
+    grav_parts<false>(p, g, w * (kParts / W), (w + 1) * (kParts / W),
+        [&](int q, const GravSumsT<Dual3>& s) {
+            put(q, 0, s.G);
+            put(q, 1, s.H);
+            put(q, 2, s.J);
+            put(q, 3, s.K);
+        });
+    __syncthreads();
+    PartFoldT<Dual3> f;
+#pragma unroll 1
+    for (int q = 0; q < kParts; ++q)
+        f(q, GravSumsT<Dual3> { get(q, 0), get(q, 1), get(q, 2), get(q, 3) });
+    Dual3 A[3];
+    grav_assemble(X, Y, Z, p, f.t, g.gm, A[0], A[1], A[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        a[i] = A[i].v;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            G[i][j] = A[i].d[j];
+    }
+}
+
 // Column-split evaluation for a workgroup of W waves that all hold the same 64 objects: wave w
 // sums parts [w kParts / W, (w + 1) kParts / W), every part sum goes to its LDS slot, and every
 // wave adds the kParts slots in part order, which is the fold of the one-wave evaluation, so the
@@ -353,15 +406,16 @@ __device__ __forceinline__ void gravity_gcrs(const double* Mg, const double* r,
 
 // Acceleration and Jacobian in the GCRS: a = M g(M^T r), G = M Gi M^T (Gi the ITRS Jacobian).
 // The acceleration is bit-identical to gravity_gcrs<false>.
+template <int W = 1>
 __device__ __forceinline__ void gravity_gcrs_jac(const double* Mg, const double* r,
-    const GravityDev& g, double* a, double (*G)[3])
+    const GravityDev& g, double* a, double (*G)[3], double* red = nullptr)
 {
     const uniform_dptr M = (uniform_dptr)Mg;
     double ri[3], gi[3], Gi[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
-    gravity_itrs_jac(ri[0], ri[1], ri[2], g, gi, Gi);
+    gravity_itrs_jac_split<W>(ri[0], ri[1], ri[2], g, red, gi, Gi);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));

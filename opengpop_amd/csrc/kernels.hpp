@@ -114,8 +114,11 @@ void launch_tangent_final(hipStream_t st, const PropState& ps7, int64_t n, doubl
 void launch_tangent_split(hipStream_t st, const PropState& ps7, int64_t n, const PropState& pc,
     int64_t* pc_ns, double* Zd, double* RVd);
 void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_checkpoint);
+// Steps k0 .. k1-1 of the plain state ps with the gravity Jacobian of every corrector iterate
+// stored (6 unique entries); the state is advanced as by the chunk kernel. `waves` (4 below 8,
+// else 8) splits the harmonic columns as in the chunk kernel; results do not depend on it.
 void launch_gj8_record(hipStream_t st, const ForceCtx& f, const PropState& ps, int64_t k0,
-    int64_t k1, int ncorr, double* Gbuf);
+    int64_t k1, int ncorr, double* Gbuf, int waves);
 void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
     int64_t k1, int ncorr, const double* Gbuf, int R, const double* seeds, double* lam);
 void launch_adjoint_combine(hipStream_t st, const int64_t* nsteps, int64_t n, int R,

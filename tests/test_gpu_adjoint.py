@@ -89,6 +89,11 @@ def test_adjoint_uneven_spans_and_checkpoint_interval(ctx20):
     _, _, J7 = ctx20.adjoint_jacobian(r0, v0, 60.0, steps, checkpoint_steps=7)
     _, _, J64 = ctx20.adjoint_jacobian(r0, v0, 60.0, steps, checkpoint_steps=64)
     assert np.array_equal(J7, J64)  # recomputation from checkpoints is deterministic
+    # automatic interval: the forward pass records the whole span (no checkpoints, no re-run)
+    ra, va, Ja = ctx20.adjoint_jacobian(r0, v0, 60.0, steps)
+    rp, vp = ctx20.propagate(r0, v0, 60.0, steps)
+    assert np.array_equal(Ja, J7)
+    assert np.array_equal(ra, rp) and np.array_equal(va, vp)
     assert np.array_equal(J7[0], np.eye(6))
     assert _rel(J7, Phi).max() < 1e-10
 
