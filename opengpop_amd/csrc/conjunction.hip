@@ -4,15 +4,12 @@
 // reference updates a per-pair running minimum in HBM after every step (492 B of state per
 // pair). Here the trajectory of every object is stored once ([step][component][object]) and a
 // pair tile (64 x 64 objects) walks the whole time axis with its running minima in registers;
-// object positions are staged through LDS 8 steps at a time. Per pair the state is the best
-// squared distance and its step.
+// object positions are staged through LDS 8 steps at a time. Per pair the state is the smallest
+// squared distance and the last 8-step stage in which its square root dropped.
 //
 // Decision rule, identical to the reference's "dist < closest" on sqrt'ed distances with the
-// first occurrence kept: a sample is taken iff d2 < best_d2 and sqrt(d2) < sqrt(best_d2).
-// The first test is implied by the second (sqrt is monotone) and filters out almost all
-// samples, so the square root runs only when a pair improves. The squared distance is formed
-// as fma(dz, dz, fma(dy, dy, dx * dx)) exactly as in the host oracle, so identical
-// trajectories give identical bits.
+// first occurrence kept; the squared distance is formed as fma(dz, dz, fma(dy, dy, dx * dx))
+// exactly as in the host oracle, so identical trajectories give identical bits.
 //
 // Refinement (CalculateClosestApproachesKernel, device.cu:93-173; interpolation.cuh:27-152):
 // one thread per candidate pair; 8th-degree Hermite interpolation through the samples at
@@ -24,10 +21,6 @@
 
 namespace ogp {
 
-#ifndef OGP_SCREEN_PREFETCH
-#define OGP_SCREEN_PREFETCH 1
-#endif
-
 namespace {
 constexpr int TA = 64, TB = 64, PA = 4, PB = 4, STAGE = 8, THREADS = 256;
 // elements of one stage (STAGE steps x 3 components x (TA + TB) objects) per thread
@@ -38,19 +31,31 @@ static_assert(STAGE * 3 * (TA + TB) % THREADS == 0, "stage must divide over the 
 // samples within a few ulps of the running minimum.
 constexpr double kShrink = 1.0 - 0x1p-50;
 
-// Stage element e -> (tile, step, component, object) and its trajectory value (0 past the end).
-__device__ __forceinline__ double stage_load(const double* __restrict__ traj, int64_t N,
-    int64_t nsteps, int64_t k0, int e, int64_t ta0, int64_t tb0, int64_t alim, int64_t blim)
+// v_min_f64 without the quieting v_max_f64 the compiler puts in front of a loop-carried operand:
+// every operand here is the result of arithmetic or the constant inf, never a signalling NaN,
+// and with one quiet NaN operand the instruction returns the other operand.
+__device__ __forceinline__ double min_f64(double a, double b)
 {
-    const int which = e / (STAGE * 3 * TA); // 0 = A, 1 = B (TA == TB)
-    const int rem = e % (STAGE * 3 * TA);
-    const int st = rem / (3 * TA);
-    const int c = (rem / TA) % 3;
-    const int o = rem % TA;
-    const int64_t k = k0 + st;
-    const int64_t obj = (which == 0 ? ta0 : tb0) + o;
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// A stage is 48 rows of 64 objects: row r holds tile r / 24 (0 = A, 1 = B), step (r % 24) / 3
+// and component r % 3. Wave w of the block loads rows w, w + 4, ..., one object per lane, so the
+// row arithmetic is wave-uniform and each load is 512 contiguous bytes. Values past the end of
+// the time axis or of a set are 0 and never screened.
+static_assert(TA == 64 && TB == 64 && THREADS == 256 && STAGE_ELEMS * 4 == STAGE * 3 * 2,
+    "stage rows assume 64-object tiles and 4 waves");
+__device__ __forceinline__ double stage_load(const double* __restrict__ traj, int64_t N,
+    int64_t nsteps, int64_t k0, int row, int lane, int64_t ta0, int64_t tb0, int64_t alim,
+    int64_t blim)
+{
+    const int which = row / 24, r = row % 24;
+    const int64_t k = k0 + r / 3;
+    const int64_t obj = (which == 0 ? ta0 : tb0) + lane;
     if (k < nsteps && obj < (which == 0 ? alim : blim))
-        return traj[(k * 9 + c) * N + obj];
+        return traj[(k * 9 + r % 3) * N + obj];
     return 0.0;
 }
 }
@@ -59,9 +64,27 @@ __device__ __forceinline__ double stage_load(const double* __restrict__ traj, in
 // objects ty*PA .. ty*PA+3 and B objects {2tx, 2tx+1, 32+2tx, 32+2tx+1}: neighbouring lanes read
 // neighbouring 16-byte pairs of the staged B positions (no LDS bank conflicts). The next stage's
 // positions are loaded into registers while the current stage is screened.
+//
+// The reference's decision is deferred instead of being made per sample. Its minimum after any
+// number of samples has the square root of the smallest d2 so far, and its step is the first
+// sample whose root equals that. So per sample only the minimum over a window of SPAN samples
+// is kept (one v_min_f64). At the end of each window the thread records whether the root
+// strictly dropped in it: certainly when the window's minimum is below mn * kShrink, not at all
+// when it is not below mn, and by exact roots in the rare case in between. After the time axis
+// the step is the first sample, in the last window where the root dropped, whose root equals
+// sqrt(mn); that scan reads the trajectory again only for pairs that are written out. A NaN
+// distance is never taken (the minimum keeps the other operand), as under the reference's
+// comparison.
 #ifndef OGP_SCREEN_WAVES_PER_EU
 #define OGP_SCREEN_WAVES_PER_EU 2
 #endif
+#ifndef OGP_SCREEN_UNROLL
+#define OGP_SCREEN_UNROLL 1
+#endif
+#ifndef OGP_SCREEN_WINDOW
+#define OGP_SCREEN_WINDOW 2
+#endif
+constexpr int SPAN = STAGE * OGP_SCREEN_WINDOW; // samples per decision window
 __global__ void __launch_bounds__(THREADS)
 __attribute__((amdgpu_waves_per_eu(OGP_SCREEN_WAVES_PER_EU)))
 screen_kernel(const double* __restrict__ traj,
@@ -82,45 +105,46 @@ screen_kernel(const double* __restrict__ traj,
 #pragma unroll
     for (int j = 0; j < PB; ++j)
         bcol[j] = (j / 2) * 32 + 2 * tx + (j % 2);
-    double best[PA][PB];
-    int32_t bk[PA][PB];
+    double mn[PA][PB];     // smallest d2 before the current window
+    double wm[PA][PB];     // smallest d2 in the current window
+    int32_t sk[PA][PB];    // last window in which sqrt(mn) strictly dropped (-1: none)
 #pragma unroll
     for (int i = 0; i < PA; ++i)
 #pragma unroll
         for (int j = 0; j < PB; ++j) {
-            best[i][j] = __builtin_inf();
-            bk[i][j] = -1;
+            mn[i][j] = __builtin_inf();
+            sk[i][j] = -1;
         }
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int lane = tid % 64;
     double pre[STAGE_ELEMS];
 #pragma unroll
     for (int q = 0; q < STAGE_ELEMS; ++q)
-        pre[q] = stage_load(traj, N, nsteps, 0, tid + q * THREADS, ta0, tb0, alim, blim);
+        pre[q] = stage_load(traj, N, nsteps, 0, 4 * q + wave, lane, ta0, tb0, alim, blim);
     for (int64_t k0 = 0; k0 < nsteps; k0 += STAGE) {
         __syncthreads(); // the previous stage has been screened by every thread
-#if !OGP_SCREEN_PREFETCH
-        if (k0 > 0) {
-#pragma unroll
-            for (int q = 0; q < STAGE_ELEMS; ++q)
-                pre[q] = stage_load(traj, N, nsteps, k0, tid + q * THREADS, ta0, tb0, alim, blim);
-        }
-#endif
 #pragma unroll
         for (int q = 0; q < STAGE_ELEMS; ++q) {
-            const int e = tid + q * THREADS;
-            const int which = e / (STAGE * 3 * TA);
-            const int rem = e % (STAGE * 3 * TA);
-            double* dst = which == 0 ? &sA[0][0][0] : &sB[0][0][0];
-            dst[rem] = pre[q];
+            const int row = 4 * q + wave;
+            double* dst = row < 24 ? &sA[0][0][0] : &sB[0][0][0];
+            dst[(row % 24) * 64 + lane] = pre[q];
         }
         __syncthreads();
-        if (OGP_SCREEN_PREFETCH && k0 + STAGE < nsteps) { // flies while this stage is screened
+        if (k0 + STAGE < nsteps) { // flies while this stage is screened
 #pragma unroll
             for (int q = 0; q < STAGE_ELEMS; ++q)
-                pre[q] = stage_load(traj, N, nsteps, k0 + STAGE, tid + q * THREADS, ta0, tb0,
+                pre[q] = stage_load(traj, N, nsteps, k0 + STAGE, 4 * q + wave, lane, ta0, tb0,
                     alim, blim);
         }
+        if (k0 % SPAN == 0) {
+#pragma unroll
+            for (int i = 0; i < PA; ++i)
+#pragma unroll
+                for (int j = 0; j < PB; ++j)
+                    wm[i][j] = __builtin_inf();
+        }
         const int smax = (int)((nsteps - k0) < STAGE ? (nsteps - k0) : STAGE);
-        for (int st = 0; st < smax; ++st) {
+        auto screen_step = [&](int st) {
             double ax[PA], ay[PA], az[PA], bx[PB], by[PB], bz[PB];
 #pragma unroll
             for (int i = 0; i < PA; ++i) {
@@ -134,11 +158,6 @@ screen_kernel(const double* __restrict__ traj,
                 by[j] = sB[st][1][bcol[j]];
                 bz[j] = sB[st][2][bcol[j]];
             }
-            const int32_t k = (int32_t)(k0 + st);
-            // Branch-free update: a sample below best * kShrink is an improvement under the
-            // reference's rule; one within a few ulps of the minimum is flagged and the step
-            // is decided exactly below (updates already made are no-ops there).
-            bool unsure = false;
 #pragma unroll
             for (int i = 0; i < PA; ++i)
 #pragma unroll
@@ -147,26 +166,44 @@ screen_kernel(const double* __restrict__ traj,
                     const double dy = ay[i] - by[j];
                     const double dz = az[i] - bz[j];
                     const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                    const bool take = d2 < best[i][j] * kShrink;
-                    unsure |= (d2 < best[i][j]) & !take;
-                    best[i][j] = take ? d2 : best[i][j];
-                    bk[i][j] = take ? k : bk[i][j];
+                    wm[i][j] = min_f64(wm[i][j], d2);
                 }
-            if (unsure) {
+        };
+#if OGP_SCREEN_UNROLL
+        if (smax == STAGE) { // full stage: the next step's reads can overlap this step's math
+#pragma unroll
+            for (int st = 0; st < STAGE; ++st)
+                screen_step(st);
+        } else
+#endif
+        {
+            for (int st = 0; st < smax; ++st)
+                screen_step(st);
+        }
+        if ((k0 + STAGE) % SPAN == 0 || k0 + STAGE >= nsteps) { // end of a window
+            const int32_t win = (int32_t)(k0 / SPAN);
+            bool unsure = false;
+#pragma unroll
+            for (int i = 0; i < PA; ++i)
+#pragma unroll
+                for (int j = 0; j < PB; ++j) {
+                    const bool drop = wm[i][j] < mn[i][j] * kShrink;
+                    unsure |= (wm[i][j] < mn[i][j]) & !drop;
+                    sk[i][j] = drop ? win : sk[i][j];
+                }
+            if (unsure) { // a drop by a few ulps at most somewhere: compare the roots exactly
 #pragma unroll
                 for (int i = 0; i < PA; ++i)
 #pragma unroll
-                    for (int j = 0; j < PB; ++j) {
-                        const double dx = ax[i] - bx[j];
-                        const double dy = ay[i] - by[j];
-                        const double dz = az[i] - bz[j];
-                        const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                        if (d2 < best[i][j] && __builtin_sqrt(d2) < __builtin_sqrt(best[i][j])) {
-                            best[i][j] = d2;
-                            bk[i][j] = k;
-                        }
-                    }
+                    for (int j = 0; j < PB; ++j)
+                        if (wm[i][j] < mn[i][j] && __builtin_sqrt(wm[i][j]) < __builtin_sqrt(mn[i][j]))
+                            sk[i][j] = win;
             }
+#pragma unroll
+            for (int i = 0; i < PA; ++i)
+#pragma unroll
+                for (int j = 0; j < PB; ++j)
+                    mn[i][j] = min_f64(mn[i][j], wm[i][j]);
         }
     }
 #pragma unroll
@@ -178,18 +215,37 @@ screen_kernel(const double* __restrict__ traj,
             if (ia >= alim || jb >= blim)
                 continue;
             // pairs that never took a sample keep the reference's initial DBL_MAX
-            const double d = bk[i][j] < 0 ? 1.7976931348623157e308 : __builtin_sqrt(best[i][j]);
+            const double d = sk[i][j] < 0 ? 1.7976931348623157e308 : __builtin_sqrt(mn[i][j]);
+            const bool cand = threshold > 0.0 && d < threshold;
+            if (!min_dist && !cand)
+                continue;
+            int32_t k = -1;
+            if (sk[i][j] >= 0) { // first sample of that window whose root is the minimum
+                const int64_t kb = (int64_t)sk[i][j] * SPAN;
+                const int64_t ke = kb + SPAN < nsteps ? kb + SPAN : nsteps;
+                for (int64_t kk = kb; kk < ke; ++kk) {
+                    const double* p = traj + kk * 9 * N;
+                    const double dx = p[ia] - p[jb];
+                    const double dy = p[N + ia] - p[N + jb];
+                    const double dz = p[2 * N + ia] - p[2 * N + jb];
+                    const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                    if (__builtin_sqrt(d2) == d) {
+                        k = (int32_t)kk;
+                        break;
+                    }
+                }
+            }
             const int64_t p = pair0 + (ia - a0) * row_len + (jb - b0);
             if (min_dist) {
                 min_dist[p] = d;
-                min_step[p] = bk[i][j];
+                min_step[p] = k;
             }
-            if (threshold > 0.0 && d < threshold) {
+            if (cand) {
                 const uint32_t slot = atomicAdd(cand_count, 1u);
                 if (slot < cand_cap) {
                     cand_a[slot] = (uint32_t)ia;
                     cand_b[slot] = (uint32_t)jb;
-                    cand_k[slot] = bk[i][j];
+                    cand_k[slot] = k;
                     cand_d[slot] = d;
                 }
             }

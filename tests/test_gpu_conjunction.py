@@ -107,6 +107,45 @@ def test_screen_bit_exact_on_same_trajectory(screened, po):
     assert np.array_equal(mink, screened["mink"])
 
 
+def test_screen_ties_bit_exact(po):
+    # No forces. Objects 0-9 (set 0) and 160-179 (set 1) are static, so their distances repeat
+    # exactly at every sample (the first sample must win); 160-169 are copies of 0-9 (distance
+    # 0). The others share one velocity and sit ~10^7 m apart, so their distances move only by
+    # the rounding of the integrator's position updates, an ulp or so per step: the roots tie
+    # or nearly tie with the running minimum at many samples. 101 samples end in a partial
+    # 8-step stage.
+    rng = np.random.default_rng(11)
+    n = 200
+    u = rng.normal(size=(n, 3))
+    r = u / np.linalg.norm(u, axis=1)[:, None] * rng.uniform(6.5e6, 7.5e6, (n, 1))
+    v = np.tile([0.0, 7.5e3, 1.0e3], (n, 1))
+    static = np.r_[0:10, 160:180]
+    v[static] = 0.0
+    r[160:170] = r[0:10]
+    sets = np.repeat(np.arange(2, dtype=np.uint32), [70, 130])
+    with ogp.Context(force=ogp.FORCE_NONE) as c:
+        ca = ogp.ClosestApproach(c, 60.0)
+        ca.initialize(r, v, sets)
+        ca.propagate(101)
+        traj = ca.trajectory()
+        mind, mink = ca.screen()
+    ref_d, ref_k = po.Oracle(degree=2).screen(traj, sets)
+    assert np.array_equal(mind, ref_d)
+    assert np.array_equal(mink, ref_k)
+    idx = lambda a, b: a * 130 + (b - 70)  # pair order: (a in set 0, b in set 1)
+    ss = [idx(a, b) for a in range(10) for b in range(160, 180)]
+    assert np.all(mink[ss] == 0)
+    assert np.all(mind[[idx(a, 160 + a) for a in range(10)]] == 0.0)
+    # the near-ties are really there for the moving pairs, and minima fall in later stages
+    p = traj[:, :, :3]
+    mov = [a for a in range(10, 70)], [b for b in range(70, 200) if not 160 <= b < 180]
+    d = np.sqrt(((p[:, mov[0], None, :] - p[:, None, mov[1], :]) ** 2).sum(-1)).reshape(len(p), -1)
+    near = (d <= d.min(0) * (1 + 1e-15)).sum(0)
+    assert (near >= 2).mean() > 0.5
+    mk = mink[[idx(a, b) for a in mov[0] for b in mov[1]]]
+    assert np.any(mk >= 8) and np.any(mk >= 96)
+
+
 def test_refine_matches_oracle_on_same_trajectory(screened, po):
     o = po.Oracle(degree=20)
     res, sets, traj = screened["res"], screened["sets"], screened["traj"]
