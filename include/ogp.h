@@ -182,9 +182,10 @@ int ogp_last_timing(const ogp_context* ctx, ogp_timing* t);
  * 256-thread blocks per helper kernel): `waves` = waves per group of 64 objects that share one
  * object group's harmonic columns (1, 2, 4, 8; 0 = autotuned), chunk_steps = GJ8 steps per
  * launch (0 = 64). Results are bit-identical for every choice. With waves = 0 the context times
- * four chunks, one per candidate, the first time it meets a group count (per power of two) and
- * keeps the fastest; ogp_set_tuning clears that table. ogp_get_tuning reports the choice for
- * n objects. */
+ * one chunk per candidate the first time it meets a group count (per power of two) and keeps the
+ * fastest; ogp_set_tuning clears that table. A shape that gives fewer waves than the device has
+ * SIMDs is not a candidate unless it is the widest, so small populations take 8 waves without
+ * trials. ogp_get_tuning reports the choice for n objects. */
 int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps);
 int ogp_get_tuning(const ogp_context* ctx, int64_t n, int* waves, int64_t* chunk_steps);
 

@@ -179,9 +179,11 @@ struct ogp_context {
         return b;
     }
 
+    int simds = 1024; // SIMDs of the device (4 per CU), read at creation
+
     // Waves per 64-object group: the override, else the tuned value for this size, else enough
-    // waves for about 1200 resident on the chip's 1024 SIMDs, which hides most of the latency of
-    // the scalar coefficient loads.
+    // waves for about 1.2 resident per SIMD, which hides most of the latency of the scalar
+    // coefficient loads.
     int waves_for(int64_t n) const
     {
         if (kind != kGravity && kind != kGravitySplit)
@@ -193,9 +195,23 @@ struct ogp_context {
             return it->second;
         const int64_t groups = (n + 63) / 64;
         int w = 1;
-        while (w < 8 && groups * w < 1200)
+        while (w < kMaxWaves && groups * w < simds + simds / 5)
             w *= 2;
         return w;
+    }
+
+    // Shapes the tuner tries at n objects. One that cannot give every SIMD a wave
+    // (groups * W < simds) leaves most of the chip idle and is not tried, unless it is the
+    // widest. Since simds and the bucket bounds are powers of two, the set is the same for every
+    // size in a tuning bucket.
+    std::vector<int> tuning_candidates(int64_t n) const
+    {
+        const int64_t groups = (n + 63) / 64;
+        std::vector<int> c;
+        for (int w = 1; w <= kMaxWaves; w *= 2)
+            if (groups * w >= simds || w == kMaxWaves)
+                c.push_back(w);
+        return c;
     }
 
     bool needs_tuning(int64_t n) const
@@ -461,7 +477,6 @@ struct ogp_context {
         for (hipStream_t s : used)
             join(s, { stream });
 
-        static const int kTrial[4] = { 1, 2, 4, 8 };
         std::vector<int64_t> k0(groups.size(), 5);
         for (bool more = true; more;) {
             more = false;
@@ -487,17 +502,22 @@ struct ogp_context {
                     }
                     k0[q] += steps > 0 ? steps : chunk;
                 };
-                if (needs_tuning(active) && g.max_steps - k0[q] + 1 >= 4 * chunk) {
-                    // the next four chunks of the class, one per candidate, run as every other
-                    // chunk does (the slices concurrently on their streams); a candidate's rate
-                    // is the sum over slices of object-steps / launch duration, and the fastest
-                    // is kept. The chunks do real work (results do not depend on W).
-                    std::vector<std::pair<size_t, double>> trial[4];
-                    for (int c = 0; c < 4; ++c)
-                        class_chunk(kTrial[c], &trial[c]);
+                const std::vector<int> cand = needs_tuning(active) ? tuning_candidates(active)
+                                                                   : std::vector<int>();
+                if (cand.size() == 1)
+                    tuned_waves[group_bucket(active)] = cand[0];
+                if (cand.size() > 1 && g.max_steps - k0[q] + 1 >= (int64_t)cand.size() * chunk) {
+                    // the next chunks of the class, one per candidate, run as every other chunk
+                    // does (the slices concurrently on their streams); a candidate's rate is the
+                    // sum over slices of object-steps / launch duration, and the fastest is kept.
+                    // The chunks do real work (results do not depend on W).
+                    const int nc = (int)cand.size();
+                    std::vector<std::pair<size_t, double>> trial[kMaxWaves];
+                    for (int c = 0; c < nc; ++c)
+                        class_chunk(cand[c], &trial[c]);
                     int best = 0;
                     double best_rate = 0.0;
-                    for (int c = 0; c < 4; ++c) {
+                    for (int c = 0; c < nc; ++c) {
                         double rate = 0.0;
                         for (const auto& t : trial[c]) {
                             const auto& sp = spans[t.first];
@@ -508,7 +528,7 @@ struct ogp_context {
                         }
                         if (rate > best_rate) {
                             best_rate = rate;
-                            best = kTrial[c];
+                            best = cand[c];
                         }
                     }
                     tuned_waves[group_bucket(active)] = best;
@@ -625,6 +645,9 @@ int ogp_create(const ogp_config* cfg, ogp_context** out)
         if (ndev < 1)
             throw ApiError(OGP_ERR_DEVICE, "no HIP device");
         HIPC(hipSetDevice(0));
+        int cus = 0;
+        HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+        c.simds = std::max(4, 4 * cus);
         HIPC(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         for (hipStream_t& s : c.lane_streams)
             HIPC(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));

@@ -255,17 +255,20 @@ def test_concurrent_slices_do_not_change_results():
 
 def test_autotuner_keeps_results_and_records_choice():
     d = scenarios.config0()
-    r0, v0 = np.tile(d["r"], (10, 1)), np.tile(d["v"], (10, 1))  # 640 objects, 10 groups
-    with ogp.Context(degree=20) as c:
-        c.set_tuning(1, 16)
-        r1, v1 = c.propagate(r0, v0, 60.0, 120)
-        c.set_tuning(0, 16)  # autotune: 115 GJ steps = 8 chunks of 16, four of them trials
-        ra, va = c.propagate(r0, v0, 60.0, 120)
-        assert np.array_equal(r1, ra) and np.array_equal(v1, va)
-        w, chunk = c.get_tuning(640)
-        assert w in (1, 2, 4, 8) and chunk == 16
-        rb, vb = c.propagate(r0, v0, 60.0, 120)  # tuned table reused
-        assert np.array_equal(r1, rb) and np.array_equal(v1, vb)
+    # 640 objects (10 groups): only the widest shape can give every SIMD a wave, so it is taken
+    # without trials; 16,640 objects (260 groups): 4 and 8 waves are timed against each other
+    for reps, choices in ((10, (8,)), (260, (4, 8))):
+        r0, v0 = np.tile(d["r"], (reps, 1)), np.tile(d["v"], (reps, 1))
+        with ogp.Context(degree=20) as c:
+            c.set_tuning(1, 16)
+            r1, v1 = c.propagate(r0, v0, 60.0, 120)
+            c.set_tuning(0, 16)  # autotune: 115 GJ steps = 8 chunks of 16
+            ra, va = c.propagate(r0, v0, 60.0, 120)
+            assert np.array_equal(r1, ra) and np.array_equal(v1, va)
+            w, chunk = c.get_tuning(len(r0))
+            assert w in choices and chunk == 16
+            rb, vb = c.propagate(r0, v0, 60.0, 120)  # tuned table reused
+            assert np.array_equal(r1, rb) and np.array_equal(v1, vb)
 
 
 def test_errors_are_reported_not_fatal(ctx20):
