@@ -253,8 +253,10 @@ int ogp_conj_last_timing(const ogp_context* ctx, ogp_conj_timing* t);
  * the acceleration is bit-identical to ogp_gravity_itrs. */
 int ogp_gravity_jacobian_itrs(ogp_context* ctx, const double* r, int64_t n, double* a, double* G);
 
-/* State-transition matrix from the variational equations: the six tangents integrated with the
- * same RK8 startup and GJ8 steps as the state (forward mode). Also returns the final state. */
+/* State-transition matrix from the variational equations of the discrete scheme (forward mode):
+ * the six tangents are carried through the same RK8 startup and GJ8 steps as the state, with
+ * the gravity Jacobian of every corrector iterate evaluated once per object. Also returns the
+ * final state (bit-identical to ogp_propagate). */
 int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
     const int64_t* steps, double* r_out, double* v_out, double* stm);
 

@@ -75,25 +75,76 @@ __global__ void tangent_init_kernel(const double* __restrict__ r0, const double*
     ns_out[L] = used ? steps[o] : 0;
 }
 
-// After the full variational run: final state and the 6x6 matrix [o][row][col].
-__global__ void tangent_final_kernel(PropState ps, int64_t n, double* __restrict__ r_out,
-    double* __restrict__ v_out, double* __restrict__ stm)
+// Forward tangent pass over GJ8 steps k0 .. k1-1: the variational equations of the discrete
+// scheme, driven by the Jacobians the record pass stored for the same steps. Lane (o, col) carries
+// the derivative with respect to initial component col of the object's (s, S, ring), in place in
+// Zd[(col * 33 + e) * n + o], and of its (r, v) in RVd[(col * 6 + e) * n + o]. The integrator is
+// linear in these, so the step arithmetic is gj8.hpp's, applied to the derivatives; the
+// acceleration of a derivative is G dr (G symmetric, six stored entries). Block b holds objects
+// 64 (b / 6) .. +63 for column b % 6: the six blocks of an object group are adjacent in the grid
+// and read the same Jacobians close together in time.
+__global__ void __launch_bounds__(64) gj8_tangent_kernel(const int64_t* __restrict__ nsteps,
+    int64_t n, double h, int64_t k0, int64_t k1, int ncorr, const double* __restrict__ Gbuf,
+    double* __restrict__ Zd, double* __restrict__ RVd)
+{
+    const int col = (int)(blockIdx.x % 6);
+    const int64_t o = (int64_t)(blockIdx.x / 6) * 64 + threadIdx.x;
+    if (o >= n)
+        return;
+    const int64_t ns = nsteps[o];
+    if (ns < k0)
+        return;
+    double* Z = Zd + (int64_t)col * 33 * n;
+    double* RV = RVd + (int64_t)col * 6 * n;
+    double* ring = Z + 6 * n;
+    const double hh = h * h;
+    double s[3], S[3], r[3], v[3];
+    for (int c = 0; c < 3; ++c) {
+        s[c] = Z[c * n + o];
+        S[c] = Z[(3 + c) * n + o];
+        r[c] = RV[c * n + o];
+        v[c] = RV[(3 + c) * n + o];
+    }
+    for (int64_t k = k0; k < k1; ++k) {
+        if (k > ns)
+            break;
+        GJ8Sums q;
+        gj8_sums(ring, n, o, k, q);
+        double sprev[3], anew[3];
+        gj8_predict(q, h, s, S, r, v, sprev);
+        for (int it = 0; it < ncorr; ++it) {
+            const double* gp = Gbuf + ((((k - k0) * ncorr + it) * 6) * n + o);
+            const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
+                         gyz = gp[4 * n], gzz = gp[5 * n];
+            anew[0] = gxx * r[0] + gxy * r[1] + gxz * r[2];
+            anew[1] = gxy * r[0] + gyy * r[1] + gyz * r[2];
+            anew[2] = gxz * r[0] + gyz * r[1] + gzz * r[2];
+            gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+        }
+        const int wslot = ring_slot(k);
+        for (int c = 0; c < 3; ++c)
+            ring[(wslot * 3 + c) * n + o] = anew[c];
+    }
+    for (int c = 0; c < 3; ++c) {
+        Z[c * n + o] = s[c];
+        Z[(3 + c) * n + o] = S[c];
+        RV[c * n + o] = r[c];
+        RV[(3 + c) * n + o] = v[c];
+    }
+}
+
+// The 6x6 matrix stm[o][row][col] from the derivatives of the final (r, v) (zero steps: the
+// identity).
+__global__ void stm_final_kernel(const int64_t* __restrict__ nsteps, int64_t n,
+    const double* __restrict__ RVd, double* __restrict__ stm)
 {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= n)
         return;
-    const int64_t N = ps.N, L0 = tlane(o, 0);
-    for (int c = 0; c < 3; ++c) {
-        r_out[3 * o + c] = ps.r[c * N + L0];
-        v_out[3 * o + c] = ps.v[c * N + L0];
-    }
-    for (int col = 0; col < 6; ++col) {
-        const int64_t L = L0 + 1 + col;
-        for (int c = 0; c < 3; ++c) {
-            stm[36 * o + 6 * c + col] = ps.r[c * N + L];
-            stm[36 * o + 6 * (3 + c) + col] = ps.v[c * N + L];
-        }
-    }
+    const int64_t ns = nsteps[o];
+    for (int col = 0; col < 6; ++col)
+        for (int e = 0; e < 6; ++e)
+            stm[36 * o + 6 * e + col] = ns == 0 ? (e == col ? 1.0 : 0.0) : RVd[(col * 6 + e) * n + o];
 }
 
 // After the variational startup: the state lanes become the compact state (n objects) of the
@@ -339,12 +390,20 @@ void launch_tangent_init(hipStream_t st, const double* r0, const double* v0, con
         r0, v0, steps, n, ps7, ns7);
 }
 
-void launch_tangent_final(hipStream_t st, const PropState& ps7, int64_t n, double* r_out,
-    double* v_out, double* stm)
+void launch_gj8_tangent(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
+    int64_t k1, int ncorr, const double* Gbuf, double* Zd, double* RVd)
+{
+    if (n > 0 && k1 > k0)
+        hipLaunchKernelGGL(gj8_tangent_kernel, dim3((unsigned)(6 * ((n + 63) / 64))), dim3(64), 0,
+            st, nsteps, n, h, k0, k1, ncorr, Gbuf, Zd, RVd);
+}
+
+void launch_stm_final(hipStream_t st, const int64_t* nsteps, int64_t n, const double* RVd,
+    double* stm)
 {
     if (n > 0)
-        hipLaunchKernelGGL(tangent_final_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-            ps7, n, r_out, v_out, stm);
+        hipLaunchKernelGGL(stm_final_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+            nsteps, n, RVd, stm);
 }
 
 void launch_tangent_split(hipStream_t st, const PropState& ps7, int64_t n, const PropState& pc,

@@ -1682,20 +1682,14 @@ PropState tangent_startup(ogp_context* ctx, const double* r0, const double* v0, 
 // checkpoints and every segment is recorded again in the reverse sweep.
 constexpr double kRecordAllBytes = 16.0 * (1 << 30);
 
-// Reverse mode for R cotangent rows seeded by seeds_host[(rho * 6 + e) * n + o]; writes the
-// final state (AoS) and out_host[(o * R + rho) * 6 + col]. C = checkpoint interval in steps,
-// 0 = automatic.
-void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double h,
-    const int64_t* steps, int64_t C, int R, const std::vector<double>& seeds_host,
-    double* r_out, double* v_out, double* out_host)
+// Startup on the variational layout, then its split into the compact state pc (n objects,
+// ctx->ad_* buffers) and the derivatives of the post-startup state: Zd (s, S, ring) and RVd
+// (r, v), see launch_tangent_split.
+PropState derivative_start(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double h, const int64_t* steps, ForceCtx* f)
 {
-    Event e0, e1, e2;
-    HIPC(hipEventRecord(e0.e, ctx->stream));
     NodeTable* nt = nullptr;
-    ForceCtx f;
-    const PropState ps7 = tangent_startup(ctx, r0, v0, n, h, steps, &nt, &f);
-    const int ncorr = ctx->cfg.n_corrections;
-    const int64_t max_steps = *std::max_element(steps, steps + n);
+    const PropState ps7 = tangent_startup(ctx, r0, v0, n, h, steps, &nt, f);
     const size_t v3 = sizeof(double) * 3 * n;
     ctx->ad_r.reserve(v3, &ctx->dev_bytes);
     ctx->ad_v.reserve(v3, &ctx->dev_bytes);
@@ -1716,6 +1710,39 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
     launch_tangent_split(ctx->stream, ps7, n, pc, ctx->ad_ns.as<int64_t>(),
         ctx->ad_Zd.as<double>(), ctx->ad_RVd.as<double>());
     ctx->check_launch();
+    return pc;
+}
+
+// Final state of the compact layout, SoA on the device, to AoS host arrays (either may be null).
+void copy_state_out(ogp_context* ctx, const PropState& pc, double* r_out, double* v_out)
+{
+    const int64_t n = pc.N;
+    std::vector<double> rs(3 * n), vs(3 * n);
+    HIPC(hipMemcpyAsync(rs.data(), pc.r, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(vs.data(), pc.v, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    for (int64_t o = 0; o < n; ++o)
+        for (int c = 0; c < 3; ++c) {
+            if (r_out)
+                r_out[3 * o + c] = rs[c * n + o];
+            if (v_out)
+                v_out[3 * o + c] = vs[c * n + o];
+        }
+}
+
+// Reverse mode for R cotangent rows seeded by seeds_host[(rho * 6 + e) * n + o]; writes the
+// final state (AoS) and out_host[(o * R + rho) * 6 + col]. C = checkpoint interval in steps,
+// 0 = automatic.
+void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double h,
+    const int64_t* steps, int64_t C, int R, const std::vector<double>& seeds_host,
+    double* r_out, double* v_out, double* out_host)
+{
+    Event e0, e1, e2;
+    HIPC(hipEventRecord(e0.e, ctx->stream));
+    ForceCtx f;
+    const PropState pc = derivative_start(ctx, r0, v0, n, h, steps, &f);
+    const int ncorr = ctx->cfg.n_corrections;
+    const int64_t max_steps = *std::max_element(steps, steps + n);
     const int64_t gj_steps = std::max<int64_t>(max_steps - 4, 0);
     const bool record_all = C == 0
         && (double)gj_steps * ncorr * 6 * (double)n * sizeof(double) <= kRecordAllBytes;
@@ -1742,9 +1769,7 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
         }
     }
     HIPC(hipEventRecord(e1.e, ctx->stream));
-    std::vector<double> rs(3 * n), vs(3 * n);
-    HIPC(hipMemcpyAsync(rs.data(), pc.r, v3, hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipMemcpyAsync(vs.data(), pc.v, v3, hipMemcpyDeviceToHost, ctx->stream));
+    copy_state_out(ctx, pc, r_out, v_out);
     // reverse pass, segment by segment from the last
     ctx->ad_lam.reserve(sizeof(double) * R * 33 * n, &ctx->dev_bytes);
     ctx->ad_seeds.reserve(sizeof(double) * R * 6 * n, &ctx->dev_bytes);
@@ -1775,13 +1800,6 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
     HIPC(hipEventElapsedTime(&b, e1.e, e2.e));
     ctx->ad_timing.forward_ms = a;
     ctx->ad_timing.adjoint_ms = b;
-    for (int64_t o = 0; o < n; ++o)
-        for (int c = 0; c < 3; ++c) {
-            if (r_out)
-                r_out[3 * o + c] = rs[c * n + o];
-            if (v_out)
-                v_out[3 * o + c] = vs[c * n + o];
-        }
 }
 
 } // namespace
@@ -1823,32 +1841,35 @@ int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, dou
     return guarded(ctx, [&] {
         check_derivative_inputs(ctx, r0, v0, n, step_s, steps);
         arg_check(r_out && v_out && stm, "null output array");
+        // forward mode: each segment of GJ8 steps is recorded (the trajectory and the Jacobian
+        // of every corrector iterate, the propagation's arithmetic) and then the six tangent
+        // columns are carried through it; one segment when the whole span's Jacobians fit
         Event e0, e1;
         HIPC(hipEventRecord(e0.e, ctx->stream));
-        NodeTable* nt = nullptr;
         ForceCtx f;
-        const PropState ps7 = tangent_startup(ctx, r0, v0, n, step_s, steps, &nt, &f);
+        const PropState pc = derivative_start(ctx, r0, v0, n, step_s, steps, &f);
+        const int ncorr = ctx->cfg.n_corrections;
         const int64_t max_steps = *std::max_element(steps, steps + n);
-        for (int64_t k0 = 5; k0 <= max_steps; k0 += ctx->chunk) {
-            const int64_t k1 = std::min<int64_t>(k0 + ctx->chunk, max_steps + 1);
-            launch_chunk(ctx->stream, kGravityTangent, f, ps7, 0, ps7.N, k0, k1,
-                ctx->cfg.n_corrections, nullptr, 1, 0, 1);
+        const int64_t gj_steps = std::max<int64_t>(max_steps - 4, 0);
+        const double per_step = (double)ncorr * 6 * (double)n * sizeof(double);
+        const int64_t C = std::max<int64_t>(1,
+            std::min<int64_t>(gj_steps, (int64_t)(kRecordAllBytes / per_step)));
+        const int W = ctx->waves_for(n);
+        ctx->ad_G.reserve(sizeof(double) * C * ncorr * 6 * n, &ctx->dev_bytes);
+        for (int64_t k0 = 5; k0 <= max_steps; k0 += C) {
+            const int64_t k1 = std::min<int64_t>(k0 + C, max_steps + 1);
+            launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>(), W);
+            launch_gj8_tangent(ctx->stream, pc.nsteps, n, step_s, k0, k1, ncorr,
+                ctx->ad_G.as<double>(), ctx->ad_Zd.as<double>(), ctx->ad_RVd.as<double>());
             ctx->check_launch();
         }
-        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
-        ctx->io_b.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
         ctx->io_c.reserve(sizeof(double) * 36 * n, &ctx->dev_bytes);
-        launch_tangent_final(ctx->stream, ps7, n, ctx->io_a.as<double>(), ctx->io_b.as<double>(),
-            ctx->io_c.as<double>());
+        launch_stm_final(ctx->stream, pc.nsteps, n, ctx->ad_RVd.as<double>(), ctx->io_c.as<double>());
         ctx->check_launch();
         HIPC(hipEventRecord(e1.e, ctx->stream));
-        HIPC(hipMemcpyAsync(r_out, ctx->io_a.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
-            ctx->stream));
-        HIPC(hipMemcpyAsync(v_out, ctx->io_b.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
-            ctx->stream));
         HIPC(hipMemcpyAsync(stm, ctx->io_c.p, sizeof(double) * 36 * n, hipMemcpyDeviceToHost,
             ctx->stream));
-        ctx->sync();
+        copy_state_out(ctx, pc, r_out, v_out);
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, e0.e, e1.e));
         ctx->ad_timing.stm_ms = ms;

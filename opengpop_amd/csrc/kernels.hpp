@@ -109,8 +109,6 @@ void launch_gravity_jac(hipStream_t st, const GravityDev& g, const double* r, in
     double* a, double* G);
 void launch_tangent_init(hipStream_t st, const double* r0, const double* v0, const int64_t* steps,
     int64_t n, const PropState& ps7, int64_t* ns7);
-void launch_tangent_final(hipStream_t st, const PropState& ps7, int64_t n, double* r_out,
-    double* v_out, double* stm);
 void launch_tangent_split(hipStream_t st, const PropState& ps7, int64_t n, const PropState& pc,
     int64_t* pc_ns, double* Zd, double* RVd);
 void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_checkpoint);
@@ -123,6 +121,12 @@ void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double
     int64_t k1, int ncorr, const double* Gbuf, int R, const double* seeds, double* lam);
 void launch_adjoint_combine(hipStream_t st, const int64_t* nsteps, int64_t n, int R,
     const double* lam, const double* Zd, const double* RVd, const double* seeds, double* out);
+// Forward tangent pass over steps k0 .. k1-1 with the Jacobians recorded for those steps:
+// Zd / RVd (tangent_split layout) are advanced in place.
+void launch_gj8_tangent(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
+    int64_t k1, int ncorr, const double* Gbuf, double* Zd, double* RVd);
+void launch_stm_final(hipStream_t st, const int64_t* nsteps, int64_t n, const double* RVd,
+    double* stm);
 int adjoint_max_corrections();
 
 } // namespace ogp
