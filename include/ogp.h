@@ -253,6 +253,13 @@ int ogp_conj_last_timing(const ogp_context* ctx, ogp_conj_timing* t);
  * the acceleration is bit-identical to ogp_gravity_itrs. */
 int ogp_gravity_jacobian_itrs(ogp_context* ctx, const double* r, int64_t n, double* a, double* G);
 
+/* HBM budget for the gravity Jacobians the derivative passes record (no counterpart in the
+ * reference): 6 doubles per object, corrector iterate and step. Up to the budget the adjoint's
+ * forward pass records the whole span (beyond it, or with checkpoint_steps > 0, it keeps
+ * checkpoints and records each segment again in the reverse sweep), and ogp_stm records and
+ * consumes segments of at most that size. Results do not depend on it. 0 = 16 GiB (default). */
+int ogp_set_derivative_memory(ogp_context* ctx, int64_t bytes);
+
 /* State-transition matrix from the variational equations of the discrete scheme (forward mode):
  * the six tangents are carried through the same RK8 startup and GJ8 steps as the state, with
  * the gravity Jacobian of every corrector iterate evaluated once per object. Also returns the

@@ -150,6 +150,8 @@ struct ogp_context {
     static constexpr int kLaneStreams = 4;
     hipStream_t lane_streams[kLaneStreams] = {};
     int slices_override = 0; // object slices per class; 0 = chosen from the class size
+    // HBM for the gravity Jacobians recorded by the derivative passes (ogp_set_derivative_memory)
+    double record_bytes = 16.0 * (1 << 30);
 
     ~ogp_context()
     {
@@ -1061,6 +1063,16 @@ int ogp_set_slices(ogp_context* ctx, int slices)
     return OGP_OK;
 }
 
+int ogp_set_derivative_memory(ogp_context* ctx, int64_t bytes)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    if (bytes < 0)
+        return fail(ctx, OGP_ERR_ARGUMENT, "bytes must be >= 0 (0 = the default 16 GiB)");
+    ctx->record_bytes = bytes > 0 ? (double)bytes : 16.0 * (1 << 30);
+    return OGP_OK;
+}
+
 int ogp_set_sources(ogp_context* ctx, const ogp_sources* sources)
 {
     if (!ctx)
@@ -1677,10 +1689,10 @@ PropState tangent_startup(ogp_context* ctx, const double* r0, const double* v0, 
     return ps7;
 }
 
-// Recorded gravity Jacobians of a whole span up to this size are kept in HBM and the record pass
-// is the forward pass; beyond it (or with an explicit checkpoint interval) the forward pass keeps
-// checkpoints and every segment is recorded again in the reverse sweep.
-constexpr double kRecordAllBytes = 16.0 * (1 << 30);
+// Recorded gravity Jacobians of a whole span up to ctx->record_bytes are kept in HBM and the
+// record pass is the forward pass; beyond it (or with an explicit checkpoint interval) the
+// forward pass keeps checkpoints and every segment is recorded again in the reverse sweep. The
+// forward-mode STM records and consumes segments of at most that size.
 
 // Startup on the variational layout, then its split into the compact state pc (n objects,
 // ctx->ad_* buffers) and the derivatives of the post-startup state: Zd (s, S, ring) and RVd
@@ -1745,7 +1757,7 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
     const int64_t max_steps = *std::max_element(steps, steps + n);
     const int64_t gj_steps = std::max<int64_t>(max_steps - 4, 0);
     const bool record_all = C == 0
-        && (double)gj_steps * ncorr * 6 * (double)n * sizeof(double) <= kRecordAllBytes;
+        && (double)gj_steps * ncorr * 6 * (double)n * sizeof(double) <= ctx->record_bytes;
     if (record_all)
         C = std::max<int64_t>(gj_steps, 1);
     else if (C == 0)
@@ -1853,7 +1865,7 @@ int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, dou
         const int64_t gj_steps = std::max<int64_t>(max_steps - 4, 0);
         const double per_step = (double)ncorr * 6 * (double)n * sizeof(double);
         const int64_t C = std::max<int64_t>(1,
-            std::min<int64_t>(gj_steps, (int64_t)(kRecordAllBytes / per_step)));
+            std::min<int64_t>(gj_steps, (int64_t)(ctx->record_bytes / per_step)));
         const int W = ctx->waves_for(n);
         ctx->ad_G.reserve(sizeof(double) * C * ncorr * 6 * n, &ctx->dev_bytes);
         for (int64_t k0 = 5; k0 <= max_steps; k0 += C) {

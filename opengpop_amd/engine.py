@@ -167,6 +167,10 @@ class Context:
         """Object slices per step-size class run on concurrent streams (0 = automatic)."""
         check(self.h, lib().ogp_set_slices(self.h, slices))
 
+    def set_derivative_memory(self, nbytes=0):
+        """HBM budget for the Jacobians recorded by the derivative passes (0 = 16 GiB)."""
+        check(self.h, lib().ogp_set_derivative_memory(self.h, int(nbytes)))
+
     def get_tuning(self, n):
         w = ctypes.c_int()
         ch = C._I64()

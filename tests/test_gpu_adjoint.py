@@ -98,6 +98,30 @@ def test_adjoint_uneven_spans_and_checkpoint_interval(ctx20):
     assert _rel(J7, Phi).max() < 1e-10
 
 
+def test_derivative_memory_budget_segments():
+    # the STM records and consumes the Jacobians segment by segment when the span's do not fit
+    # the budget, and the adjoint falls back to checkpoints: the results are unchanged, bit for
+    # bit (the same arithmetic, with the state passing through memory between segments)
+    r0, v0 = scenarios.leo(12, seed=36)
+    steps = np.array([0, 1, 3, 4, 5, 6, 12, 13, 70, 133, 200, 69])
+    with ogp.Context(degree=20) as c:
+        r1, v1, P1 = c.stm(r0, v0, 60.0, steps)
+        c.set_derivative_memory(7 * 7 * 6 * 12 * 8)  # 7 steps of 7 iterates x 6 entries x 12
+        r2, v2, P2 = c.stm(r0, v0, 60.0, steps)
+        _, _, Ja = c.adjoint_jacobian(r0, v0, 60.0, steps)
+        _, _, J64 = c.adjoint_jacobian(r0, v0, 60.0, steps, checkpoint_steps=64)
+        with pytest.raises(ogp.OgpError):
+            c.set_derivative_memory(-1)
+        c.set_derivative_memory(0)
+        _, _, J = c.adjoint_jacobian(r0, v0, 60.0, steps)
+        rp, vp = c.propagate(r0, v0, 60.0, steps)
+    assert np.array_equal(P1, P2) and np.array_equal(r1, r2) and np.array_equal(v1, v2)
+    assert np.array_equal(r1, rp) and np.array_equal(v1, vp)
+    assert np.array_equal(Ja, J64) and np.array_equal(J, J64)
+    assert np.array_equal(P1[0], np.eye(6))
+    assert _rel(P1[1:], J[1:]).max() < 1e-10
+
+
 def _oracle_fd(po, r0, v0, n_steps, hr, hv):
     o = po.Oracle(degree=20)
     fd = np.zeros((len(r0), 6, 6))
