@@ -18,7 +18,8 @@ complete propagation of that population. Side legs (same line):
   cpu       the FP64 host oracle on a stratified 2,500-object sample of the headline workload,
             timed on this host's cores; also the GPU-vs-oracle difference on that sample
   refpoint  10,000 LEO, 30 s, 720 steps, degree 70, end to end from host arrays (the workload
-            of the reference's published H100 figure)
+            of the reference's published H100 figure), and the same through the command-line
+            tool as a whole process (JSON in, JSON out)
   sources   the headline workload with the Sun, the Moon and drag added to gravity (the
             acceleration-source extension point): object-steps/s and cost vs gravity alone
   perturb   the Monte-Carlo population generator: 10^8 satellites written to HBM (satellites/s
@@ -265,11 +266,31 @@ def leg_refpoint(ogp, scenarios):
         ctx.propagate(r0, v0, 30.0, 720)
         t = ctx.timing()
     wall = time.perf_counter() - t0
-    return dict(workload="10,000 LEO, 30 s, 720 steps (6 h), JGM3 70x70, host arrays, "
-                         "context creation included (first use of the wave tuner)",
-                wall_s=wall, object_steps_per_s_end_to_end=7.2e6 / wall,
-                device_ms=t["total_ms"], published_h100_wall_s=18.0,
-                published_h100_object_steps_per_s=3.99e5)
+    out = dict(workload="10,000 LEO, 30 s, 720 steps (6 h), JGM3 70x70, host arrays, "
+                        "context creation included (first use of the wave tuner)",
+               wall_s=wall, object_steps_per_s_end_to_end=7.2e6 / wall,
+               device_ms=t["total_ms"], published_h100_wall_s=18.0,
+               published_h100_object_steps_per_s=3.99e5)
+    # the same workload through the command-line tool, whole process (JSON in, JSON out), as the
+    # published figure is measured
+    import subprocess
+    import tempfile
+    cli = os.path.join(ROOT, "opengpop_amd", "opengpop")
+    out["cli_wall_s"] = None
+    if os.path.exists(cli):
+        with tempfile.TemporaryDirectory() as td:
+            inp, res = os.path.join(td, "in.json"), os.path.join(td, "out.json")
+            with open(inp, "w") as f:
+                json.dump({"propagationStart": {"Year": 2025, "Month": 1, "Day": 1, "Hour": 0,
+                                                "Min": 0, "Sec": 0.0},
+                           "propagationSeconds": 21600, "positions": r0.tolist(),
+                           "velocities": v0.tolist()}, f)
+            t0 = time.perf_counter()
+            subprocess.run([cli, "propagate", "-i", inp, "-o", res, "--data-dir",
+                            os.path.join(ROOT, "data")], check=True, capture_output=True,
+                           timeout=300)
+            out["cli_wall_s"] = time.perf_counter() - t0
+    return out
 
 
 def leg_adjoint(ogp, scenarios, with_oracle):

@@ -8,10 +8,13 @@
 // Extensions (all optional; defaults reproduce the reference): --step, --degree,
 // --corrections, --precision fp64|split, --data-dir, --threshold (closest-approach: report only
 // pairs whose sampled minimum distance is below the threshold, in metres), and the acceleration
-// sources added to gravity: --sun, --moon, --drag C_D*A/m (m^2/kg).
+// sources added to gravity: --sun, --moon, --drag C_D*A/m (m^2/kg); --timing prints the wall
+// time of each phase (read and parse, context, compute, write) to stderr.
 #include "ogp.h"
 
 #include <array>
+#include <charconv>
+#include <chrono>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -254,13 +257,22 @@ std::string format_double(double v)
         return "null";
     if (v == 0)
         return std::signbit(v) ? "-0.0" : "0.0";
+    // The smallest precision whose correctly rounded form reads back as v. No decimal shorter
+    // than the shortest round-trip one (std::to_chars without a precision) reads back, so the
+    // search starts at its digit count and normally ends there.
     char buf[64];
-    int prec = 1;
+    std::to_chars_result r = std::to_chars(buf, buf + sizeof buf - 1, v, std::chars_format::scientific);
+    int prec = 0;
+    for (const char* c = buf; c < r.ptr && *c != 'e'; ++c)
+        prec += *c >= '0' && *c <= '9';
     for (; prec <= 17; ++prec) {
-        std::snprintf(buf, sizeof buf, "%.*e", prec - 1, v);
-        if (std::strtod(buf, nullptr) == v)
+        r = std::to_chars(buf, buf + sizeof buf - 1, v, std::chars_format::scientific, prec - 1);
+        double back = 0.0;
+        std::from_chars(buf, r.ptr, back);
+        if (back == v)
             break;
     }
+    *r.ptr = '\0';
     // buf = [-]d.ddde[+-]XX ; split into digits and decimal exponent
     std::string sbuf(buf);
     bool neg = sbuf[0] == '-';
@@ -345,6 +357,7 @@ struct Options {
     double threshold = 0.0;
     int sources = 0;      // OGP_SOURCE_* bits
     double drag_bc = 0.0; // C_D A / m with --drag
+    bool timing = false;  // --timing: phase wall times on stderr
 };
 
 [[noreturn]] void usage(const char* msg)
@@ -353,7 +366,8 @@ struct Options {
         std::cerr << "error: " << msg << "\n";
     std::cerr << "usage: opengpop {propagate|closest-approach} -i INPUT [-o OUTPUT]\n"
                  "       [--step S] [--degree L] [--corrections K] [--precision fp64|split]\n"
-                 "       [--data-dir DIR] [--threshold METRES] [--sun] [--moon] [--drag CDA_M]\n";
+                 "       [--data-dir DIR] [--threshold METRES] [--sun] [--moon] [--drag CDA_M]\n"
+                 "       [--timing]\n";
     std::exit(2);
 }
 
@@ -455,6 +469,15 @@ void check(ogp_context* ctx, int rc)
 
 int run(const Options& opt)
 {
+    using Clock = std::chrono::steady_clock;
+    Clock::time_point tp = Clock::now();
+    auto phase = [&](const char* name) {
+        const Clock::time_point now = Clock::now();
+        if (opt.timing)
+            std::cerr << "timing: " << name << " "
+                      << std::chrono::duration<double>(now - tp).count() << " s\n";
+        tp = now;
+    };
     std::ifstream f(opt.input);
     if (!f.is_open())
         throw std::runtime_error("cannot open input " + opt.input);
@@ -465,7 +488,9 @@ int run(const Options& opt)
     Json in = ps.value();
     Common c = read_common(in, opt);
     const int64_t n = (int64_t)c.r.size();
+    phase("read and parse");
     std::unique_ptr<ogp_context, void (*)(ogp_context*)> ctx(make_context(opt, c), ogp_destroy);
+    phase("context (device, data sets, tables)");
     Json out = Json::object();
     out.o["propagationStart"] = c.start;
     out.o["propagationSeconds"] = Json::integer(c.seconds);
@@ -516,10 +541,13 @@ int run(const Options& opt)
         }
         out.o["closestApproaches"] = recs;
     }
+    phase("compute");
     std::ofstream o(opt.output);
     if (!o.is_open())
         throw std::runtime_error("cannot write " + opt.output);
     dump(out, o, 0);
+    o.close();
+    phase("write");
     std::cout << "Saved to: " << opt.output << "\n";
     return 0;
 }
@@ -563,6 +591,8 @@ int main(int argc, char** argv)
             opt.data_dir = next();
         else if (a == "--threshold")
             opt.threshold = std::atof(next().c_str());
+        else if (a == "--timing")
+            opt.timing = true;
         else if (a == "--sun")
             opt.sources |= OGP_SOURCE_SUN;
         else if (a == "--moon")
