@@ -6,6 +6,7 @@ CPU tests cover the number formatting and the argument errors; GPU tests run bot
 inputs in the reference's format and check the results against the oracle.
 """
 import json
+import math
 import os
 import random
 import struct
@@ -57,6 +58,16 @@ def test_number_format_matches_nlohmann_rules():
           5e-324, 1.7976931348623157e308, 7612.6081732238688, 0.0025, 3e21, 46800.0, 42.0]
     for _ in range(400):
         xs.append(rng.uniform(-1, 1) * 10 ** rng.uniform(-12, 22))
+    # random bit patterns: every exponent, subnormals, and powers of two with their neighbours
+    # (where the round-trip interval is asymmetric)
+    for _ in range(2000):
+        x = struct.unpack("<d", struct.pack("<Q", rng.getrandbits(64)))[0]
+        if math.isfinite(x):
+            xs.append(x)
+    for e in range(-1074, 1024, 7):
+        p2 = math.ldexp(1.0, e)
+        xs += [p2, math.nextafter(p2, 0.0), math.nextafter(p2, math.inf)]
+    xs = [x for x in xs if math.isfinite(x)]
     out = _run(["--format-double"] + [repr(x) for x in xs])
     assert out.returncode == 0
     got = out.stdout.split()
