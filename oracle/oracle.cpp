@@ -1277,6 +1277,21 @@ struct Hermite {
     }
 };
 
+// ApproachFunc (interpolation.cuh:100-125): f(t) = (A - B)(t) . (A - B)'(t) for two objects
+// interpolated per axis.
+struct Approach {
+    const Hermite *A[3], *B[3];
+    double operator()(double t) const
+    {
+        double dx = A[0]->val(t) - B[0]->val(t), dy = A[1]->val(t) - B[1]->val(t),
+               dz = A[2]->val(t) - B[2]->val(t);
+        double ux = A[0]->der(t) - B[0]->der(t), uy = A[1]->der(t) - B[1]->der(t),
+               uz = A[2]->der(t) - B[2]->der(t);
+        return (dx * ux) + (dy * uy) + (dz * uz);
+    }
+};
+
+// FindRoot (interpolation.cuh:127-152): secant iteration from (x0, x1); a flat secant returns x1.
 template <class F> double find_root(F f, double x0, double x1, double tol = 1e-7, int maxit = 100)
 {
     double f0 = f(x0), f1 = f(x1);
@@ -1614,6 +1629,26 @@ double orc_hermite_eval(const double* c, double dt, double t, int deriv)
     return deriv ? hp.der(t) : hp.val(t);
 }
 
+double orc_find_root_poly(const double* coef, int ncoef, double x0, double x1, double tol)
+{
+    auto f = [&](double x) {
+        double y = 0.0;
+        for (int i = ncoef - 1; i >= 0; --i)
+            y = y * x + coef[i];
+        return y;
+    };
+    return find_root(f, x0, x1, tol);
+}
+
+double orc_approach_func(const double* qa, const double* qb, double dt, double t, double x1,
+    double tol, int root)
+{
+    const Hermite A[3] = { Hermite(qa, dt), Hermite(qa + 9, dt), Hermite(qa + 18, dt) };
+    const Hermite B[3] = { Hermite(qb, dt), Hermite(qb + 9, dt), Hermite(qb + 18, dt) };
+    const Approach f { { &A[0], &A[1], &A[2] }, { &B[0], &B[1], &B[2] } };
+    return root ? find_root(f, t, x1, tol) : f(t);
+}
+
 int orc_screen(const double* traj, int comps, int nobj, int nsteps, const unsigned* set,
     double* mind, int* mink)
 {
@@ -1670,11 +1705,7 @@ int orc_refine(const double* traj, int nobj, int nsteps, double h, unsigned a, u
     };
     Hermite Ax = mk(a, 0), Ay = mk(a, 1), Az = mk(a, 2);
     Hermite Bx = mk(b, 0), By = mk(b, 1), Bz = mk(b, 2);
-    auto f = [&](double t) {
-        double dx = Ax.val(t) - Bx.val(t), dy = Ay.val(t) - By.val(t), dz = Az.val(t) - Bz.val(t);
-        double ux = Ax.der(t) - Bx.der(t), uy = Ay.der(t) - By.der(t), uz = Az.der(t) - Bz.der(t);
-        return (dx * ux) + (dy * uy) + (dz * uz);
-    };
+    const Approach f { { &Ax, &Ay, &Az }, { &Bx, &By, &Bz } };
     double t = find_root(f, 0.0, h);
     out[1] = t + k * h;
     double dx = Ax.val(t) - Bx.val(t), dy = Ay.val(t) - By.val(t), dz = Az.val(t) - Bz.val(t);

@@ -125,6 +125,12 @@ int orc_partition_pair(const unsigned* set_ids, int n_ids, unsigned idx, unsigne
 // HermitePolynomial (interpolation.cuh:27-98): coefficients and value/derivative.
 void orc_hermite(const double* pva9, double dt, double* coeffs9);
 double orc_hermite_eval(const double* coeffs9, double dt, double t, int deriv);
+// FindRoot (interpolation.cuh:127-152) on the polynomial sum_i coef[i] x^i, from (x0, x1).
+double orc_find_root_poly(const double* coef, int ncoef, double x0, double x1, double tol);
+// ApproachFunc (interpolation.cuh:100-125) of objects A and B, each given per axis as 9 Hermite
+// data (p, v, a at 0, dt, 2 dt): f(t) when root == 0, else FindRoot(f, t, x1, tol).
+double orc_approach_func(const double* qa27, const double* qb27, double dt, double t, double x1,
+    double tol, int root);
 // Screening over stored trajectories traj[step][obj][3 or 9] (stride comps):
 // for every cross-set pair p: min distance (first occurrence) and its step.
 int orc_screen(const double* traj, int comps, int n_obj, int n_steps, const unsigned* set_ids,

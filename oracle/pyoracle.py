@@ -80,6 +80,10 @@ class Oracle:
         L.orc_hermite.argtypes = [_P, _D, _P]
         L.orc_hermite_eval.restype = _D
         L.orc_hermite_eval.argtypes = [_P, _D, _D, _I]
+        L.orc_find_root_poly.restype = _D
+        L.orc_find_root_poly.argtypes = [_P, _I, _D, _D, _D]
+        L.orc_approach_func.restype = _D
+        L.orc_approach_func.argtypes = [_P, _P, _D, _D, _D, _D, _I]
         L.orc_screen.argtypes = [_P, _I, _I, _I, _P, _P, _P]
         L.orc_refine.argtypes = [_P, _I, _I, _D, ctypes.c_uint, ctypes.c_uint, _I, _D, _P]
         self.loaded = False
@@ -337,6 +341,20 @@ class Oracle:
                              _ptr(mink)) != 0:
             raise RuntimeError("invalid set ids")
         return mind, mink
+
+    def find_root_poly(self, coef, x0, x1, tol=1e-7):
+        c = np.ascontiguousarray(coef, dtype=np.float64)
+        return self.L.orc_find_root_poly(_ptr(c), len(c), x0, x1, tol)
+
+    def approach_func(self, qa, qb, dt, t):
+        a = np.ascontiguousarray(qa, dtype=np.float64).reshape(27)
+        b = np.ascontiguousarray(qb, dtype=np.float64).reshape(27)
+        return self.L.orc_approach_func(_ptr(a), _ptr(b), dt, t, 0.0, 0.0, 0)
+
+    def approach_root(self, qa, qb, dt, x0, x1, tol=1e-7):
+        a = np.ascontiguousarray(qa, dtype=np.float64).reshape(27)
+        b = np.ascontiguousarray(qb, dtype=np.float64).reshape(27)
+        return self.L.orc_approach_func(_ptr(a), _ptr(b), dt, x0, x1, tol, 1)
 
     def refine(self, traj, h, a, b, k, min_dist):
         traj = np.ascontiguousarray(traj, dtype=np.float64)

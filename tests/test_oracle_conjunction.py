@@ -63,6 +63,33 @@ def test_hermite_coefficients_kat(oracle):
     assert oracle.hermite_eval(c, 1.0, 0.5, deriv=True) == -0.25
 
 
+def test_find_root_kats(oracle):
+    # closest_approach_utils_test.cu:271-296: the secant from (x0, x1)
+    assert abs(oracle.find_root_poly([-6.0, 3.0], 0.0, 1.0) - 2.0) < 1e-7
+    assert abs(oracle.find_root_poly([-9.0, 0.0, 1.0], 1.0, 5.0) - 3.0) < 1e-7
+    assert abs(oracle.find_root_poly([-9.0, 0.0, 1.0], -1.0, -5.0) + 3.0) < 1e-7
+    # f(-2) == f(2): a flat secant returns x1 rather than dividing by zero
+    assert oracle.find_root_poly([0.0, 0.0, 1.0], -2.0, 2.0) == 2.0
+
+
+# closest_approach_utils_test.cu:298-347: A(t) = (t, 2, 0), B(t) = (t, t^2/2 - 3t, 0), given as
+# Hermite data (p, v, a) at t = 0, 1, 2
+_QA = [[0, 1, 0, 1, 1, 0, 2, 1, 0], [2, 0, 0, 2, 0, 0, 2, 0, 0], [0] * 9]
+_QB = [[0, 1, 0, 1, 1, 0, 2, 1, 0], [0, -3, 1, -2.5, -2, 1, -4, -1, 1], [0] * 9]
+
+
+def test_approach_function_kat(oracle):
+    # f(t) = (A - B)(t) . (A - B)'(t) = (-t^2/2 + 3t + 2)(3 - t)
+    for t, want in ((0.0, 6.0), (1.0, 9.0), (2.0, 6.0), (3.0, 0.0), (4.0, -6.0)):
+        got = oracle.approach_func(_QA, _QB, 1.0, t)
+        assert abs(got - want) <= 4 * np.spacing(max(abs(want), 1.0))
+
+
+def test_root_of_approach_function(oracle):
+    tol = 1e-12
+    assert abs(oracle.approach_root(_QA, _QB, 1.0, 2.0, 10.0, tol) - 3.0) < tol
+
+
 def _run_phases(oracle, po, case):
     """Multi-phase ClosestApproach::Propagate: every phase starts a new startup from the
     current state and appends its samples; screening and refinement see all samples."""
