@@ -419,8 +419,9 @@ def _cpu_model():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (objects shard "
-              "across GPUs with no exchange; see the design notes)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported. Propagation "
+              "shards by objects with no exchange between GPUs, so N GPUs would run N times "
+              "this line's per-GPU work.", file=sys.stderr)
         sys.exit(2)
     legs = set(args.legs.split(","))
     import torch

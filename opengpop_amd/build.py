@@ -17,7 +17,8 @@ ARCH = os.environ.get("OGP_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", f"--offload-arch={ARCH}",
-          # packed FP32 instructions stay off (numerical hygiene)
+          # no packed FP32 instructions (v_pk_fma_f32 and kin): some of their forms have given
+          # wrong results on this hardware without an error
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(ROOT, "include")]
 
