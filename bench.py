@@ -8,15 +8,16 @@ Prints ONE JSON line. The headline `value` is FP64 object-steps/s of the mixed-r
 degree/order 70, inputs resident in HBM (ogp_propagate_device), startup included. One step = one
 complete propagation of that population. Side legs (same line):
   split     50,000 LEO, 60 s, 7 days, degree 70: FP64 and split-precision object-steps/s and
-            the split path's final-position error (against the FP64 GPU path for all objects,
-            and against the oracle for a sample)
+            the split path's final-position error against the FP64 GPU path for all objects
   pairs     20,000 objects with 1,000 forced encounters, set 0 = first 2,000, 3 days at 60 s,
             degree 20: propagation, screen (10 km) and refinement; pair-steps/s of the screen
   adjoint   10,000 LEO, 60 s, 24 h, degree 20: the 6x6 d(final)/d(initial) per object by the
             discrete adjoint and by the variational equations; agreement and cost vs the forward
-            propagation (and central differences of the oracle for two objects)
+            propagation
   cpu       the FP64 host oracle on a stratified 2,500-object sample of the headline workload,
-            timed on this host's cores; also the GPU-vs-oracle difference on that sample
+            timed on this host's cores (cpu_baseline). Run last, it is also the checker: the
+            GPU-vs-oracle difference on that sample, on 32 objects of the split workload, and
+            the adjoint against central differences of the oracle for two objects
   refpoint  10,000 LEO, 30 s, 720 steps, degree 70, end to end from host arrays (the workload
             of the reference's published H100 figure), and the same through the command-line
             tool as a whole process (JSON in, JSON out)
@@ -155,7 +156,7 @@ def leg_fp64(ogp, scenarios, torch, args):
     return out
 
 
-def leg_split(ogp, scenarios, args, oracle_threads):
+def leg_split(ogp, scenarios, args):
     c2 = scenarios.config2()
     n = len(c2["r"])
     res = {}
@@ -192,18 +193,11 @@ def leg_split(ogp, scenarios, args, oracle_threads):
                                  fp64_flops_per_object_step=f64,
                                  elapsed_ms_per_launch=t_launch * 1e3,
                                  avg_launch_ms=s["launch_ms_sum"] / s["launches"])
-    if oracle_threads:
-        from oracle import pyoracle as po
-        o = po.Oracle(degree=70)
-        k = np.arange(0, n, n // 32)[:32]
-        t0 = time.perf_counter()
-        ro, _ = o.propagate(po.GRAVITY, c2["r"][k], c2["v"][k], c2["step"][k], c2["steps"][k],
-                            threads=oracle_threads)
-        log(f"split leg oracle sample: {time.perf_counter() - t0:.1f} s")
-        e_s = np.linalg.norm(res["split"]["r"][k] - ro, axis=1)
-        e_d = np.linalg.norm(res["fp64"]["r"][k] - ro, axis=1)
-        out["vs_oracle_sample_m"] = dict(objects=len(k), split_max=float(e_s.max()),
-                                         fp64_max=float(e_d.max()))
+    # 32 objects for the cpu leg's check against the oracle
+    k = np.arange(0, n, n // 32)[:32]
+    out["_oracle_sample"] = dict(r0=c2["r"][k], v0=c2["v"][k], step=c2["step"][k],
+                                 steps=c2["steps"][k], split_r=res["split"]["r"][k],
+                                 fp64_r=res["fp64"]["r"][k])
     return out
 
 
@@ -240,7 +234,8 @@ def leg_pairs(ogp, scenarios):
 
 
 def leg_cpu(sample, threads):
-    """FP64 oracle on every 40th object of the headline workload (stratified over regimes)."""
+    """FP64 oracle on every 40th object of the headline workload (stratified over regimes), the
+    cpu_baseline; returns it with the GPU-vs-oracle difference on that sample."""
     from oracle import pyoracle as po
     c1, idx = sample["c1"], sample["idx"]
     o = po.Oracle(degree=70)
@@ -257,6 +252,38 @@ def leg_cpu(sample, threads):
                        f"{int(work)} object-steps, FP64 oracle, OpenMP over objects, {dt:.1f} s",
                 cpu_model=_cpu_model()), dict(objects=len(idx), max_dr_m=float(dr.max()),
                                              max_dv_mps=float(dv.max()))
+
+
+def oracle_checks(split_sample, adjoint_sample, threads):
+    """Checks of the split and adjoint legs against the oracle, run within the cpu leg: the
+    final positions of 32 objects of the split workload, and central differences of the
+    oracle's propagation for two objects of the adjoint workload."""
+    from oracle import pyoracle as po
+    out = {}
+    if split_sample is not None:
+        s = split_sample
+        t0 = time.perf_counter()
+        ro, _ = po.Oracle(degree=70).propagate(po.GRAVITY, s["r0"], s["v0"], s["step"], s["steps"],
+                                               threads=threads)
+        log(f"split oracle sample: {time.perf_counter() - t0:.1f} s")
+        out["split"] = dict(objects=len(ro),
+                            split_max=float(np.linalg.norm(s["split_r"] - ro, axis=1).max()),
+                            fp64_max=float(np.linalg.norm(s["fp64_r"] - ro, axis=1).max()))
+    if adjoint_sample is not None:
+        a = adjoint_sample
+        o = po.Oracle(degree=20)
+        n = len(a["r0"])
+        fd = np.zeros((n, 6, 6))
+        x0 = np.hstack([a["r0"], a["v0"]])
+        for col in range(6):
+            d = np.zeros((n, 6))
+            d[:, col] = 1e-3 if col < 3 else 1e-6
+            rp, vp = o.propagate(po.GRAVITY, (x0 + d)[:, :3], (x0 + d)[:, 3:], a["h"], a["steps"])
+            rm, vm = o.propagate(po.GRAVITY, (x0 - d)[:, :3], (x0 - d)[:, 3:], a["h"], a["steps"])
+            fd[:, :, col] = np.hstack([rp - rm, vp - vm]) / (2 * d[0, col])
+        e = np.linalg.norm(a["J"] - fd, axis=(1, 2)) / np.linalg.norm(fd, axis=(1, 2))
+        out["adjoint"] = dict(objects=n, max=float(e.max()))
+    return out
 
 
 def leg_refpoint(ogp, scenarios):
@@ -293,7 +320,7 @@ def leg_refpoint(ogp, scenarios):
     return out
 
 
-def leg_adjoint(ogp, scenarios, with_oracle):
+def leg_adjoint(ogp, scenarios):
     """Differentiation workload: 10,000 LEO, 60 s, 1,440 steps (24 h), JGM3 20x20: the 6x6
     d(final)/d(initial) per object by the adjoint (the forward pass recording the gravity
     Jacobians of the whole span, 4.8 GB; and with checkpoints every 64 steps) and by the
@@ -327,20 +354,9 @@ def leg_adjoint(ogp, scenarios, with_oracle):
                vjp_cost_vs_forward=(tv["forward_ms"] + tv["adjoint_ms"]) / fwd_ms,
                stm_cost_vs_forward=stm_ms / fwd_ms,
                adjoint_vs_stm_rel=dict(max=float(rel.max()), median=float(np.median(rel))))
-    if with_oracle:
-        from oracle import pyoracle as po
-        o = po.Oracle(degree=20)
-        k = [0, 5000]
-        fd = np.zeros((len(k), 6, 6))
-        x0 = np.hstack([r0[k], v0[k]])
-        for col in range(6):
-            d = np.zeros((len(k), 6))
-            d[:, col] = 1e-3 if col < 3 else 1e-6
-            rp, vp = o.propagate(po.GRAVITY, (x0 + d)[:, :3], (x0 + d)[:, 3:], h, steps[k])
-            rm, vm = o.propagate(po.GRAVITY, (x0 - d)[:, :3], (x0 - d)[:, 3:], h, steps[k])
-            fd[:, :, col] = np.hstack([rp - rm, vp - vm]) / (2 * d[0, col])
-        e = np.linalg.norm(J[k] - fd, axis=(1, 2)) / np.linalg.norm(fd, axis=(1, 2))
-        out["adjoint_vs_oracle_central_differences_rel"] = dict(objects=len(k), max=float(e.max()))
+    # two objects for the cpu leg's central differences of the oracle's propagation
+    k = [0, 5000]
+    out["_oracle_sample"] = dict(r0=r0[k], v0=v0[k], h=h, steps=steps[k], J=J[k])
     return out
 
 
@@ -446,16 +462,16 @@ def main():
         line["config"]["object_steps"] = f["object_steps"]
         line["device"] = f["device"]
         line["wave_tuning"] = f["tuning"]
+    split_sample = adjoint_sample = None
     if "split" in legs:
-        line["split"] = leg_split(ogp, scenarios, args, threads if "cpu" in legs else 0)
+        line["split"] = leg_split(ogp, scenarios, args)
+        split_sample = line["split"].pop("_oracle_sample")
     if "pairs" in legs:
         line["pairs"] = leg_pairs(ogp, scenarios)
         log(f"pairs leg: {line['pairs']['pair_steps_per_s']:.3e} pair-steps/s")
-    if "cpu" in legs and sample is not None:
-        line["cpu_baseline"], line["parity_sample"] = leg_cpu(sample, threads)
-        log(f"cpu leg: {line['cpu_baseline']['value']:.3e} object-steps/s on {threads} threads")
     if "adjoint" in legs:
-        line["adjoint"] = leg_adjoint(ogp, scenarios, "cpu" in legs)
+        line["adjoint"] = leg_adjoint(ogp, scenarios)
+        adjoint_sample = line["adjoint"].pop("_oracle_sample")
         log(f"adjoint leg: jacobian {line['adjoint']['jacobian_cost_vs_forward']:.1f}x forward")
     if "refpoint" in legs:
         line["reference_point"] = leg_refpoint(ogp, scenarios)
@@ -468,6 +484,17 @@ def main():
         log(f"perturb leg: {line['perturb']['satellites_per_s']:.3e} satellites/s")
     if "copy" in legs:
         line["copy_calibration"] = leg_copy(torch)
+    # the cpu leg, last: the oracle timed on the host (cpu_baseline) and, as the checker, the
+    # GPU results of the other legs against it
+    if "cpu" in legs:
+        if sample is not None:
+            line["cpu_baseline"], line["parity_sample"] = leg_cpu(sample, threads)
+            log(f"cpu leg: {line['cpu_baseline']['value']:.3e} object-steps/s on {threads} threads")
+        checks = oracle_checks(split_sample, adjoint_sample, threads)
+        if "split" in checks:
+            line["split"]["vs_oracle_sample_m"] = checks["split"]
+        if "adjoint" in checks:
+            line["adjoint"]["adjoint_vs_oracle_central_differences_rel"] = checks["adjoint"]
     print(json.dumps(line), flush=True)
 
 
