@@ -209,6 +209,20 @@ def test_identical_objects_bit_identical(ctx70):
     assert all(np.array_equal(r[0], r[i]) and np.array_equal(v[0], v[i]) for i in range(8))
 
 
+def test_million_objects_match_subsets():
+    # 1,048,600 objects (16,385 groups of 64, the last one partial): state offsets, grids and
+    # tuning buckets at this size; an object's result does not depend on the batch it runs in
+    n = 1_048_600
+    r0, v0 = scenarios.leo(n, seed=41)
+    with ogp.Context(degree=20) as c:
+        r, v = c.propagate(r0, v0, 60.0, 12)
+        head = c.propagate(r0[:640], v0[:640], 60.0, 12)
+        tail = c.propagate(r0[-600:], v0[-600:], 60.0, 12)
+    assert np.isfinite(r).all() and np.isfinite(v).all()
+    assert np.array_equal(r[:640], head[0]) and np.array_equal(v[:640], head[1])
+    assert np.array_equal(r[-600:], tail[0]) and np.array_equal(v[-600:], tail[1])
+
+
 @pytest.mark.parametrize("precision", ["fp64", "split"])
 def test_launch_shape_does_not_change_results(precision):
     # 300 objects = four full 64-object groups and a partial one; 140 steps = three chunks
