@@ -2,8 +2,9 @@
  *
  * One context = one device (device 0) + its HIP streams (one for the calls, four for the
  * concurrent object slices of a propagation, all at the default priority) + the loaded
- * datasets. A context is not thread-safe; separate contexts are independent. Every call is synchronous at return, like
- * the reference (which synchronises after nearly every launch). Errors never exit or throw
+ * datasets. A context is not thread-safe; separate contexts are independent. Every call is
+ * synchronous at return, like the reference (which synchronises after nearly every launch).
+ * Errors never exit or throw
  * across this boundary: every entry point returns OGP_OK or an error code, and
  * ogp_last_error(ctx) returns the message (ogp_last_error(NULL) the last context-less one).
  *
