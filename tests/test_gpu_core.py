@@ -181,6 +181,17 @@ def test_24h_degree70_matches_oracle(ctx70, oracle70):
     assert np.abs(v - vo).max() < 1e-6
 
 
+def test_mixed_regimes_degree70_match_oracle(ctx70, oracle70):
+    # the headline workload's shape at its parity bar: 256 objects, 80% LEO at 30 s and 20%
+    # MEO/GEO at 300 s, spans U(6, 24) h, degree 70; about 4 x 10^5 object-steps of the oracle
+    # on 16 threads
+    d = scenarios.mixed(256, seed=17)
+    r, v = ctx70.propagate(d["r"], d["v"], d["step"], d["steps"])
+    ro, vo = oracle70.propagate(0, d["r"], d["v"], d["step"], d["steps"], threads=16)
+    assert np.abs(r - ro).max() < 1e-3
+    assert np.abs(v - vo).max() < 1e-6
+
+
 def test_mixed_steps_and_uneven_spans_match_oracle(ctx20, po):
     d = scenarios.mixed(40, seed=5)
     steps = np.minimum(d["steps"], 300)  # keep the oracle run short
