@@ -80,6 +80,16 @@ def test_split_precision_position_error_two_days():
     assert e.max() < 0.25
 
 
+def test_split_precision_seven_days_within_a_metre_of_oracle(oracle70):
+    # the split-precision bar: 7 days of LEO at 60 s (10,080 steps), degree 70, final
+    # positions within 1 m of the FP64 oracle
+    r0, v0 = scenarios.leo(64, seed=2031)
+    with ogp.Context(degree=70, precision="split") as c:
+        r, _ = c.propagate(r0, v0, 60.0, 10080)
+    ro, _ = oracle70.propagate(0, r0, v0, 60.0, 10080, threads=16)
+    assert np.linalg.norm(r - ro, axis=1).max() < 1.0
+
+
 def test_time_nodes_match_oracle(ctx70, oracle70):
     offs = np.concatenate([np.arange(0, 86400.0 * 3, 977.0), [1e-3, 59.999]])
     M, mjd = ctx70.time_nodes(60.0, 4, offs)
