@@ -5,7 +5,8 @@
 // pair). Here the trajectory of every object is stored once ([step][component][object]) and a
 // pair tile (64 x 64 objects) walks the whole time axis with its running minima in registers;
 // object positions are staged through LDS 8 steps at a time. Per pair the state is the smallest
-// squared distance and the last 8-step stage in which its square root dropped.
+// squared distance and the last decision window (of one or more stages) in which its square
+// root dropped.
 //
 // Decision rule, identical to the reference's "dist < closest" on sqrt'ed distances with the
 // first occurrence kept; the squared distance is formed as fma(dz, dz, fma(dy, dy, dx * dx))
