@@ -153,3 +153,21 @@ def test_source_errors(ctx20):
     with ogp.Context(force=ogp.FORCE_NONE) as c:
         with pytest.raises(ogp.OgpError):
             c.set_sources(ogp.SOURCE_SUN, 0.0)
+
+
+def test_conjunction_trajectory_is_the_propagation_with_sources():
+    # the conjunction search stores the states the propagation produces, sources included
+    # (its own copy of the context's sources and drag coefficients): after one phase of 40
+    # steps every stored row equals a propagation of that many steps, bit for bit
+    r0, v0 = scenarios.leo(96, seed=51)
+    sets = np.repeat(np.arange(2, dtype=np.uint32), [32, 64])
+    with ogp.Context(degree=20) as c:
+        c.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON | ogp.SOURCE_DRAG, drag_bc=0.02)
+        ca = ogp.ClosestApproach(c, 60.0)
+        ca.initialize(r0, v0, sets)
+        ca.propagate(40)
+        traj = ca.trajectory()
+        ends = {k: c.propagate(r0, v0, 60.0, k) for k in (1, 4, 5, 23, 40)}
+    assert traj.shape[:2] == (40, 96)
+    for k, (r, v) in ends.items():
+        assert np.array_equal(traj[k - 1, :, :3], r) and np.array_equal(traj[k - 1, :, 3:6], v), k
