@@ -155,13 +155,14 @@ def test_source_errors(ctx20):
             c.set_sources(ogp.SOURCE_SUN, 0.0)
 
 
-def test_conjunction_trajectory_is_the_propagation_with_sources():
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_conjunction_trajectory_is_the_propagation_with_sources(precision):
     # the conjunction search stores the states the propagation produces, sources included
     # (its own copy of the context's sources and drag coefficients): after one phase of 40
     # steps every stored row equals a propagation of that many steps, bit for bit
     r0, v0 = scenarios.leo(96, seed=51)
     sets = np.repeat(np.arange(2, dtype=np.uint32), [32, 64])
-    with ogp.Context(degree=20) as c:
+    with ogp.Context(degree=20, precision=precision) as c:
         c.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON | ogp.SOURCE_DRAG, drag_bc=0.02)
         ca = ogp.ClosestApproach(c, 60.0)
         ca.initialize(r0, v0, sets)
