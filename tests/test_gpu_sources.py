@@ -172,3 +172,18 @@ def test_conjunction_trajectory_is_the_propagation_with_sources(precision):
     assert traj.shape[:2] == (40, 96)
     for k, (r, v) in ends.items():
         assert np.array_equal(traj[k - 1, :, :3], r) and np.array_equal(traj[k - 1, :, 3:6], v), k
+
+
+def test_stepwise_with_sources_equals_bulk():
+    # the stepwise propagator (its own copy of the drag coefficients) gives, after every step,
+    # the states of a bulk propagation of that many steps, with all sources and per-object drag
+    r0, v0 = scenarios.leo(40, seed=52)
+    bc = np.linspace(0.0, 0.03, 40)
+    with ogp.Context(degree=20) as c:
+        c.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON | ogp.SOURCE_DRAG, 0.01)
+        c.set_drag_coefficients(bc)
+        p = ogp.GpuPropagator(c, r0, v0, 60.0, 12)
+        steps = [p.step()[:2] for _ in range(12)]
+        for k in (1, 4, 5, 12):
+            r, v = c.propagate(r0, v0, 60.0, k)
+            assert np.array_equal(steps[k - 1][0], r) and np.array_equal(steps[k - 1][1], v), k
