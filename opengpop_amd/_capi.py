@@ -69,6 +69,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise OSError(f"{LIB_PATH} is missing: build it with python -m opengpop_amd.build")
+    # PyTorch ships its own libamdhip64.so (soname libamdhip64.so.7) and loads it by a path of
+    # its own. Loaded first, it satisfies libogp.so's libamdhip64.so.7 as well and the process
+    # has one HIP runtime; loaded after libogp.so, it brings a second runtime that sees no
+    # device, and device tensors passed to the *_device entry points could not be made.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     sig = {
         "ogp_default_config": (None, [ctypes.POINTER(OgpConfig)]),

@@ -318,3 +318,19 @@ def test_errors_are_reported_not_fatal(ctx20):
     # the context is still usable afterwards
     r, _ = ctx20.propagate([[7e6, 0, 0]], [[0, 7.5e3, 0]], 60.0, 5)
     assert np.all(np.isfinite(r))
+
+
+def test_device_resident_equals_host_arrays(ctx20):
+    # ogp_propagate_device (SoA device arrays, the headline's path) and ogp_propagate (AoS
+    # host arrays) give the same bits for a mixed population with per-object steps and spans
+    torch = pytest.importorskip("torch")
+    d = scenarios.mixed(300, seed=23)
+    steps = np.minimum(d["steps"], 150)
+    r, v = ctx20.propagate(d["r"], d["v"], d["step"], steps)
+    dev = torch.device("cuda")
+    R0 = torch.tensor(d["r"].T.copy(), device=dev)
+    V0 = torch.tensor(d["v"].T.copy(), device=dev)
+    Ro, Vo = torch.empty_like(R0), torch.empty_like(V0)
+    torch.cuda.synchronize()
+    ctx20.propagate_device(R0, V0, len(steps), d["step"], steps, Ro, Vo)
+    assert np.array_equal(Ro.cpu().numpy().T, r) and np.array_equal(Vo.cpu().numpy().T, v)
