@@ -587,11 +587,15 @@ template <class F> int guarded(ogp_context* ctx, F&& fn)
     }
 }
 
+// Copies a host table to the device and waits: callers pass temporaries that are destroyed
+// right after the call.
 template <typename T> void upload(ogp_context* c, DevBuf& b, const std::vector<T>& v)
 {
     b.reserve(sizeof(T) * std::max<size_t>(v.size(), 1), &c->dev_bytes);
-    if (!v.empty())
+    if (!v.empty()) {
         HIPC(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
+        c->sync();
+    }
 }
 
 void check_finite(const double* p, int64_t n, const char* what)
