@@ -404,6 +404,65 @@ __device__ __forceinline__ void gravity_gcrs(const double* Mg, const double* r,
         a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
 }
 
+// Lead/helper form of gravity_gcrs for a workgroup of W > 1 waves in which only wave 0 (the
+// lead) integrates. The lead rotates r into the ITRS and leaves the rotated position in pos
+// (3 x 64 doubles) before the first barrier; every wave then sums its parts at that position
+// (gravity_helper for waves 1 .. W-1) and after the second barrier the lead alone folds the
+// parts, assembles and rotates back. The arithmetic is that of gravity_gcrs<SPLIT, W>, so the
+// result is bit-identical to it. Barrier hazards: pos is rewritten only after the second
+// barrier, behind the helpers' reads; red is rewritten only after the next first barrier, behind
+// the lead's fold.
+template <bool SPLIT, int W>
+__device__ __forceinline__ void gravity_gcrs_lead(const double* Mg, const double* r,
+    const GravityDev& g, double* red, double* pos, double* a)
+{
+    static_assert(W > 1 && kParts % W == 0, "waves must divide the part count");
+    const uniform_dptr M = (uniform_dptr)Mg;
+    const int lane = threadIdx.x % 64;
+    double ri[3], gi[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
+        pos[i * 64 + lane] = ri[i];
+    }
+    const GravPoint p = grav_point(ri[0], ri[1], ri[2], g.radius);
+    __syncthreads();
+    grav_parts<SPLIT>(p, g, 0, kParts / W, [&](int q, const GravSums& s) {
+        red[(q * 4 + 0) * 64 + lane] = s.G;
+        red[(q * 4 + 1) * 64 + lane] = s.H;
+        red[(q * 4 + 2) * 64 + lane] = s.J;
+        red[(q * 4 + 3) * 64 + lane] = s.K;
+    });
+    __syncthreads();
+    PartFold f;
+#pragma unroll
+    for (int q = 0; q < kParts; ++q)
+        f(q, GravSums { red[(q * 4 + 0) * 64 + lane], red[(q * 4 + 1) * 64 + lane],
+                 red[(q * 4 + 2) * 64 + lane], red[(q * 4 + 3) * 64 + lane] });
+    grav_assemble(ri[0], ri[1], ri[2], p, f.t, g.gm, gi[0], gi[1], gi[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
+}
+
+// The part of one gravity_gcrs_lead evaluation that wave w (1 <= w < W) runs.
+template <bool SPLIT, int W>
+__device__ __forceinline__ void gravity_helper(const GravityDev& g, double* red,
+    const double* pos, int w)
+{
+    const int lane = threadIdx.x % 64;
+    __syncthreads();
+    const GravPoint p = grav_point(pos[lane], pos[64 + lane], pos[128 + lane], g.radius);
+    grav_parts<SPLIT>(p, g, w * (kParts / W), (w + 1) * (kParts / W),
+        [&](int q, const GravSums& s) {
+            red[(q * 4 + 0) * 64 + lane] = s.G;
+            red[(q * 4 + 1) * 64 + lane] = s.H;
+            red[(q * 4 + 2) * 64 + lane] = s.J;
+            red[(q * 4 + 3) * 64 + lane] = s.K;
+        });
+    __syncthreads();
+}
+
 // Acceleration and Jacobian in the GCRS: a = M g(M^T r), G = M Gi M^T (Gi the ITRS Jacobian).
 // The acceleration is bit-identical to gravity_gcrs<false>.
 template <int W = 1>
@@ -646,6 +705,17 @@ __device__ __forceinline__ void force_prop(const ForceCtx& f, const double* x, c
         const double t = (double)(node - 4) * f.h;
         analytic_force<KIND>(x, v, t, a);
     }
+}
+
+// force_prop for the lead wave of a lead/helper workgroup (gravity kinds, W > 1); bit-identical
+// to force_prop<KIND, W>.
+template <int KIND, int W>
+__device__ __forceinline__ void force_prop_lead(const ForceCtx& f, const double* x,
+    const double* v, int64_t node, double* a, double* red, double* pos)
+{
+    static_assert(KIND == kGravity || KIND == kGravitySplit, "gravity kinds only");
+    gravity_gcrs_lead<KIND == kGravitySplit, W>(f.prop_M + 9 * node, x, f.grav, red, pos, a);
+    add_sources(f.src, f.prop_tb + 6 * node, x, v, a);
 }
 
 } // namespace ogp

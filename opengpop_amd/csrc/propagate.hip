@@ -2,10 +2,13 @@
 //
 // Layout: a workgroup of W waves (W = 1, 2, 4 or 8) holds one group of 64 objects; lane l of
 // every wave owns object 64 * blockIdx.x + l. The spherical-harmonic columns of each force
-// evaluation are divided between the W waves and their partial sums meet in LDS
-// (device.hpp, gravity_itrs_split); everything else - the integrator arithmetic, its global
-// loads and stores - runs redundantly and bit-identically in all W waves, so no wave ever reads
-// a value written by another wave through global memory.
+// evaluation are divided between the W waves and their partial sums meet in LDS.
+//   * GJ8 steps (gj8_chunk_kernel): wave 0 alone runs the integrator, its global loads and
+//     stores, the fold of the parts and the corrector; it hands the rotated position to the
+//     other waves through LDS, and they only sum their parts (device.hpp, gravity_gcrs_lead).
+//   * Startup (startup_kernel): every wave runs the RK8 and startup-corrector arithmetic
+//     redundantly and bit-identically (device.hpp, gravity_itrs_split), so no wave reads a
+//     value written by another wave through global memory.
 #include "gj8.hpp"
 #include "kernels.hpp"
 
@@ -220,6 +223,7 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     int64_t traj_steps)
 {
     __shared__ double red[ExchangeArea<W>::size];
+    __shared__ double pos[W > 1 ? 3 * 64 : 1]; // the lead's ITRS positions (gravity_gcrs_lead)
     Exchange ex { red };
     const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
     if (t >= nobj)
@@ -230,6 +234,21 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     const int64_t ns = ps.nsteps[i];
     if (ns < k0)
         return;
+    if constexpr (W > 1) {
+        // Waves 1 .. W-1 only sum their harmonic parts; wave 0 integrates. Every wave of the
+        // group holds the same objects in its lanes, so all of them take the same iterations
+        // below and pass the same barriers (two per evaluation).
+        const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+        if (w > 0) {
+            for (int64_t k = k0; k < k1; ++k) {
+                if (k > ns)
+                    continue;
+                for (int it = 0; it < ncorr; ++it)
+                    gravity_helper<KIND == kGravitySplit, W>(f.grav, red, pos, w);
+            }
+            return;
+        }
+    }
     const double h = f.h, hh = h * h;
     double r[3], v[3], s[3], S[3];
 #pragma unroll
@@ -248,7 +267,10 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
         gj8_predict(q, h, s, S, r, v, sprev);
         // P E (E C)^ncorr: the predictor's evaluation is the first corrector evaluation.
         for (int it = 0; it < ncorr; ++it) {
-            force_prop<KIND, W>(f, r, v, k + 4, anew, ex);
+            if constexpr (W > 1)
+                force_prop_lead<KIND, W>(f, r, v, k + 4, anew, red, pos);
+            else
+                force_prop<KIND, W>(f, r, v, k + 4, anew, ex);
             gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         }
         const int wslot = ring_slot(k);
