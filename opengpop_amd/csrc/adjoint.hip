@@ -215,6 +215,7 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
     int64_t k1, int ncorr, double* __restrict__ Gbuf)
 {
     __shared__ double red[W > 1 ? kJacExchange : 1];
+    __shared__ double pos[W > 1 ? 3 * 64 : 1]; // the lead's ITRS positions
     const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
     const int64_t n = ps.N;
     if (i >= n)
@@ -222,10 +223,22 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
     const int64_t ns = ps.nsteps[i];
     if (ns < k0)
         return;
-    // the W waves of a workgroup hold the same 64 objects and share the harmonic columns; each
-    // keeps the whole integrator state and writes its own ring slots (which it reads back), and
-    // wave 0 writes the Jacobians and the state at the end
-    const bool writer = threadIdx.x < 64;
+    if constexpr (W > 1) {
+        // The W waves of a workgroup hold the same 64 objects and share the harmonic columns.
+        // Waves 1 .. W-1 only sum their Dual3 parts (gravity_jac_helper); wave 0 integrates,
+        // records the Jacobians and writes the ring and the state. All waves take the same
+        // iterations, so they pass the same barriers (two per evaluation).
+        const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+        if (w > 0) {
+            for (int64_t k = k0; k < k1; ++k) {
+                if (k > ns)
+                    break;
+                for (int it = 0; it < ncorr; ++it)
+                    gravity_jac_helper<W>(f.grav, red, pos, w);
+            }
+            return;
+        }
+    }
     const double h = f.h, hh = h * h;
     double r[3], v[3], s[3], S[3];
     for (int c = 0; c < 3; ++c) {
@@ -243,11 +256,13 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         gj8_predict(q, h, s, S, r, v, sprev);
         for (int it = 0; it < ncorr; ++it) {
             double G[3][3];
-            gravity_gcrs_jac<W>(f.prop_M + 9 * (k + 4), r, f.grav, anew, G, red);
+            if constexpr (W > 1)
+                gravity_gcrs_jac_lead<W>(f.prop_M + 9 * (k + 4), r, f.grav, red, pos, anew, G);
+            else
+                gravity_gcrs_jac<W>(f.prop_M + 9 * (k + 4), r, f.grav, anew, G, red);
             const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
-            if (writer)
-                for (int e = 0; e < 6; ++e)
-                    Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
+            for (int e = 0; e < 6; ++e)
+                Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
             gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         }
         const int wslot = ring_slot(k);
@@ -255,13 +270,12 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
             ps.hist[(wslot * 3 + c) * n + i] = anew[c];
     }
     // the state after the segment: the record pass can then serve as the forward pass
-    if (writer)
-        for (int c = 0; c < 3; ++c) {
-            ps.s[c * n + i] = s[c];
-            ps.S[c * n + i] = S[c];
-            ps.r[c * n + i] = r[c];
-            ps.v[c * n + i] = v[c];
-        }
+    for (int c = 0; c < 3; ++c) {
+        ps.s[c * n + i] = s[c];
+        ps.S[c * n + i] = S[c];
+        ps.r[c * n + i] = r[c];
+        ps.v[c * n + i] = v[c];
+    }
 }
 
 // Reverse pass over steps k1-1 .. k0 for R cotangent rows. lam[(rho * 33 + e) * n + o] holds the

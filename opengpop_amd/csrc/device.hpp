@@ -297,18 +297,12 @@ __device__ __forceinline__ void gravity_itrs_jac(
 // lane) go to LDS, red holding kParts * 16 * 64 doubles, and every wave folds the kParts slots in
 // part order, so value and Jacobian are bit-identical to gravity_itrs_jac for every W.
 constexpr int kJacExchange = kParts * 16 * 64;
+
+// Wave w's Dual3 part sums (parts [w kParts / W, (w + 1) kParts / W)) into their LDS slots.
 template <int W>
-__device__ __forceinline__ void gravity_itrs_jac_split(double x, double y, double z,
-    const GravityDev& g, double* red, double* a, double (*G)[3])
+__device__ __forceinline__ void jac_parts_to_lds(const GravPointT<Dual3>& p, const GravityDev& g,
+    double* red, int w)
 {
-    if (W == 1) {
-        gravity_itrs_jac(x, y, z, g, a, G);
-        return;
-    }
-    static_assert(kParts % W == 0, "waves must divide the part count");
-    const Dual3 X = dvar(x, 0), Y = dvar(y, 1), Z = dvar(z, 2);
-    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
     const int lane = threadIdx.x % 64;
     auto put = [&](int q, int t, const Dual3& d) {
         double* o = red + (q * 16 + 4 * t) * 64 + lane;
@@ -317,11 +311,6 @@ __device__ __forceinline__ void gravity_itrs_jac_split(double x, double y, doubl
         o[128] = d.d[1];
         o[192] = d.d[2];
     };
-    auto get = [&](int q, int t) {
-        const double* o = red + (q * 16 + 4 * t) * 64 + lane;
-        return Dual3(o[0], o[64], o[128], o[192]);
-    };
-    __syncthreads();
     grav_parts<false>(p, g, w * (kParts / W), (w + 1) * (kParts / W),
         [&](int q, const GravSumsT<Dual3>& s) {
             put(q, 0, s.G);
@@ -329,7 +318,17 @@ __device__ __forceinline__ void gravity_itrs_jac_split(double x, double y, doubl
             put(q, 2, s.J);
             put(q, 3, s.K);
         });
-    __syncthreads();
+}
+
+// The kParts exchanged Dual3 part sums, folded in part order, then the ITRS value and Jacobian.
+__device__ __forceinline__ void jac_fold_assemble(const Dual3& X, const Dual3& Y, const Dual3& Z,
+    const GravPointT<Dual3>& p, const GravityDev& g, const double* red, double* a, double (*G)[3])
+{
+    const int lane = threadIdx.x % 64;
+    auto get = [&](int q, int t) {
+        const double* o = red + (q * 16 + 4 * t) * 64 + lane;
+        return Dual3(o[0], o[64], o[128], o[192]);
+    };
     PartFoldT<Dual3> f;
 #pragma unroll 1
     for (int q = 0; q < kParts; ++q)
@@ -343,6 +342,24 @@ __device__ __forceinline__ void gravity_itrs_jac_split(double x, double y, doubl
         for (int j = 0; j < 3; ++j)
             G[i][j] = A[i].d[j];
     }
+}
+
+template <int W>
+__device__ __forceinline__ void gravity_itrs_jac_split(double x, double y, double z,
+    const GravityDev& g, double* red, double* a, double (*G)[3])
+{
+    if (W == 1) {
+        gravity_itrs_jac(x, y, z, g, a, G);
+        return;
+    }
+    static_assert(kParts % W == 0, "waves must divide the part count");
+    const Dual3 X = dvar(x, 0), Y = dvar(y, 1), Z = dvar(z, 2);
+    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+    __syncthreads();
+    jac_parts_to_lds<W>(p, g, red, w);
+    __syncthreads();
+    jac_fold_assemble(X, Y, Z, p, g, red, a, G);
 }
 
 // Column-split evaluation for a workgroup of W waves that all hold the same 64 objects: wave w
@@ -463,18 +480,10 @@ __device__ __forceinline__ void gravity_helper(const GravityDev& g, double* red,
     __syncthreads();
 }
 
-// Acceleration and Jacobian in the GCRS: a = M g(M^T r), G = M Gi M^T (Gi the ITRS Jacobian).
-// The acceleration is bit-identical to gravity_gcrs<false>.
-template <int W = 1>
-__device__ __forceinline__ void gravity_gcrs_jac(const double* Mg, const double* r,
-    const GravityDev& g, double* a, double (*G)[3], double* red = nullptr)
+// a = M gi and G = M Gi M^T for the column-major ITRS->GCRS matrix M.
+__device__ __forceinline__ void jac_to_gcrs(const uniform_dptr M, const double* gi,
+    const double (*Gi)[3], double* a, double (*G)[3])
 {
-    const uniform_dptr M = (uniform_dptr)Mg;
-    double ri[3], gi[3], Gi[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
-    gravity_itrs_jac_split<W>(ri[0], ri[1], ri[2], g, red, gi, Gi);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
@@ -490,6 +499,60 @@ __device__ __forceinline__ void gravity_gcrs_jac(const double* Mg, const double*
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             G[i][j] = M[i] * T[0][j] + M[3 + i] * T[1][j] + M[6 + i] * T[2][j];
+}
+
+// Acceleration and Jacobian in the GCRS: a = M g(M^T r), G = M Gi M^T (Gi the ITRS Jacobian).
+// The acceleration is bit-identical to gravity_gcrs<false>.
+template <int W = 1>
+__device__ __forceinline__ void gravity_gcrs_jac(const double* Mg, const double* r,
+    const GravityDev& g, double* a, double (*G)[3], double* red = nullptr)
+{
+    const uniform_dptr M = (uniform_dptr)Mg;
+    double ri[3], gi[3], Gi[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
+    gravity_itrs_jac_split<W>(ri[0], ri[1], ri[2], g, red, gi, Gi);
+    jac_to_gcrs(M, gi, Gi, a, G);
+}
+
+// Lead/helper form of gravity_gcrs_jac<W> for W > 1 (the derivative record kernel), as
+// gravity_gcrs_lead: wave 0 leaves the ITRS position in pos before the first barrier, every wave
+// sums its Dual3 parts there, and wave 0 alone folds, assembles and rotates. Bit-identical to
+// gravity_gcrs_jac<W>; the barrier hazards are those of gravity_gcrs_lead.
+template <int W>
+__device__ __forceinline__ void gravity_gcrs_jac_lead(const double* Mg, const double* r,
+    const GravityDev& g, double* red, double* pos, double* a, double (*G)[3])
+{
+    static_assert(W > 1 && kParts % W == 0, "waves must divide the part count");
+    const uniform_dptr M = (uniform_dptr)Mg;
+    const int lane = threadIdx.x % 64;
+    double ri[3], gi[3], Gi[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
+        pos[i * 64 + lane] = ri[i];
+    }
+    const Dual3 X = dvar(ri[0], 0), Y = dvar(ri[1], 1), Z = dvar(ri[2], 2);
+    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
+    __syncthreads();
+    jac_parts_to_lds<W>(p, g, red, 0);
+    __syncthreads();
+    jac_fold_assemble(X, Y, Z, p, g, red, gi, Gi);
+    jac_to_gcrs(M, gi, Gi, a, G);
+}
+
+// The part of one gravity_gcrs_jac_lead evaluation that wave w (1 <= w < W) runs.
+template <int W>
+__device__ __forceinline__ void gravity_jac_helper(const GravityDev& g, double* red,
+    const double* pos, int w)
+{
+    const int lane = threadIdx.x % 64;
+    __syncthreads();
+    const Dual3 X = dvar(pos[lane], 0), Y = dvar(pos[64 + lane], 1), Z = dvar(pos[128 + lane], 2);
+    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
+    jac_parts_to_lds<W>(p, g, red, w);
+    __syncthreads();
 }
 
 // Force of the variational-equation lanes (kGravityTangent) at node matrix Mg.

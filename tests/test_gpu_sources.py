@@ -187,3 +187,23 @@ def test_stepwise_with_sources_equals_bulk():
         for k in (1, 4, 5, 12):
             r, v = c.propagate(r0, v0, 60.0, k)
             assert np.array_equal(steps[k - 1][0], r) and np.array_equal(steps[k - 1][1], v), k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_sources_launch_shape_does_not_change_results(precision):
+    # With 2..8 waves per group only the group's first wave integrates and adds the sources;
+    # the result must equal the one-wave launch bit for bit. 200 objects = three full groups
+    # and a partial one; per-object drag; spans end inside the second and third 64-step chunks.
+    d = scenarios.mixed(200, seed=41)
+    steps = np.minimum(d["steps"], 150)
+    steps[::7] = 70
+    bc = np.linspace(0.0, 0.03, 200)
+    with ogp.Context(degree=20, precision=precision) as c:
+        c.set_sources(ALL, 0.01)
+        c.set_drag_coefficients(bc)
+        out = []
+        for w in (1, 2, 4, 8):
+            c.set_tuning(w, 64)
+            out.append(c.propagate(d["r"], d["v"], d["step"], steps))
+        for r, v in out[1:]:
+            assert np.array_equal(r, out[0][0]) and np.array_equal(v, out[0][1])
