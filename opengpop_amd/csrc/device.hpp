@@ -422,13 +422,35 @@ __device__ __forceinline__ void gravity_gcrs(const double* Mg, const double* r,
 }
 
 // Lead/helper form of gravity_gcrs for a workgroup of W > 1 waves in which only wave 0 (the
-// lead) integrates. The lead rotates r into the ITRS and leaves the rotated position in pos
-// (3 x 64 doubles) before the first barrier; every wave then sums its parts at that position
-// (gravity_helper for waves 1 .. W-1) and after the second barrier the lead alone folds the
-// parts, assembles and rotates back. The arithmetic is that of gravity_gcrs<SPLIT, W>, so the
-// result is bit-identical to it. Barrier hazards: pos is rewritten only after the second
-// barrier, behind the helpers' reads; red is rewritten only after the next first barrier, behind
-// the lead's fold.
+// lead) integrates. The lead rotates r into the ITRS, computes the evaluation point's geometry
+// (grav_point) and leaves it in pos (kPointLen x 64 doubles) before the first barrier; every
+// wave then sums its parts from that geometry (gravity_helper for waves 1 .. W-1) and after the
+// second barrier the lead alone folds the parts, assembles and rotates back. The arithmetic is
+// that of gravity_gcrs<SPLIT, W>, so the result is bit-identical to it. Barrier hazards: pos is
+// rewritten only after the second barrier, behind the helpers' reads; red is rewritten only
+// after the next first barrier, behind the lead's fold.
+constexpr int kPointLen = 8; // doubles of a GravPoint
+__device__ __forceinline__ void put_point(double* pos, int lane, const GravPoint& p)
+{
+    const double v[kPointLen] = { p.ir, p.eps, p.rho, p.er, p.rr2, p.cphi, p.U1, p.W1 };
+#pragma unroll
+    for (int e = 0; e < kPointLen; ++e)
+        pos[e * 64 + lane] = v[e];
+}
+__device__ __forceinline__ GravPoint get_point(const double* pos, int lane)
+{
+    GravPoint p;
+    p.ir = pos[lane];
+    p.eps = pos[64 + lane];
+    p.rho = pos[128 + lane];
+    p.er = pos[192 + lane];
+    p.rr2 = pos[256 + lane];
+    p.cphi = pos[320 + lane];
+    p.U1 = pos[384 + lane];
+    p.W1 = pos[448 + lane];
+    return p;
+}
+
 template <bool SPLIT, int W>
 __device__ __forceinline__ void gravity_gcrs_lead(const double* Mg, const double* r,
     const GravityDev& g, double* red, double* pos, double* a)
@@ -438,11 +460,10 @@ __device__ __forceinline__ void gravity_gcrs_lead(const double* Mg, const double
     const int lane = threadIdx.x % 64;
     double ri[3], gi[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 3; ++i)
         ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
-        pos[i * 64 + lane] = ri[i];
-    }
     const GravPoint p = grav_point(ri[0], ri[1], ri[2], g.radius);
+    put_point(pos, lane, p);
     __syncthreads();
     grav_parts<SPLIT>(p, g, 0, kParts / W, [&](int q, const GravSums& s) {
         red[(q * 4 + 0) * 64 + lane] = s.G;
@@ -469,7 +490,7 @@ __device__ __forceinline__ void gravity_helper(const GravityDev& g, double* red,
 {
     const int lane = threadIdx.x % 64;
     __syncthreads();
-    const GravPoint p = grav_point(pos[lane], pos[64 + lane], pos[128 + lane], g.radius);
+    const GravPoint p = get_point(pos, lane);
     grav_parts<SPLIT>(p, g, w * (kParts / W), (w + 1) * (kParts / W),
         [&](int q, const GravSums& s) {
             red[(q * 4 + 0) * 64 + lane] = s.G;

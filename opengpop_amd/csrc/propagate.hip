@@ -4,8 +4,9 @@
 // every wave owns object 64 * blockIdx.x + l. The spherical-harmonic columns of each force
 // evaluation are divided between the W waves and their partial sums meet in LDS.
 //   * GJ8 steps (gj8_chunk_kernel): wave 0 alone runs the integrator, its global loads and
-//     stores, the fold of the parts and the corrector; it hands the rotated position to the
-//     other waves through LDS, and they only sum their parts (device.hpp, gravity_gcrs_lead).
+//     stores, the fold of the parts and the corrector; it hands the geometry of each
+//     evaluation point to the other waves through LDS, and they only sum their parts
+//     (device.hpp, gravity_gcrs_lead).
 //   * Startup (startup_kernel): every wave runs the RK8 and startup-corrector arithmetic
 //     redundantly and bit-identically (device.hpp, gravity_itrs_split), so no wave reads a
 //     value written by another wave through global memory.
@@ -223,7 +224,7 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     int64_t traj_steps)
 {
     __shared__ double red[ExchangeArea<W>::size];
-    __shared__ double pos[W > 1 ? 3 * 64 : 1]; // the lead's ITRS positions (gravity_gcrs_lead)
+    __shared__ double pos[W > 1 ? kPointLen * 64 : 1]; // the lead's geometry (gravity_gcrs_lead)
     Exchange ex { red };
     const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
     if (t >= nobj)
