@@ -260,6 +260,27 @@ def test_launch_shape_does_not_change_results(precision):
             assert np.array_equal(r, out[0][0]) and np.array_equal(v, out[0][1])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_ragged_spans_every_wave_count(precision):
+    # Spans 0 .. 140 spread over every group, so objects end in the startup, at every point of
+    # the three 64-step chunks, and next to objects that run on. With 2..8 waves per group only
+    # the first wave integrates, and the others must follow it step for step: the result
+    # equals the one-wave launch bit for bit, and a span of 0 leaves the state unchanged.
+    d = scenarios.mixed(330, seed=17)
+    steps = (np.arange(330) * 37) % 141
+    with ogp.Context(degree=20, precision=precision) as c:
+        out = []
+        for w in (1, 2, 4, 8):
+            c.set_tuning(w, 64)
+            out.append(c.propagate(d["r"], d["v"], d["step"], steps))
+        for r, v in out[1:]:
+            assert np.array_equal(r, out[0][0]) and np.array_equal(v, out[0][1])
+        zero = steps == 0
+        assert zero.any()
+        assert np.array_equal(out[0][0][zero], d["r"][zero])
+        assert np.array_equal(out[0][1][zero], d["v"][zero])
+
+
 def test_concurrent_slices_do_not_change_results():
     # 900 objects in two step-size classes (720 at 30 s, 180 at 300 s) with uneven spans, cut
     # into 1..7 slices per class on concurrent streams; also with the autotuner's trial chunks
