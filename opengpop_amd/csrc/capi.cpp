@@ -1412,6 +1412,11 @@ int ogp_conj_propagate(ogp_context* ctx, int64_t n_steps)
         HIPC(hipMemcpyAsync(ctx->cj_v.p, ctx->st_v.p, sizeof(double) * 3 * n,
             hipMemcpyDeviceToDevice, ctx->stream));
         ctx->sync();
+        // a later phase starts at another time origin and never reuses this phase's table:
+        // drop it, so that a loop of short phases does not grow the cache (the epoch's own
+        // table, t0 = 0, is shared with the other propagation calls and kept)
+        if (t0 != 0.0)
+            ctx->nodes.erase(std::make_pair(ctx->cj_h, t0));
         ctx->cj_steps = total;
         ctx->cj_timing.propagate_ms = ctx->timing.total_ms;
     });
