@@ -15,9 +15,11 @@ complete propagation of that population. Side legs (same line):
             discrete adjoint and by the variational equations; agreement and cost vs the forward
             propagation
   cpu       the FP64 host oracle on a stratified 2,500-object sample of the headline workload,
-            timed on this host's cores (cpu_baseline). Run last, it is also the checker: the
-            GPU-vs-oracle difference on that sample, on 32 objects of the split workload, and
-            the adjoint against central differences of the oracle for two objects
+            timed on this host's cores (cpu_baseline; plus a single-core rate on 125 objects).
+            Run last, it is also the checker: the GPU-vs-oracle difference on that sample, the
+            full pairs workload propagated, screened and refined by the oracle against the GPU's
+            candidate set, 32 objects of the split workload, and the adjoint against central
+            differences of the oracle for two objects
   refpoint  10,000 LEO, 30 s, 720 steps, degree 70, end to end from host arrays (the workload
             of the reference's published H100 figure), and the same through the command-line
             tool as a whole process (JSON in, JSON out)
@@ -40,6 +42,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "object-steps/s at deg-70 gravity: FP64 vs split-prec (≤1 m @7 d); pairs/s"
 CPU_SAMPLE_STRIDE = 40  # cpu leg: every 40th object of the headline workload (~12 s on 16 threads)
+CPU_SINGLE_STRIDE = 800  # single-core rate: every 800th object (~8 s on one thread)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (equal to its FP64 matrix figure)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
@@ -79,7 +82,8 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--legs", default="fp64,split,pairs,adjoint,cpu,refpoint,sources,perturb")
-    p.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    p.add_argument("--cpu-threads", type=int, default=0,
+                   help="0 = min(16, CPUs in the affinity mask)")
     return p.parse_args()
 
 
@@ -218,7 +222,9 @@ def leg_pairs(ogp, scenarios):
         t2 = time.perf_counter()
         t = ca.timing()
     pair_steps = t["pair_steps"]
-    return dict(
+    gpu = dict(a=np.asarray(res.a), b=np.asarray(res.b), step=np.asarray(res.step),
+               distance=np.asarray(res.distance), time=np.asarray(res.time))
+    return dict(_oracle_sample=dict(c4=c4, gpu=gpu),
         workload="20,000 objects (1,000 forced pairs), set 0 = first 2,000, 3 d at 60 s, "
                  "JGM3 20x20, 10 km screen",
         pairs=int(t["pairs"]), pair_steps=pair_steps, candidates=int(t["candidates"]),
@@ -235,7 +241,8 @@ def leg_pairs(ogp, scenarios):
 
 def leg_cpu(sample, threads):
     """FP64 oracle on every 40th object of the headline workload (stratified over regimes), the
-    cpu_baseline; returns it with the GPU-vs-oracle difference on that sample."""
+    cpu_baseline, on `threads` cores; then every 800th object on one core (the single-core rate).
+    Returns it with the GPU-vs-oracle difference on the first sample."""
     from oracle import pyoracle as po
     c1, idx = sample["c1"], sample["idx"]
     o = po.Oracle(degree=70)
@@ -246,12 +253,64 @@ def leg_cpu(sample, threads):
     work = float(c1["steps"][idx].sum())
     dr = np.linalg.norm(r - sample["r"], axis=1)
     dv = np.linalg.norm(v - sample["v"], axis=1)
+    one = np.arange(0, len(c1["r"]), CPU_SINGLE_STRIDE)
+    t0 = time.perf_counter()
+    o.propagate(po.GRAVITY, c1["r"][one], c1["v"][one], c1["step"][one], c1["steps"][one],
+                threads=1)
+    dt1 = time.perf_counter() - t0
+    work1 = float(c1["steps"][one].sum())
     return dict(value=work / dt, unit="object-steps/s", cores=threads, kind="port",
                 sample=f"{len(idx)} objects (every {CPU_SAMPLE_STRIDE}th of the 100,000-object mixed "
                        f"workload), "
                        f"{int(work)} object-steps, FP64 oracle, OpenMP over objects, {dt:.1f} s",
-                cpu_model=_cpu_model()), dict(objects=len(idx), max_dr_m=float(dr.max()),
-                                             max_dv_mps=float(dv.max()))
+                single_core=dict(value=work1 / dt1, unit="object-steps/s",
+                                 sample=f"{len(one)} objects (every {CPU_SINGLE_STRIDE}th), "
+                                        f"{int(work1)} object-steps, 1 thread, {dt1:.1f} s"),
+                cpu_model=_cpu_model(), nproc=os.cpu_count(),
+                affinity_cpus=len(os.sched_getaffinity(0)),
+                threads_rule="min(16, CPUs in this process's affinity mask): the box's CPU share "
+                             "for one GPU is 16"), \
+        dict(objects=len(idx), max_dr_m=float(dr.max()), max_dv_mps=float(dv.max()))
+
+
+def pairs_oracle_check(sample, threads):
+    """The conjunction search of the pairs leg at full size against the oracle: the oracle
+    propagates all 20,000 objects over the 3-day window (4,320 steps, degree 20), screens the
+    36 M cross-set pairs at 10 km with the per-pair rule of the reference (in a blocked loop
+    order) and refines its candidates. The candidate pair indices and their sampled minimum steps
+    must be equal; TCA and miss distance are compared with the oracle's refinement."""
+    from oracle import pyoracle as po
+    c4, gpu = sample["c4"], sample["gpu"]
+    o = po.Oracle(degree=20)
+    t0 = time.perf_counter()
+    _, _, traj = o.propagate(po.GRAVITY, c4["r"], c4["v"], c4["step"], c4["steps"],
+                             trajectory=True, threads=threads)
+    t1 = time.perf_counter()
+    log(f"pairs oracle: propagation {t1 - t0:.1f} s")
+    mind, mink = o.screen_blocked(traj, c4["sets"], 10e3, threads=threads)
+    t2 = time.perf_counter()
+    log(f"pairs oracle: screen {t2 - t1:.1f} s")
+    cand = np.nonzero(mind < 10e3)[0]
+    na = int((c4["sets"] == 0).sum())
+    nb = len(c4["sets"]) - na
+    oa, ob = cand // nb, na + cand % nb  # two sets: pair index = a * |B| + (b - |A|)
+    same_set = (len(cand) == len(gpu["a"]) and np.array_equal(oa, gpu["a"])
+                and np.array_equal(ob, gpu["b"]))
+    out = dict(candidates_oracle=int(len(cand)), candidates_gpu=int(len(gpu["a"])),
+               candidate_set_equal=bool(same_set),
+               min_step_equal=bool(same_set and np.array_equal(mink[cand], gpu["step"])),
+               nearest_to_threshold_m=float(np.abs(mind - 10e3).min()),
+               oracle_propagate_s=t1 - t0, oracle_screen_s=t2 - t1, threads=threads)
+    if same_set:
+        dt = dd = 0.0
+        for q in range(len(cand)):
+            ref = o.refine(traj, c4["step"], int(oa[q]), int(ob[q]), int(mink[cand[q]]),
+                           float(mind[cand[q]]))
+            dt = max(dt, abs(ref["time"] - gpu["time"][q]))
+            dd = max(dd, abs(ref["distance"] - gpu["distance"][q]))
+        out.update(max_tca_diff_s=dt, max_miss_diff_m=dd)
+    del traj
+    return out
 
 
 def oracle_checks(split_sample, adjoint_sample, threads):
@@ -446,7 +505,7 @@ def main():
     import opengpop_amd as ogp
     from opengpop_amd import scenarios
     ogp.lib()  # fails loudly when libogp.so is missing
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
 
     line = dict(metric=METRIC, value=None, unit="object-steps/s", n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=None, higher_is_better=True, scaling="weak",
@@ -462,12 +521,13 @@ def main():
         line["config"]["object_steps"] = f["object_steps"]
         line["device"] = f["device"]
         line["wave_tuning"] = f["tuning"]
-    split_sample = adjoint_sample = None
+    split_sample = adjoint_sample = pairs_sample = None
     if "split" in legs:
         line["split"] = leg_split(ogp, scenarios, args)
         split_sample = line["split"].pop("_oracle_sample")
     if "pairs" in legs:
         line["pairs"] = leg_pairs(ogp, scenarios)
+        pairs_sample = line["pairs"].pop("_oracle_sample")
         log(f"pairs leg: {line['pairs']['pair_steps_per_s']:.3e} pair-steps/s")
     if "adjoint" in legs:
         line["adjoint"] = leg_adjoint(ogp, scenarios)
@@ -490,6 +550,9 @@ def main():
         if sample is not None:
             line["cpu_baseline"], line["parity_sample"] = leg_cpu(sample, threads)
             log(f"cpu leg: {line['cpu_baseline']['value']:.3e} object-steps/s on {threads} threads")
+        if pairs_sample is not None:
+            line["pairs"]["vs_oracle"] = pairs_oracle_check(pairs_sample, threads)
+            pairs_sample = None
         checks = oracle_checks(split_sample, adjoint_sample, threads)
         if "split" in checks:
             line["split"]["vs_oracle_sample_m"] = checks["split"]

@@ -1678,6 +1678,103 @@ int orc_screen(const double* traj, int comps, int nobj, int nsteps, const unsign
     return 0;
 }
 
+// The same per-pair rule as orc_screen in a blocked loop order, for screens of 10^7-10^8 pairs
+// over thousands of steps. orc_screen keeps, per pair, the first sample whose root is strictly
+// below the running minimum; since sqrt is monotone and correctly rounded, its final distance is
+// sqrt(min d2) and its step the first sample whose root equals that. So here the positions are
+// transposed to [step][component][object] and tiles of 32 x 64 pairs keep min d2 over the whole
+// time axis (a NaN is never taken, as under "dd < best"); the step is then found by a second
+// scan, only for the pairs that need it (root below threshold, or every pair when
+// threshold <= 0; the others get -1). Pairs without a sample below DBL_MAX get DBL_MAX and -1,
+// as in orc_screen.
+int orc_screen_blocked(const double* traj, int comps, int nobj, int nsteps, const unsigned* set,
+    double threshold, double* mind, int* mink, int threads)
+{
+    Partition p;
+    if (nobj <= 0 || !p.calc(set, nobj))
+        return 1;
+#ifdef _OPENMP
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+    const size_t N = (size_t)nobj;
+    std::vector<double> P((size_t)nsteps * 3 * N);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (long k = 0; k < nsteps; ++k)
+        for (size_t o = 0; o < N; ++o)
+            for (int c = 0; c < 3; ++c)
+                P[((size_t)k * 3 + c) * N + o] = traj[((size_t)k * N + o) * comps + c];
+    constexpr int TA = 32, TB = 64;
+    struct Tile {
+        unsigned s, a0, na, b0, nb;
+    };
+    std::vector<Tile> tiles;
+    for (unsigned s = 0; s + 1 < p.numSets; ++s) {
+        const unsigned A0 = p.offsets[s], A1 = p.offsets[s + 1], B1 = (unsigned)nobj;
+        for (unsigned a = A0; a < A1; a += TA)
+            for (unsigned b = A1; b < B1; b += TB)
+                tiles.push_back({ s, a, std::min<unsigned>(TA, A1 - a), b,
+                    std::min<unsigned>(TB, B1 - b) });
+    }
+    const double dmax = std::numeric_limits<double>::max();
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+    for (long t = 0; t < (long)tiles.size(); ++t) {
+        const Tile T = tiles[t];
+        alignas(64) double m[TA][TB];
+        for (unsigned i = 0; i < TA; ++i)
+            for (unsigned j = 0; j < TB; ++j)
+                m[i][j] = std::numeric_limits<double>::infinity();
+        for (int k = 0; k < nsteps; ++k) {
+            const double* px = &P[(size_t)k * 3 * N];
+            const double* py = px + N;
+            const double* pz = py + N;
+            const double* bx = px + T.b0;
+            const double* by = py + T.b0;
+            const double* bz = pz + T.b0;
+            for (unsigned i = 0; i < T.na; ++i) {
+                const double ax = px[T.a0 + i], ay = py[T.a0 + i], az = pz[T.a0 + i];
+                double* mi = m[i];
+                for (unsigned j = 0; j < T.nb; ++j) {
+                    const double dx = ax - bx[j], dy = ay - by[j], dz = az - bz[j];
+                    const double d2 = std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+                    mi[j] = d2 < mi[j] ? d2 : mi[j];
+                }
+            }
+        }
+        const unsigned A0 = p.offsets[T.s], B0 = p.offsets[T.s + 1];
+        for (unsigned i = 0; i < T.na; ++i)
+            for (unsigned j = 0; j < T.nb; ++j) {
+                const unsigned a = T.a0 + i, b = T.b0 + j;
+                const size_t q = (size_t)p.mapoff[T.s] + (size_t)(a - A0) * p.higher[T.s] + (b - B0);
+                const double root = std::sqrt(m[i][j]);
+                if (!(root < dmax)) {
+                    mind[q] = dmax;
+                    mink[q] = -1;
+                    continue;
+                }
+                mind[q] = root;
+                mink[q] = -1;
+                if (threshold > 0.0 && !(root < threshold))
+                    continue;
+                for (int k = 0; k < nsteps; ++k) {
+                    const double* px = &P[(size_t)k * 3 * N];
+                    const double dx = px[a] - px[b], dy = px[N + a] - px[N + b],
+                                 dz = px[2 * N + a] - px[2 * N + b];
+                    if (std::sqrt(std::fma(dz, dz, std::fma(dy, dy, dx * dx))) == root) {
+                        mink[q] = k;
+                        break;
+                    }
+                }
+            }
+    }
+    return 0;
+}
+
 int orc_refine(const double* traj, int nobj, int nsteps, double h, unsigned a, unsigned b, int k,
     double mind, double* out)
 {

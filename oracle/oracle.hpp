@@ -135,6 +135,10 @@ double orc_approach_func(const double* qa27, const double* qb27, double dt, doub
 // for every cross-set pair p: min distance (first occurrence) and its step.
 int orc_screen(const double* traj, int comps, int n_obj, int n_steps, const unsigned* set_ids,
     double* min_dist, int* min_step);
+// The same rule in a blocked loop order for large screens: min_dist for every pair, min_step for
+// the pairs below threshold (every pair when threshold <= 0), -1 for the others.
+int orc_screen_blocked(const double* traj, int comps, int n_obj, int n_steps,
+    const unsigned* set_ids, double threshold, double* min_dist, int* min_step, int threads);
 // Refine pair (a, b) whose sampled minimum is at step k: TCA and miss distance and the
 // positions of A and B at TCA (CalculateClosestApproachesKernel, device.cu:93-173).
 int orc_refine(const double* traj, int n_obj, int n_steps, double h, unsigned a, unsigned b,

@@ -85,6 +85,7 @@ class Oracle:
         L.orc_approach_func.restype = _D
         L.orc_approach_func.argtypes = [_P, _P, _D, _D, _D, _D, _I]
         L.orc_screen.argtypes = [_P, _I, _I, _I, _P, _P, _P]
+        L.orc_screen_blocked.argtypes = [_P, _I, _I, _I, _P, _D, _P, _P, _I]
         L.orc_refine.argtypes = [_P, _I, _I, _D, ctypes.c_uint, ctypes.c_uint, _I, _D, _P]
         self.loaded = False
         if degree is not None:
@@ -339,6 +340,20 @@ class Oracle:
         mink = np.zeros(npairs, dtype=np.int32)
         if self.L.orc_screen(_ptr(traj), comps, nobj, nsteps, _ptr(ids), _ptr(mind),
                              _ptr(mink)) != 0:
+            raise RuntimeError("invalid set ids")
+        return mind, mink
+
+    def screen_blocked(self, traj, set_ids, threshold=0.0, threads=0):
+        """orc_screen's rule in a blocked loop order: (min_dist, min_step) per pair, with the step
+        only for pairs below threshold (every pair when threshold <= 0) and -1 for the others."""
+        traj = np.ascontiguousarray(traj, dtype=np.float64)
+        nsteps, nobj, comps = traj.shape
+        ids = np.ascontiguousarray(set_ids, dtype=np.uint32)
+        npairs = self.partition(ids)["num_pairs"]
+        mind = np.zeros(npairs)
+        mink = np.zeros(npairs, dtype=np.int32)
+        if self.L.orc_screen_blocked(_ptr(traj), comps, nobj, nsteps, _ptr(ids), threshold,
+                                     _ptr(mind), _ptr(mink), threads) != 0:
             raise RuntimeError("invalid set ids")
         return mind, mink
 
