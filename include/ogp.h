@@ -127,6 +127,23 @@ int ogp_get_sources(const ogp_context* ctx, ogp_sources* sources);
  * coefficient. */
 int ogp_set_drag_coefficients(ogp_context* ctx, const double* bc, int64_t n);
 
+/* Device-pointer forms of the force-model boundary, for callers that keep their state in HBM
+ * (the reference's IForceModel takes device pointers, iforce_model.cuh:19-33). Arrays are AoS
+ * [n][3] device buffers; nothing crosses PCIe.
+ *
+ * ForceModel::ComputePropagationAcceleration (force_model.cu:91-109): as ogp_accel on device
+ * arrays; d_a[i * out_stride + c] receives object i's component c. */
+int ogp_accel_device(ogp_context* ctx, const double* d_r, const double* d_v, int64_t n,
+    double step_s, int64_t node, double* d_a, int64_t out_stride);
+
+/* ForceModel::ComputeGenerationAcceleration (force_model.cu:59-89): the acceleration at RK8
+ * stage k_index (0..9) of startup step step_num (+-1 .. +-4) of a run with step step_s, written
+ * into column k_index of the caller's K matrix d_K [n][10][3]. The stage's time node is the
+ * reference's ((step_num > 0 ? step_num - 1 : step_num) + 4) * 10 + k_index (with
+ * OGP_QUIRK_BACKWARD_TIMES, the reference's mirrored times of backward stages). */
+int ogp_accel_generation(ogp_context* ctx, const double* d_r, const double* d_v, int64_t n,
+    double step_s, int step_num, int k_index, double* d_K);
+
 /* FMTimePoints::CalculateParameters + GetParametersDev (time_points.cu:126-207): the
  * ITRS->GCRS matrix (column-major, 9 per node) and MJD(UTC) of `len` nodes at offsets
  * offsets_s (seconds) from (epoch - before * step_s). */
@@ -145,6 +162,25 @@ int ogp_propagate(ogp_context* ctx, const double* r0, const double* v0, int64_t 
  * nothing crosses PCIe. */
 int ogp_propagate_device(ogp_context* ctx, const double* d_r0_soa, const double* d_v0_soa,
     int64_t n, const double* step_s, const int64_t* steps, double* d_r_soa, double* d_v_soa);
+
+/* Per-object failure flags (no counterpart in the reference, which exits the process on any
+ * error, gpu_utils.cuh:24-48). As ogp_propagate / ogp_propagate_device, but instead of failing
+ * the call: an object whose initial state holds a non-finite value is skipped (its output is its
+ * input) and flagged OGP_OBJECT_NONFINITE_INPUT; one whose final state is non-finite is flagged
+ * OGP_OBJECT_NONFINITE_RESULT. status (host, n entries, caller order) receives the flags; every
+ * other object's result is bit-identical to a run without the flagged objects. (The unchecked
+ * entry points reject a non-finite initial state with OGP_ERR_NONFINITE; ogp_propagate_device
+ * checks it on the device.) */
+enum {
+    OGP_OBJECT_OK = 0,
+    OGP_OBJECT_NONFINITE_INPUT = 1,
+    OGP_OBJECT_NONFINITE_RESULT = 2
+};
+int ogp_propagate_checked(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    const double* step_s, const int64_t* steps, double* r_out, double* v_out, int32_t* status);
+int ogp_propagate_device_checked(ogp_context* ctx, const double* d_r0_soa, const double* d_v0_soa,
+    int64_t n, const double* step_s, const int64_t* steps, double* d_r_soa, double* d_v_soa,
+    int32_t* status);
 
 /* PerturbationGenerator::generatePerturbedSatellites (perturbation_generator.cu:40-79,
  * perturbation_generator.cuh:24-43): n satellites at the epoch position r0 with velocity
@@ -196,10 +232,38 @@ int ogp_get_tuning(const ogp_context* ctx, int64_t n, int* waves, int64_t* chunk
  * of at least 4096 objects, else 1), at most 16. Results are bit-identical for every choice. */
 int ogp_set_slices(ogp_context* ctx, int slices);
 
-/* Stepwise GpuPropagator (gpu_propagator.cuh:50-58): init with n objects and one step size;
- * each ogp_prop_step performs one GpuPropagator::Step (startup on the first call, startup
- * nodes for steps 1-4, GJ8 from step 5) and optionally returns r, v and the acceleration
- * column the reference hands to the conjunction screen (gpu_propagator.cu:68-72). */
+/* GpuPropagator (gpu_propagator.cuh:26-59, gpu_propagator.cu:28-154): a stepwise propagator
+ * of n objects with one step size. Each propagator owns its integrator state (s, S, the
+ * acceleration history), so any number of them, and any other call of the context, may be
+ * interleaved. It belongs to its context: destroy it first (a context destroyed before it
+ * frees its device memory, and its later calls return OGP_ERR_STATE).
+ *
+ * ogp_propagator_create: the GpuPropagator constructor. d_r, d_v are the caller's AoS [n][3]
+ * device buffers: the first step reads the epoch state from them, and every step writes the
+ * state at that step back into them, in place (gpu_propagator.cu:128-129 advances them).
+ * ogp_propagator_create_host: the same from host arrays; the propagator then owns the device
+ * buffers.
+ * ogp_propagator_step: GpuPropagator::Step(hostPositions, hostVelocities, devAcceleration):
+ * the startup on the first call, the startup nodes for steps 1-4, a GJ8 step from step 5; then
+ * optionally the state to host arrays and the acceleration column the reference hands to the
+ * conjunction screen (gpu_propagator.cu:68-72) to a device array d_acc [n][3]. Each may be null.
+ * ogp_propagator_acceleration: that acceleration column of the last step, to a host array.
+ * ogp_propagator_copy_state_to_host: GpuPropagator::CopyStateToHost.
+ * ogp_propagator_destroy: GpuPropagator::FreeAllMemory and the destructor. */
+typedef struct ogp_propagator ogp_propagator;
+int ogp_propagator_create(ogp_context* ctx, double* d_r, double* d_v, int64_t n, double step_s,
+    int64_t max_steps, ogp_propagator** out);
+int ogp_propagator_create_host(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double step_s, int64_t max_steps, ogp_propagator** out);
+int ogp_propagator_step(ogp_propagator* p, double* host_r, double* host_v, double* d_acc);
+int ogp_propagator_acceleration(const ogp_propagator* p, double* host_a);
+int ogp_propagator_copy_state_to_host(const ogp_propagator* p, double* host_r, double* host_v);
+void ogp_propagator_destroy(ogp_propagator* p);
+
+/* The context's own stepwise propagator (one per context, host arrays): ogp_prop_init
+ * replaces it, ogp_prop_step performs one GpuPropagator::Step and returns r, v and the
+ * acceleration column as ogp_propagator_step / _acceleration do; the arrays hold the n objects
+ * of the last ogp_prop_init. Its state is its own, as above. */
 int ogp_prop_init(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
     int64_t max_steps);
 int ogp_prop_step(ogp_context* ctx, double* r_out, double* v_out, double* a_out);
@@ -208,6 +272,17 @@ int ogp_prop_step(ogp_context* ctx, double* r_out, double* v_out, double* a_out)
  * acceleration matrix, each [n][9][3] (host). */
 int ogp_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
     double* pos, double* vel, double* s, double* S, double* acc);
+/* The same on device arrays (the FullStartupData of gpu_startup.cuh:95-120): d_r0, d_v0 AoS
+ * [n][3]; outputs [n][9][3] device arrays, each may be null. */
+int ogp_startup_device(ogp_context* ctx, const double* d_r0, const double* d_v0, int64_t n,
+    double step_s, double* d_pos, double* d_vel, double* d_s, double* d_S, double* d_acc);
+
+/* Predictor::Predict(stepNum) (prediction.cu:49-74) on the device arrays Predictor::SetPointers
+ * names: AoS d_r, d_v [n][3] receive the predicted state, d_S [n][3] is advanced, d_s [n][3] is
+ * read, and the history d_acc [n][9][3] is shifted by one column with the acceleration at
+ * propagation node step_num + 4 (gravity plus the enabled sources) written into column 8. */
+int ogp_predict_device(ogp_context* ctx, double* d_r, double* d_v, double* d_acc,
+    const double* d_s, double* d_S, int64_t n, double step_s, int64_t step_num);
 
 /* Known-answer hooks that run the device's own predictor and corrector arithmetic on given
  * inputs (integrator_prediction_test.cu, integrator_test.cu:207-245). n objects, host arrays

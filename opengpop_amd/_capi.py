@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(PKG, "libogp.so")
 DATA_DIR = os.path.join(os.path.dirname(PKG), "data")
 
 OK, ERR_ARGUMENT, ERR_DATA, ERR_DEVICE, ERR_STATE, ERR_NONFINITE = range(6)
+OBJECT_OK, OBJECT_NONFINITE_INPUT, OBJECT_NONFINITE_RESULT = range(3)
 FORCE_GRAVITY, FORCE_NONE, FORCE_CONST, FORCE_POLY7, FORCE_POLY8, FORCE_POLY9, FORCE_EXP, \
     FORCE_DAMPED = range(8)
 PRECISION_FP64, PRECISION_SPLIT = 0, 1
@@ -105,7 +106,23 @@ def lib():
                                           ctypes.POINTER(_I64)]),
         "ogp_prop_init": (ctypes.c_int, [_P, _P, _P, _I64, _D, _I64]),
         "ogp_prop_step": (ctypes.c_int, [_P, _P, _P, _P]),
+        "ogp_propagator_create": (ctypes.c_int, [_P, _P, _P, _I64, _D, _I64,
+                                                 ctypes.POINTER(_P)]),
+        "ogp_propagator_create_host": (ctypes.c_int, [_P, _P, _P, _I64, _D, _I64,
+                                                      ctypes.POINTER(_P)]),
+        "ogp_propagator_step": (ctypes.c_int, [_P, _P, _P, _P]),
+        "ogp_propagator_acceleration": (ctypes.c_int, [_P, _P]),
+        "ogp_propagator_copy_state_to_host": (ctypes.c_int, [_P, _P, _P]),
+        "ogp_propagator_destroy": (None, [_P]),
         "ogp_startup": (ctypes.c_int, [_P, _P, _P, _I64, _D, _P, _P, _P, _P, _P]),
+        "ogp_startup_device": (ctypes.c_int, [_P, _P, _P, _I64, _D, _P, _P, _P, _P, _P]),
+        "ogp_predict_device": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I64, _D, _I64]),
+        "ogp_accel_device": (ctypes.c_int, [_P, _P, _P, _I64, _D, _I64, _P, _I64]),
+        "ogp_accel_generation": (ctypes.c_int, [_P, _P, _P, _I64, _D, ctypes.c_int,
+                                                ctypes.c_int, _P]),
+        "ogp_propagate_checked": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P, _P]),
+        "ogp_propagate_device_checked": (ctypes.c_int, [_P, _P, _P, _I64, _P, _P, _P, _P,
+                                                        _P]),
         "ogp_kat_predict": (ctypes.c_int, [_P, _P, _P, _P, _I64, _D, _P, _P]),
         "ogp_kat_finalize_correction": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I64, _D]),
         "ogp_conj_init": (ctypes.c_int, [_P, _P, _P, _P, _I64, _D]),

@@ -100,6 +100,11 @@ struct Event {
 
 } // namespace
 
+struct ogp_propagator;
+// Frees the device memory of every propagator of a context that is being destroyed and marks
+// them detached (defined with ogp_propagator below).
+void detach_propagators(ogp_context* ctx);
+
 struct ogp_context {
     ogp_config cfg {};
     std::string data_dir;
@@ -122,13 +127,14 @@ struct ogp_context {
 
     // propagation state (internal, class-sorted order)
     DevBuf st_r, st_v, st_s, st_S, st_hist, st_ns, perm, io_a, io_b, io_c;
+    DevBuf d_flags, d_count; // per-object flags of the input / result checks
     DevBuf sc[7];
     ogp_timing timing {};
 
-    // stepwise propagator
-    int64_t sw_n = 0, sw_step = 0, sw_max = 0;
-    double sw_h = 0;
-    DevBuf sw_traj;
+    // stepwise propagators: every live one (ogp_propagator_*), and the context's own
+    // (ogp_prop_init / ogp_prop_step), which is one of them
+    std::vector<ogp_propagator*> props;
+    ogp_propagator* sw = nullptr;
 
     // conjunction
     int64_t cj_n = 0, cj_steps = 0, cj_cap_steps = 0;
@@ -155,6 +161,7 @@ struct ogp_context {
 
     ~ogp_context()
     {
+        detach_propagators(this);
         for (hipStream_t s : lane_streams)
             if (s)
                 (void)hipStreamDestroy(s);
@@ -228,9 +235,8 @@ struct ogp_context {
     // and conjunction states keep their own copies.
     ogp_sources sources {};
     std::vector<double> drag_bc_obj;
-    DevBuf d_bc, sw_bc, cj_bc;
+    DevBuf d_bc, cj_bc;
     const double* bc_obj = nullptr;
-    const double* sw_bc_ptr = nullptr;
     const double* cj_bc_ptr = nullptr;
     SourcesDev sources_dev() const
     {
@@ -366,6 +372,14 @@ struct ogp_context {
         st_hist.reserve(9 * v3, &dev_bytes);
         st_ns.reserve(sizeof(int64_t) * N, &dev_bytes);
         perm.reserve(sizeof(int64_t) * N, &dev_bytes);
+        reserve_scratch(N);
+    }
+
+    // startup work areas for N objects (shared by every startup of the context; each call
+    // that runs a startup completes it before it returns)
+    void reserve_scratch(int64_t N)
+    {
+        const size_t v3 = sizeof(double) * 3 * N;
         const int slots[7] = { 9, 9, 9, 9, 9, 10, 10 };
         for (int q = 0; q < 7; ++q)
             sc[q].reserve(slots[q] * v3, &dev_bytes);
@@ -791,6 +805,58 @@ int ogp_accel(ogp_context* ctx, const double* r, const double* v, int64_t n, dou
     });
 }
 
+int ogp_accel_device(ogp_context* ctx, const double* d_r, const double* d_v, int64_t n,
+    double step_s, int64_t node, double* d_a, int64_t out_stride)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n >= 0 && std::isfinite(step_s) && step_s > 0 && node >= 0 && out_stride >= 3,
+            "invalid accel arguments");
+        arg_check(n == 0 || (d_r && d_a), "null array");
+        if (ctx->cfg.force != OGP_FORCE_GRAVITY)
+            throw ApiError(OGP_ERR_STATE, "context was created without the gravity field");
+        arg_check(d_v || !(ctx->sources.mask & kSourceDrag), "drag needs the velocities");
+        if (n == 0)
+            return;
+        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, nullptr);
+        NodeTable* nt = ctx->node_table(step_s, node);
+        launch_gravity_aos(ctx->stream, ctx->kind == kGravitySplit, ctx->gdev,
+            nt->prop_M.as<double>() + 9 * node, d_r, n, d_a, out_stride, 64, ctx->sources_dev(),
+            nt->prop_tb.as<double>() + 6 * node, d_v);
+        ctx->check_launch();
+        ctx->sync();
+    });
+}
+
+int ogp_accel_generation(ogp_context* ctx, const double* d_r, const double* d_v, int64_t n,
+    double step_s, int step_num, int k_index, double* d_K)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n >= 0 && std::isfinite(step_s) && step_s > 0, "invalid accel arguments");
+        // force_model.cu:62-65 rejects the stage index; the step is one of the 4 + 4 RK8 steps
+        arg_check(k_index >= 0 && k_index < 10, "kIndex must be between 0 and 9");
+        arg_check(step_num != 0 && step_num >= -4 && step_num <= 4,
+            "stepNum must be one of -4 .. -1, 1 .. 4");
+        arg_check(n == 0 || (d_r && d_K), "null array");
+        if (ctx->cfg.force != OGP_FORCE_GRAVITY)
+            throw ApiError(OGP_ERR_STATE, "context was created without the gravity field");
+        arg_check(d_v || !(ctx->sources.mask & kSourceDrag), "drag needs the velocities");
+        if (n == 0)
+            return;
+        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, nullptr);
+        NodeTable* nt = ctx->node_table(step_s, 4);
+        const int idx = ((step_num > 0 ? step_num - 1 : step_num) + 4) * 10 + k_index;
+        launch_gravity_aos(ctx->stream, ctx->kind == kGravitySplit, ctx->gdev,
+            nt->gen_M.as<double>() + 9 * idx, d_r, n, d_K + 3 * k_index, 30, 64,
+            ctx->sources_dev(), nt->gen_tb.as<double>() + 6 * idx, d_v);
+        ctx->check_launch();
+        ctx->sync();
+    });
+}
+
 int ogp_time_nodes(ogp_context* ctx, double step_s, int before, const double* offsets_s,
     int64_t len, double* M, double* mjd)
 {
@@ -858,6 +924,162 @@ void validate_prop(const double* step_s, const int64_t* steps, int64_t n)
     }
 }
 
+// Object flags for the failure-flag entry points (ogp.h: OGP_OBJECT_*).
+bool finite6(const double* r, const double* v)
+{
+    return std::isfinite(r[0]) && std::isfinite(r[1]) && std::isfinite(r[2])
+        && std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+
+void record_counts(ogp_context* ctx, const int64_t* steps, int64_t n)
+{
+    double os = 0, gj = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        os += (double)steps[i];
+        gj += (double)std::max<int64_t>(steps[i] - 4, 0);
+    }
+    ctx->timing.object_steps = os;
+    ctx->timing.gj_object_steps = gj;
+}
+
+// The runner's propagate() from host arrays. With status, objects with a non-finite initial
+// state are skipped (zero steps) and flagged, and non-finite results are flagged; without it a
+// non-finite initial state fails the call.
+void propagate_host(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    const double* step_s, const int64_t* steps_in, double* r_out, double* v_out, int32_t* status)
+{
+    arg_check(n == 0 || (r0 && v0 && step_s && steps_in && r_out && v_out), "null array");
+    validate_prop(step_s, steps_in, n);
+    if (n == 0)
+        return;
+    std::vector<int64_t> steps(steps_in, steps_in + n);
+    if (status) {
+        for (int64_t i = 0; i < n; ++i) {
+            status[i] = finite6(r0 + 3 * i, v0 + 3 * i) ? OGP_OBJECT_OK : OGP_OBJECT_NONFINITE_INPUT;
+            if (status[i] != OGP_OBJECT_OK)
+                steps[i] = 0;
+        }
+    } else {
+        check_finite(r0, 3 * n, "positions");
+        check_finite(v0, 3 * n, "velocities");
+    }
+    std::vector<int64_t> perm;
+    auto groups = plan_groups(step_s, steps.data(), n, perm);
+    std::vector<int64_t> ns(n);
+    for (int64_t p = 0; p < n; ++p)
+        ns[p] = steps[perm[p]];
+    ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, perm.data());
+    ctx->reserve_state(n);
+    ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+    ctx->io_b.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+    Event h0, h1, d0, d1;
+    HIPC(hipEventRecord(h0.e, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->perm.p, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->io_a.p, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->io_b.p, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    launch_permute(ctx->stream, 0, ctx->io_a.as<double>(), ctx->perm.as<int64_t>(),
+        ctx->st_r.as<double>(), n);
+    launch_permute(ctx->stream, 0, ctx->io_b.as<double>(), ctx->perm.as<int64_t>(),
+        ctx->st_v.as<double>(), n);
+    ctx->check_launch();
+    HIPC(hipEventRecord(h1.e, ctx->stream));
+    ctx->run_groups(groups, n, 0.0, nullptr, 0, ns.data());
+    HIPC(hipEventRecord(d0.e, ctx->stream));
+    launch_permute(ctx->stream, 1, ctx->st_r.as<double>(), ctx->perm.as<int64_t>(),
+        ctx->io_a.as<double>(), n);
+    launch_permute(ctx->stream, 1, ctx->st_v.as<double>(), ctx->perm.as<int64_t>(),
+        ctx->io_b.as<double>(), n);
+    ctx->check_launch();
+    HIPC(hipMemcpyAsync(r_out, ctx->io_a.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+        ctx->stream));
+    HIPC(hipMemcpyAsync(v_out, ctx->io_b.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+        ctx->stream));
+    HIPC(hipEventRecord(d1.e, ctx->stream));
+    ctx->sync();
+    float a = 0, b = 0;
+    HIPC(hipEventElapsedTime(&a, h0.e, h1.e));
+    HIPC(hipEventElapsedTime(&b, d0.e, d1.e));
+    ctx->timing.h2d_ms = a;
+    ctx->timing.d2h_ms = b;
+    record_counts(ctx, steps.data(), n);
+    if (status)
+        for (int64_t i = 0; i < n; ++i)
+            if (status[i] == OGP_OBJECT_OK && !finite6(r_out + 3 * i, v_out + 3 * i))
+                status[i] = OGP_OBJECT_NONFINITE_RESULT;
+}
+
+// The same on device-resident SoA arrays. The initial state is checked on the device: without
+// status a non-finite value fails the call before any propagation, with status the object is
+// skipped and flagged (and non-finite results are flagged after the propagation).
+void propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v0, int64_t n,
+    const double* step_s, const int64_t* steps_in, double* d_r, double* d_v, int32_t* status)
+{
+    arg_check(n == 0 || (d_r0 && d_v0 && step_s && steps_in && d_r && d_v), "null array");
+    validate_prop(step_s, steps_in, n);
+    if (n == 0)
+        return;
+    std::vector<int64_t> steps(steps_in, steps_in + n);
+    ctx->d_flags.reserve(sizeof(int32_t) * n, &ctx->dev_bytes);
+    ctx->d_count.reserve(sizeof(uint32_t), &ctx->dev_bytes);
+    const uint32_t zero = 0;
+    HIPC(hipMemcpyAsync(ctx->d_count.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
+    launch_finite_soa(ctx->stream, d_r0, d_v0, n, ctx->d_flags.as<int32_t>(),
+        ctx->d_count.as<uint32_t>(), OGP_OBJECT_NONFINITE_INPUT, true);
+    ctx->check_launch();
+    uint32_t bad = 0;
+    HIPC(hipMemcpyAsync(&bad, ctx->d_count.p, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    std::vector<int32_t> flags;
+    if (bad > 0 || status) {
+        flags.resize(n);
+        if (bad > 0)
+            HIPC(hipMemcpy(flags.data(), ctx->d_flags.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    }
+    if (bad > 0) {
+        const int64_t first = std::find_if(flags.begin(), flags.end(),
+                                  [](int32_t f) { return f != 0; }) - flags.begin();
+        if (!status)
+            throw ApiError(OGP_ERR_NONFINITE,
+                "non-finite value in the initial state of object " + std::to_string(first));
+        for (int64_t i = 0; i < n; ++i)
+            if (flags[i] != 0)
+                steps[i] = 0;
+    }
+    std::vector<int64_t> perm;
+    auto groups = plan_groups(step_s, steps.data(), n, perm);
+    std::vector<int64_t> ns(n);
+    for (int64_t p = 0; p < n; ++p)
+        ns[p] = steps[perm[p]];
+    ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, perm.data());
+    ctx->reserve_state(n);
+    HIPC(hipMemcpyAsync(ctx->perm.p, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    launch_permute(ctx->stream, 2, d_r0, ctx->perm.as<int64_t>(), ctx->st_r.as<double>(), n);
+    launch_permute(ctx->stream, 2, d_v0, ctx->perm.as<int64_t>(), ctx->st_v.as<double>(), n);
+    ctx->check_launch();
+    ctx->run_groups(groups, n, 0.0, nullptr, 0, ns.data());
+    launch_permute(ctx->stream, 3, ctx->st_r.as<double>(), ctx->perm.as<int64_t>(), d_r, n);
+    launch_permute(ctx->stream, 3, ctx->st_v.as<double>(), ctx->perm.as<int64_t>(), d_v, n);
+    ctx->check_launch();
+    if (status) {
+        launch_finite_soa(ctx->stream, d_r, d_v, n, ctx->d_flags.as<int32_t>(),
+            ctx->d_count.as<uint32_t>(), OGP_OBJECT_NONFINITE_RESULT, false);
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(status, ctx->d_flags.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+    }
+    ctx->sync();
+    record_counts(ctx, steps.data(), n);
+    ctx->timing.h2d_ms = ctx->timing.d2h_ms = 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -867,63 +1089,17 @@ int ogp_propagate(ogp_context* ctx, const double* r0, const double* v0, int64_t 
 {
     if (!ctx)
         return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] { propagate_host(ctx, r0, v0, n, step_s, steps, r_out, v_out, nullptr); });
+}
+
+int ogp_propagate_checked(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    const double* step_s, const int64_t* steps, double* r_out, double* v_out, int32_t* status)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
     return guarded(ctx, [&] {
-        arg_check(n == 0 || (r0 && v0 && step_s && steps && r_out && v_out), "null array");
-        validate_prop(step_s, steps, n);
-        if (n == 0)
-            return;
-        check_finite(r0, 3 * n, "positions");
-        check_finite(v0, 3 * n, "velocities");
-        std::vector<int64_t> perm;
-        auto groups = plan_groups(step_s, steps, n, perm);
-        std::vector<int64_t> ns(n);
-        for (int64_t p = 0; p < n; ++p)
-            ns[p] = steps[perm[p]];
-        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, perm.data());
-        ctx->reserve_state(n);
-        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
-        ctx->io_b.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
-        Event h0, h1, d0, d1;
-        HIPC(hipEventRecord(h0.e, ctx->stream));
-        HIPC(hipMemcpyAsync(ctx->perm.p, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        HIPC(hipMemcpyAsync(ctx->io_a.p, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        HIPC(hipMemcpyAsync(ctx->io_b.p, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        launch_permute(ctx->stream, 0, ctx->io_a.as<double>(), ctx->perm.as<int64_t>(),
-            ctx->st_r.as<double>(), n);
-        launch_permute(ctx->stream, 0, ctx->io_b.as<double>(), ctx->perm.as<int64_t>(),
-            ctx->st_v.as<double>(), n);
-        ctx->check_launch();
-        HIPC(hipEventRecord(h1.e, ctx->stream));
-        ctx->run_groups(groups, n, 0.0, nullptr, 0, ns.data());
-        HIPC(hipEventRecord(d0.e, ctx->stream));
-        launch_permute(ctx->stream, 1, ctx->st_r.as<double>(), ctx->perm.as<int64_t>(),
-            ctx->io_a.as<double>(), n);
-        launch_permute(ctx->stream, 1, ctx->st_v.as<double>(), ctx->perm.as<int64_t>(),
-            ctx->io_b.as<double>(), n);
-        ctx->check_launch();
-        HIPC(hipMemcpyAsync(r_out, ctx->io_a.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
-            ctx->stream));
-        HIPC(hipMemcpyAsync(v_out, ctx->io_b.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
-            ctx->stream));
-        HIPC(hipEventRecord(d1.e, ctx->stream));
-        ctx->sync();
-        float a = 0, b = 0;
-        HIPC(hipEventElapsedTime(&a, h0.e, h1.e));
-        HIPC(hipEventElapsedTime(&b, d0.e, d1.e));
-        ctx->timing.h2d_ms = a;
-        ctx->timing.d2h_ms = b;
-        double os = 0, gj = 0;
-        for (int64_t i = 0; i < n; ++i) {
-            os += (double)steps[i];
-            gj += (double)std::max<int64_t>(steps[i] - 4, 0);
-        }
-        ctx->timing.object_steps = os;
-        ctx->timing.gj_object_steps = gj;
+        arg_check(n == 0 || status, "null status array");
+        propagate_host(ctx, r0, v0, n, step_s, steps, r_out, v_out, status);
     });
 }
 
@@ -932,38 +1108,18 @@ int ogp_propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v
 {
     if (!ctx)
         return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] { propagate_device(ctx, d_r0, d_v0, n, step_s, steps, d_r, d_v, nullptr); });
+}
+
+int ogp_propagate_device_checked(ogp_context* ctx, const double* d_r0, const double* d_v0,
+    int64_t n, const double* step_s, const int64_t* steps, double* d_r, double* d_v,
+    int32_t* status)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
     return guarded(ctx, [&] {
-        arg_check(n == 0 || (d_r0 && d_v0 && step_s && steps && d_r && d_v), "null array");
-        validate_prop(step_s, steps, n);
-        if (n == 0)
-            return;
-        std::vector<int64_t> perm;
-        auto groups = plan_groups(step_s, steps, n, perm);
-        std::vector<int64_t> ns(n);
-        for (int64_t p = 0; p < n; ++p)
-            ns[p] = steps[perm[p]];
-        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, perm.data());
-        ctx->reserve_state(n);
-        HIPC(hipMemcpyAsync(ctx->perm.p, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        launch_permute(ctx->stream, 2, d_r0, ctx->perm.as<int64_t>(), ctx->st_r.as<double>(), n);
-        launch_permute(ctx->stream, 2, d_v0, ctx->perm.as<int64_t>(), ctx->st_v.as<double>(), n);
-        ctx->check_launch();
-        ctx->run_groups(groups, n, 0.0, nullptr, 0, ns.data());
-        launch_permute(ctx->stream, 3, ctx->st_r.as<double>(), ctx->perm.as<int64_t>(), d_r, n);
-        launch_permute(ctx->stream, 3, ctx->st_v.as<double>(), ctx->perm.as<int64_t>(), d_v, n);
-        ctx->check_launch();
-        ctx->sync();
-        double os = 0, gj = 0;
-        for (int64_t i = 0; i < n; ++i) {
-            os += (double)steps[i];
-            gj += (double)std::max<int64_t>(steps[i] - 4, 0);
-        }
-        ctx->timing.object_steps = os;
-        ctx->timing.gj_object_steps = gj;
-        ctx->timing.h2d_ms = ctx->timing.d2h_ms = 0;
+        arg_check(n == 0 || status, "null status array");
+        propagate_device(ctx, d_r0, d_v0, n, step_s, steps, d_r, d_v, status);
     });
 }
 
@@ -1137,85 +1293,265 @@ int ogp_last_timing(const ogp_context* ctx, ogp_timing* t)
 }
 
 // ---- stepwise propagator --------------------------------------------------------------
+} // extern "C"
+
+// GpuPropagator (gpu_propagator.cu:28-154) with its own integrator state. The caller's AoS
+// buffers r_aos / v_aos hold the state at the current step (advanced in place); the SoA state
+// (r, v, s, S, acceleration ring) and the trajectory rows of the last step live here.
+struct ogp_propagator {
+    ogp_context* ctx = nullptr; // null once the context has been destroyed
+    int64_t n = 0, step = 0, max_steps = 0;
+    double h = 0;
+    double* r_aos = nullptr;
+    double* v_aos = nullptr;
+    DevBuf own_r, own_v;                  // device buffers of a host-created propagator
+    DevBuf r, v, s, S, hist, ns, traj, bc, acc_tmp;
+    const double* bc_ptr = nullptr;
+    int64_t last_row = -1; // trajectory row of the last step (r, v, a SoA)
+
+    void release()
+    {
+        DevBuf* all[] = { &own_r, &own_v, &r, &v, &s, &S, &hist, &ns, &traj, &bc, &acc_tmp };
+        for (DevBuf* b : all)
+            b->release();
+    }
+    ~ogp_propagator()
+    {
+        release();
+        if (ctx) {
+            auto& v = ctx->props;
+            v.erase(std::remove(v.begin(), v.end(), this), v.end());
+            if (ctx->sw == this)
+                ctx->sw = nullptr;
+        }
+    }
+    PropState state()
+    {
+        PropState ps;
+        ps.r = r.as<double>();
+        ps.v = v.as<double>();
+        ps.s = s.as<double>();
+        ps.S = S.as<double>();
+        ps.hist = hist.as<double>();
+        ps.nsteps = ns.as<int64_t>();
+        ps.N = n;
+        return ps;
+    }
+};
+
+void detach_propagators(ogp_context* ctx)
+{
+    delete ctx->sw; // the context's own propagator dies with it
+    for (ogp_propagator* p : ctx->props) {
+        p->release();
+        p->ctx = nullptr;
+    }
+    ctx->props.clear();
+}
+
+namespace {
+
+ogp_propagator* new_propagator(ogp_context* ctx, int64_t n, double step_s, int64_t max_steps)
+{
+    arg_check(n > 0, "GpuPropagator needs at least one object");
+    arg_check(std::isfinite(step_s) && step_s > 0, "step size must be positive");
+    arg_check(max_steps >= 1, "max_steps must be at least 1");
+    auto p = std::make_unique<ogp_propagator>();
+    p->n = n;
+    p->h = step_s;
+    p->max_steps = max_steps;
+    int64_t* ctr = &ctx->dev_bytes;
+    const size_t v3 = sizeof(double) * 3 * n;
+    p->r.reserve(v3, ctr);
+    p->v.reserve(v3, ctr);
+    p->s.reserve(v3, ctr);
+    p->S.reserve(v3, ctr);
+    p->hist.reserve(9 * v3, ctr);
+    p->ns.reserve(sizeof(int64_t) * n, ctr);
+    p->traj.reserve(9 * 4 * v3 / 3, ctr);
+    p->acc_tmp.reserve(v3, ctr);
+    std::vector<int64_t> ns(n, max_steps);
+    HIPC(hipMemcpy(p->ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    p->bc_ptr = ctx->stage_bc(p->bc, n, nullptr);
+    p->ctx = ctx;
+    ctx->props.push_back(p.get());
+    return p.release();
+}
+
+ogp_context* live_ctx(const ogp_propagator* p)
+{
+    if (!p->ctx)
+        throw ApiError(OGP_ERR_STATE, "the propagator's context has been destroyed");
+    return p->ctx;
+}
+
+// GpuPropagator::Step (gpu_propagator.cu:28-77)
+void propagator_step(ogp_propagator& p, double* host_r, double* host_v, double* d_acc)
+{
+    ogp_context* ctx = live_ctx(&p);
+    const int64_t n = p.n, k = p.step + 1;
+    if (k > p.max_steps)
+        throw ApiError(OGP_ERR_STATE, "step beyond max_steps of the propagator");
+    NodeTable* nt = ctx->node_table(p.h, p.max_steps);
+    const int W = ctx->waves_for(n);
+    ctx->bc_obj = p.bc_ptr;
+    const ForceCtx f = ctx->force_ctx(nt, p.h);
+    const PropState ps = p.state();
+    double* traj = p.traj.as<double>();
+    hipStream_t st = ctx->stream;
+    if (k == 1) {
+        // GpuStartup::Startup reads the epoch state from the caller's buffers
+        launch_aos_to_soa(st, p.r_aos, ps.r, n);
+        launch_aos_to_soa(st, p.v_aos, ps.v, n);
+        ctx->reserve_scratch(n);
+        launch_startup(st, ctx->kind, f, ctx->rk8_variant, ps, ctx->scratch(), 0, n,
+            ctx->cfg.n_corrections, traj, 4, W);
+        ctx->check_launch();
+    }
+    if (k <= 4) {
+        p.last_row = k - 1;
+    } else {
+        launch_chunk(st, ctx->kind, f, ps, 0, n, k, k + 1, ctx->cfg.n_corrections, traj, k, 1, W);
+        ctx->check_launch();
+        p.last_row = 0;
+    }
+    const double* row = traj + p.last_row * 9 * n;
+    launch_soa_to_aos(st, row, p.r_aos, n);
+    launch_soa_to_aos(st, row + 3 * n, p.v_aos, n);
+    if (d_acc)
+        launch_soa_to_aos(st, row + 6 * n, d_acc, n);
+    ctx->check_launch();
+    ctx->sync();
+    p.step = k;
+    if (host_r)
+        HIPC(hipMemcpy(host_r, p.r_aos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (host_v)
+        HIPC(hipMemcpy(host_v, p.v_aos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+}
+
+void propagator_acceleration(const ogp_propagator& p, double* host_a)
+{
+    ogp_context* ctx = live_ctx(&p);
+    if (p.last_row < 0)
+        throw ApiError(OGP_ERR_STATE, "no step has been taken");
+    const int64_t n = p.n;
+    launch_soa_to_aos(ctx->stream, p.traj.as<double>() + (p.last_row * 9 + 6) * n,
+        p.acc_tmp.as<double>(), n);
+    ctx->check_launch();
+    ctx->sync();
+    HIPC(hipMemcpy(host_a, p.acc_tmp.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+}
+
+int fail_prop(const ogp_propagator* p, int code, const std::string& msg)
+{
+    return fail(p ? p->ctx : nullptr, code, msg);
+}
+
+template <class F> int guarded_prop(ogp_propagator* p, F&& fn)
+{
+    if (!p)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null propagator");
+    try {
+        fn();
+        return OGP_OK;
+    } catch (const ApiError& e) {
+        return fail_prop(p, e.code, e.what());
+    } catch (const std::exception& e) {
+        return fail_prop(p, OGP_ERR_DEVICE, e.what());
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int ogp_propagator_create(ogp_context* ctx, double* d_r, double* d_v, int64_t n, double step_s,
+    int64_t max_steps, ogp_propagator** out)
+{
+    if (!ctx || !out)
+        return fail(ctx, OGP_ERR_ARGUMENT, "null context or output pointer");
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        arg_check(d_r && d_v, "null state buffer");
+        ogp_propagator* p = new_propagator(ctx, n, step_s, max_steps);
+        p->r_aos = d_r;
+        p->v_aos = d_v;
+        *out = p;
+    });
+}
+
+int ogp_propagator_create_host(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
+    double step_s, int64_t max_steps, ogp_propagator** out)
+{
+    if (!ctx || !out)
+        return fail(ctx, OGP_ERR_ARGUMENT, "null context or output pointer");
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        arg_check(r0 && v0, "null state array");
+        arg_check(n > 0, "GpuPropagator needs at least one object");
+        check_finite(r0, 3 * n, "positions");
+        check_finite(v0, 3 * n, "velocities");
+        std::unique_ptr<ogp_propagator> p(new_propagator(ctx, n, step_s, max_steps));
+        p->own_r.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        p->own_v.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
+        HIPC(hipMemcpy(p->own_r.p, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(p->own_v.p, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        p->r_aos = p->own_r.as<double>();
+        p->v_aos = p->own_v.as<double>();
+        *out = p.release();
+    });
+}
+
+int ogp_propagator_step(ogp_propagator* p, double* host_r, double* host_v, double* d_acc)
+{
+    return guarded_prop(p, [&] { propagator_step(*p, host_r, host_v, d_acc); });
+}
+
+int ogp_propagator_acceleration(const ogp_propagator* p, double* host_a)
+{
+    return guarded_prop(const_cast<ogp_propagator*>(p), [&] {
+        arg_check(host_a != nullptr, "null output array");
+        propagator_acceleration(*p, host_a);
+    });
+}
+
+int ogp_propagator_copy_state_to_host(const ogp_propagator* p, double* host_r, double* host_v)
+{
+    return guarded_prop(const_cast<ogp_propagator*>(p), [&] {
+        live_ctx(p);
+        if (host_r)
+            HIPC(hipMemcpy(host_r, p->r_aos, sizeof(double) * 3 * p->n, hipMemcpyDeviceToHost));
+        if (host_v)
+            HIPC(hipMemcpy(host_v, p->v_aos, sizeof(double) * 3 * p->n, hipMemcpyDeviceToHost));
+    });
+}
+
+void ogp_propagator_destroy(ogp_propagator* p) { delete p; }
+
 int ogp_prop_init(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
     int64_t max_steps)
 {
     if (!ctx)
         return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
-    return guarded(ctx, [&] {
-        arg_check(n > 0 && r0 && v0, "GpuPropagator needs at least one object");
-        arg_check(std::isfinite(step_s) && step_s > 0, "step size must be positive");
-        arg_check(max_steps >= 1, "max_steps must be at least 1");
-        check_finite(r0, 3 * n, "positions");
-        check_finite(v0, 3 * n, "velocities");
-        ctx->reserve_state(n);
-        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
-        HIPC(hipMemcpyAsync(ctx->io_a.p, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->st_r.as<double>(), n);
-        ctx->sync();
-        HIPC(hipMemcpyAsync(ctx->io_a.p, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        launch_aos_to_soa(ctx->stream, ctx->io_a.as<double>(), ctx->st_v.as<double>(), n);
-        std::vector<int64_t> ns(n, max_steps);
-        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
-            ctx->stream));
-        ctx->sw_traj.reserve(sizeof(double) * 9 * 4 * n, &ctx->dev_bytes);
-        ctx->check_launch();
-        ctx->sync();
-        ctx->sw_bc_ptr = ctx->stage_bc(ctx->sw_bc, n, nullptr);
-        ctx->sw_n = n;
-        ctx->sw_step = 0;
-        ctx->sw_max = max_steps;
-        ctx->sw_h = step_s;
-    });
+    ogp_propagator* p = nullptr;
+    const int rc = ogp_propagator_create_host(ctx, r0, v0, n, step_s, max_steps, &p);
+    if (rc != OGP_OK)
+        return rc;
+    delete ctx->sw;
+    ctx->sw = p;
+    return OGP_OK;
 }
 
 int ogp_prop_step(ogp_context* ctx, double* r_out, double* v_out, double* a_out)
 {
     if (!ctx)
         return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
-    return guarded(ctx, [&] {
-        if (ctx->sw_n <= 0)
-            throw ApiError(OGP_ERR_STATE, "ogp_prop_step before ogp_prop_init");
-        const int64_t n = ctx->sw_n;
-        const int64_t k = ctx->sw_step + 1;
-        if (k > ctx->sw_max)
-            throw ApiError(OGP_ERR_STATE, "step beyond max_steps of ogp_prop_init");
-        NodeTable* nt = ctx->node_table(ctx->sw_h, ctx->sw_max);
-        const int W = ctx->waves_for(n);
-        ctx->bc_obj = ctx->sw_bc_ptr;
-        const ForceCtx f = ctx->force_ctx(nt, ctx->sw_h);
-        const PropState ps = ctx->prop_state(n);
-        double* traj = ctx->sw_traj.as<double>();
-        if (k == 1) {
-            launch_startup(ctx->stream, ctx->kind, f, ctx->rk8_variant, ps, ctx->scratch(), 0, n,
-                ctx->cfg.n_corrections, traj, 4, W);
-            ctx->check_launch();
-        }
-        const double* row;
-        if (k <= 4) {
-            row = traj + (k - 1) * 9 * n;
-        } else {
-            launch_chunk(ctx->stream, ctx->kind, f, ps, 0, n, k, k + 1, ctx->cfg.n_corrections,
-                traj, k, 1, W);
-            ctx->check_launch();
-            row = traj;
-        }
-        ctx->io_a.reserve(sizeof(double) * 3 * n, &ctx->dev_bytes);
-        double* outs[3] = { r_out, v_out, a_out };
-        for (int q = 0; q < 3; ++q) {
-            if (!outs[q])
-                continue;
-            launch_soa_to_aos(ctx->stream, row + 3 * q * n, ctx->io_a.as<double>(), n);
-            ctx->check_launch();
-            HIPC(hipMemcpyAsync(outs[q], ctx->io_a.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
-                ctx->stream));
-            ctx->sync();
-        }
-        ctx->sync();
-        ctx->sw_step = k;
-    });
+    if (!ctx->sw)
+        return fail(ctx, OGP_ERR_STATE, "ogp_prop_step before ogp_prop_init");
+    int rc = ogp_propagator_step(ctx->sw, r_out, v_out, nullptr);
+    if (rc == OGP_OK && a_out)
+        rc = ogp_propagator_acceleration(ctx->sw, a_out);
+    return rc;
 }
 
 int ogp_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n, double step_s,
@@ -1258,6 +1594,74 @@ int ogp_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
                     for (int c = 0; c < 3; ++c)
                         outs[q][27 * i + 3 * sl + c] = host[(sl * 3 + c) * n + i];
         }
+    });
+}
+
+int ogp_startup_device(ogp_context* ctx, const double* d_r0, const double* d_v0, int64_t n,
+    double step_s, double* d_pos, double* d_vel, double* d_s, double* d_S, double* d_acc)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n > 0 && d_r0 && d_v0 && std::isfinite(step_s) && step_s > 0,
+            "invalid startup arguments");
+        ctx->reserve_state(n);
+        ctx->d_flags.reserve(sizeof(int32_t) * n, &ctx->dev_bytes);
+        ctx->d_count.reserve(sizeof(uint32_t), &ctx->dev_bytes);
+        launch_aos_to_soa(ctx->stream, d_r0, ctx->st_r.as<double>(), n);
+        launch_aos_to_soa(ctx->stream, d_v0, ctx->st_v.as<double>(), n);
+        const uint32_t zero = 0;
+        uint32_t bad = 0;
+        HIPC(hipMemcpyAsync(ctx->d_count.p, &zero, sizeof(zero), hipMemcpyHostToDevice,
+            ctx->stream));
+        launch_finite_soa(ctx->stream, ctx->st_r.as<double>(), ctx->st_v.as<double>(), n,
+            ctx->d_flags.as<int32_t>(), ctx->d_count.as<uint32_t>(), 1, true);
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(&bad, ctx->d_count.p, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        if (bad > 0)
+            throw ApiError(OGP_ERR_NONFINITE, "non-finite value in the initial state of "
+                    + std::to_string(bad) + " object(s)");
+        std::vector<int64_t> ns(n, 4);
+        HIPC(hipMemcpyAsync(ctx->st_ns.p, ns.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+            ctx->stream));
+        NodeTable* nt = ctx->node_table(step_s, 4);
+        const int W = ctx->waves_for(n);
+        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, nullptr);
+        launch_startup(ctx->stream, ctx->kind, ctx->force_ctx(nt, step_s), ctx->rk8_variant,
+            ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0, W);
+        // scratch [slot][c][obj] -> caller [obj][slot][c]
+        double* outs[5] = { d_pos, d_vel, d_acc, d_s, d_S };
+        for (int q = 0; q < 5; ++q)
+            if (outs[q])
+                launch_slots_to_aos(ctx->stream, ctx->sc[q].as<double>(), outs[q], n, 9);
+        ctx->check_launch();
+        ctx->sync();
+    });
+}
+
+int ogp_predict_device(ogp_context* ctx, double* d_r, double* d_v, double* d_acc,
+    const double* d_s, double* d_S, int64_t n, double step_s, int64_t step_num)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        arg_check(n >= 0 && std::isfinite(step_s) && step_s > 0 && step_num >= 0,
+            "invalid predictor arguments");
+        arg_check(n == 0 || (d_r && d_v && d_acc && d_s && d_S), "null array");
+        if (ctx->cfg.force != OGP_FORCE_GRAVITY)
+            throw ApiError(OGP_ERR_STATE, "context was created without the gravity field");
+        if (n == 0)
+            return;
+        ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, nullptr);
+        NodeTable* nt = ctx->node_table(step_s, step_num);
+        const int64_t node = step_num + 4;
+        launch_predict_aos(ctx->stream, d_acc, d_s, d_S, n, step_s, d_r, d_v);
+        launch_gravity_aos(ctx->stream, ctx->kind == kGravitySplit, ctx->gdev,
+            nt->prop_M.as<double>() + 9 * node, d_r, n, d_acc + 24, 27, 64, ctx->sources_dev(),
+            nt->prop_tb.as<double>() + 6 * node, d_v);
+        ctx->check_launch();
+        ctx->sync();
     });
 }
 

@@ -346,7 +346,10 @@ void launch_gravity_aos(hipStream_t st, bool split, const GravityDev& g, const d
 // Known-answer hooks: the predictor and corrector arithmetic of gj8_chunk_kernel on given
 // inputs (AoS [n][27] histories, [n][3] vectors).
 // =========================================================================================
-__global__ void kat_predict_kernel(const double* acc27, const double* s3, double* S3, int64_t n,
+// SHIFT: Predictor::Predict's MoveAccelerationsKernel too (prediction.cu:63-65,
+// dev_functions.cu:20-32): columns 0-7 of the history take columns 1-8 after the sums.
+template <bool SHIFT>
+__global__ void kat_predict_kernel(double* acc27, const double* s3, double* S3, int64_t n,
     double h, double* r3, double* v3)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,6 +369,10 @@ __global__ void kat_predict_kernel(const double* acc27, const double* s3, double
         v3[3 * i + c] = ((Pv + s) + alast / 2) * h;
         r3[3 * i + c] = ((Pr + S) * h) * h;
     }
+    if (SHIFT)
+        for (int q = 0; q < 8; ++q)
+            for (int c = 0; c < 3; ++c)
+                acc27[27 * i + 3 * q + c] = acc27[27 * i + 3 * (q + 1) + c];
 }
 
 __global__ void kat_finalize_kernel(double* r3, double* v3, const double* acc27, const double* s3,
@@ -384,8 +391,16 @@ __global__ void kat_finalize_kernel(double* r3, double* v3, const double* acc27,
 void launch_kat_predict(hipStream_t st, const double* acc27, const double* s3, double* S3,
     int64_t n, double h, double* r3, double* v3)
 {
-    hipLaunchKernelGGL(kat_predict_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
-        acc27, s3, S3, n, h, r3, v3);
+    hipLaunchKernelGGL(kat_predict_kernel<false>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
+        const_cast<double*>(acc27), s3, S3, n, h, r3, v3);
+}
+
+void launch_predict_aos(hipStream_t st, double* acc27, const double* s3, double* S3, int64_t n,
+    double h, double* r3, double* v3)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(kat_predict_kernel<true>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0,
+            st, acc27, s3, S3, n, h, r3, v3);
 }
 
 void launch_kat_finalize(hipStream_t st, double* r3, double* v3, const double* acc27,
@@ -478,6 +493,46 @@ void launch_aos_to_soa(hipStream_t st, const double* aos, double* soa, int64_t n
     if (n > 0)
         hipLaunchKernelGGL(aos_to_soa_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
             aos, soa, n);
+}
+
+__global__ void slots_to_aos_kernel(const double* src, double* dst, int64_t n, int nslots)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int q = 0; q < nslots; ++q)
+        for (int c = 0; c < 3; ++c)
+            dst[(i * nslots + q) * 3 + c] = src[(q * 3 + c) * n + i];
+}
+
+void launch_slots_to_aos(hipStream_t st, const double* src, double* dst, int64_t n, int nslots)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(slots_to_aos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+            st, src, dst, n, nslots);
+}
+
+__global__ void finite_soa_kernel(const double* r, const double* v, int64_t n, int32_t* flags,
+    uint32_t* count, int32_t code, int fresh)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    bool ok = true;
+    for (int c = 0; c < 3; ++c)
+        ok = ok && isfinite(r[c * n + i]) && isfinite(v[c * n + i]);
+    const int32_t prev = fresh ? 0 : flags[i];
+    if (!ok && prev == 0)
+        atomicAdd(count, 1u);
+    flags[i] = !ok && prev == 0 ? code : prev;
+}
+
+void launch_finite_soa(hipStream_t st, const double* r, const double* v, int64_t n, int32_t* flags,
+    uint32_t* count, int32_t code, bool fresh)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(finite_soa_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+            r, v, n, flags, count, code, fresh ? 1 : 0);
 }
 
 void launch_soa_to_aos(hipStream_t st, const double* soa, double* aos, int64_t n)

@@ -91,6 +91,18 @@ void launch_kat_predict(hipStream_t st, const double* acc27, const double* s3, d
     int64_t n, double h, double* r3, double* v3);
 void launch_kat_finalize(hipStream_t st, double* r3, double* v3, const double* acc27,
     const double* s3, const double* S3, int64_t n, double h);
+// Predictor::Predict without its force evaluation (prediction.cu:49-68): the predicted r, v and S
+// of AoS [n][27] histories, then the history shifted by one column (column 8 is left for the new
+// acceleration).
+void launch_predict_aos(hipStream_t st, double* acc27, const double* s3, double* S3, int64_t n,
+    double h, double* r3, double* v3);
+// [slot][component][object] (n objects) -> [object][slot][component]
+void launch_slots_to_aos(hipStream_t st, const double* src, double* dst, int64_t n, int nslots);
+// flags[i] = code for every object i whose SoA r or v (stride n) holds a non-finite value and
+// whose flag is still 0 (fresh: every flag is taken as 0 and rewritten); *count counts the
+// objects newly flagged.
+void launch_finite_soa(hipStream_t st, const double* r, const double* v, int64_t n, int32_t* flags,
+    uint32_t* count, int32_t code, bool fresh);
 
 // Permutations between the caller's order and the internal (class-sorted) order.
 // mode 0: AoS src -> SoA dst gather (dst[p] = src[perm[p]]); 1: SoA -> AoS scatter;

@@ -97,16 +97,71 @@ class Context:
                                           ptr(v)))
         return r, v
 
-    def propagate_device(self, r0_soa, v0_soa, n, step, steps, r_soa, v_soa):
+    def propagate_checked(self, r0, v0, step, steps):
+        """ogp_propagate_checked: (r, v, status) with per-object flags (OBJECT_*) instead of a
+        failed call for non-finite states."""
+        r0 = f64(r0).reshape(-1, 3)
+        v0 = f64(v0).reshape(-1, 3)
+        n = len(r0)
+        h = np.ascontiguousarray(np.broadcast_to(np.asarray(step, np.float64), (n,)))
+        ns = np.ascontiguousarray(np.broadcast_to(np.asarray(steps, np.int64), (n,)))
+        r = np.zeros_like(r0)
+        v = np.zeros_like(v0)
+        st = np.zeros(n, np.int32)
+        check(self.h, lib().ogp_propagate_checked(self.h, ptr(r0), ptr(v0), n, ptr(h), ptr(ns),
+                                                  ptr(r), ptr(v), ptr(st)))
+        return r, v, st
+
+    def propagate_device(self, r0_soa, v0_soa, n, step, steps, r_soa, v_soa, status=False):
         """ogp_propagate_device on device-resident SoA arrays (x[n], y[n], z[n] blocks).
         Array arguments are device addresses: ints or objects with data_ptr() (torch tensors).
-        step and steps are host values (scalars or per-object arrays)."""
+        step and steps are host values (scalars or per-object arrays). With status=True the
+        checked entry point runs and the per-object flags are returned."""
         addr = [a.data_ptr() if hasattr(a, "data_ptr") else int(a)
                 for a in (r0_soa, v0_soa, r_soa, v_soa)]
         h = np.ascontiguousarray(np.broadcast_to(np.asarray(step, np.float64), (n,)))
         ns = np.ascontiguousarray(np.broadcast_to(np.asarray(steps, np.int64), (n,)))
+        if status:
+            st = np.zeros(n, np.int32)
+            check(self.h, lib().ogp_propagate_device_checked(self.h, addr[0], addr[1], n, ptr(h),
+                                                             ptr(ns), addr[2], addr[3], ptr(st)))
+            return st
         check(self.h, lib().ogp_propagate_device(self.h, addr[0], addr[1], n, ptr(h), ptr(ns),
                                                  addr[2], addr[3]))
+        return None
+
+    # -- device-pointer forms of the force model, startup and predictor -------------------
+    @staticmethod
+    def _addr(a):
+        if a is None:
+            return None
+        return a.data_ptr() if hasattr(a, "data_ptr") else int(a)
+
+    def accel_device(self, d_r, d_v, n, step, node, d_a, out_stride=3):
+        """ForceModel::ComputePropagationAcceleration on device AoS arrays."""
+        A = self._addr
+        check(self.h, lib().ogp_accel_device(self.h, A(d_r), A(d_v), n, step, node, A(d_a),
+                                             out_stride))
+
+    def accel_generation(self, d_r, d_v, n, step, step_num, k_index, d_K):
+        """ForceModel::ComputeGenerationAcceleration: column k_index of the device K matrix
+        [n][10][3] at RK8 stage k_index of startup step step_num."""
+        A = self._addr
+        check(self.h, lib().ogp_accel_generation(self.h, A(d_r), A(d_v), n, step, step_num,
+                                                 k_index, A(d_K)))
+
+    def startup_device(self, d_r0, d_v0, n, step, d_pos=None, d_vel=None, d_s=None, d_S=None,
+                       d_acc=None):
+        """GpuStartup::Startup on device arrays: outputs [n][9][3] (each optional)."""
+        A = self._addr
+        check(self.h, lib().ogp_startup_device(self.h, A(d_r0), A(d_v0), n, step, A(d_pos),
+                                               A(d_vel), A(d_s), A(d_S), A(d_acc)))
+
+    def predict_device(self, d_r, d_v, d_acc, d_s, d_S, n, step, step_num):
+        """Predictor::Predict(step_num) on the device arrays of Predictor::SetPointers."""
+        A = self._addr
+        check(self.h, lib().ogp_predict_device(self.h, A(d_r), A(d_v), A(d_acc), A(d_s), A(d_S),
+                                               n, step, step_num))
 
     def perturbed_satellites(self, r0, v0, n, strength, seed=1234, obj_r=None, obj_v=None):
         """PerturbationGenerator::generatePerturbedSatellites: (r, v) host arrays [n (+1)][3]
@@ -269,21 +324,55 @@ class Context:
 
 
 class GpuPropagator:
-    """Stepwise propagator (GpuPropagator::Step, gpu_propagator.cu:28-77)."""
+    """Stepwise propagator (GpuPropagator, gpu_propagator.cu:28-154) with its own state: any
+    number of them may share a context. From host arrays (the propagator owns its device state),
+    or with device=True from the caller's AoS [n][3] device buffers r0, v0 (ints or objects with
+    data_ptr(), e.g. torch tensors), which every step advances in place."""
 
-    def __init__(self, ctx, r0, v0, step, max_steps):
+    def __init__(self, ctx, r0, v0, step, max_steps, n=None, device=False):
         self.ctx = ctx
-        r0 = f64(r0).reshape(-1, 3)
-        v0 = f64(v0).reshape(-1, 3)
-        self.n = len(r0)
-        check(ctx.h, lib().ogp_prop_init(ctx.h, ptr(r0), ptr(v0), self.n, step, max_steps))
+        h = ctypes.c_void_p()
+        if device:
+            self.n = int(n)
+            addr = [a.data_ptr() if hasattr(a, "data_ptr") else int(a) for a in (r0, v0)]
+            check(ctx.h, lib().ogp_propagator_create(ctx.h, addr[0], addr[1], self.n, step,
+                                                     max_steps, ctypes.byref(h)))
+        else:
+            r0 = f64(r0).reshape(-1, 3)
+            v0 = f64(v0).reshape(-1, 3)
+            self.n = len(r0)
+            check(ctx.h, lib().ogp_propagator_create_host(ctx.h, ptr(r0), ptr(v0), self.n, step,
+                                                          max_steps, ctypes.byref(h)))
+        self.h = h
 
-    def step(self):
+    def step(self, d_acc=None):
+        """One GpuPropagator::Step. Returns (r, v, a) as host arrays; with d_acc (a device
+        address) the acceleration column is also written there, as the reference does."""
         r = np.zeros((self.n, 3))
         v = np.zeros((self.n, 3))
         a = np.zeros((self.n, 3))
-        check(self.ctx.h, lib().ogp_prop_step(self.ctx.h, ptr(r), ptr(v), ptr(a)))
+        dacc = None if d_acc is None else (d_acc.data_ptr() if hasattr(d_acc, "data_ptr")
+                                           else int(d_acc))
+        check(self.ctx.h, lib().ogp_propagator_step(self.h, ptr(r), ptr(v), dacc))
+        check(self.ctx.h, lib().ogp_propagator_acceleration(self.h, ptr(a)))
         return r, v, a
+
+    def copy_state_to_host(self):
+        r = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        check(self.ctx.h, lib().ogp_propagator_copy_state_to_host(self.h, ptr(r), ptr(v)))
+        return r, v
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ogp_propagator_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ClosestApproach:
