@@ -309,6 +309,20 @@ int ogp_conj_find(ogp_context* ctx, double threshold_m, int64_t* count);
  * sampled minimum step, refined miss distance, TCA (s from the epoch) and positions at TCA. */
 int ogp_conj_results(ogp_context* ctx, uint32_t* obj_a, uint32_t* obj_b, int32_t* step,
     double* distance, double* time, double* pos_a, double* pos_b);
+/* Allocation-free form of ogp_conj_find with threshold_m > 0 (no counterpart in the reference;
+ * for callers that cannot allocate device memory: tight serving loops, code that must not call
+ * hipMalloc). It calls no device allocator: the candidate counter, the candidate queue and the
+ * refinement results live in a file-scope device array of the library (5.25 MiB per device,
+ * held until the process ends), which limits a call to OGP_NOALLOC_MAX_CANDIDATES candidates.
+ * The records (as ogp_conj_results gives them, sorted by pair) go straight into the caller's
+ * host arrays of `capacity` entries (pos_a, pos_b: 3 per entry); *count receives the number of
+ * candidates, also when it exceeds the capacity or the limit, which is an OGP_ERR_ARGUMENT.
+ * The trajectory is the one ogp_conj_propagate stored. Every call on a device shares that
+ * device's array: calls that overlap on one device (from several threads or contexts) race. */
+#define OGP_NOALLOC_MAX_CANDIDATES 65536
+int ogp_conj_find_noalloc(ogp_context* ctx, double threshold_m, int64_t capacity,
+    uint32_t* obj_a, uint32_t* obj_b, int32_t* step, double* distance, double* time,
+    double* pos_a, double* pos_b, int64_t* count);
 /* Sampled screening minimum of every cross-set pair (pair index order). */
 int ogp_conj_screen(ogp_context* ctx, double* min_dist, int32_t* min_step);
 int ogp_conj_num_pairs(const ogp_context* ctx, int64_t* num_pairs);

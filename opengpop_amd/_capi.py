@@ -129,6 +129,8 @@ def lib():
         "ogp_conj_propagate": (ctypes.c_int, [_P, _I64]),
         "ogp_conj_find": (ctypes.c_int, [_P, _D, ctypes.POINTER(_I64)]),
         "ogp_conj_results": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+        "ogp_conj_find_noalloc": (ctypes.c_int, [_P, _D, _I64, _P, _P, _P, _P, _P, _P, _P,
+                                                 ctypes.POINTER(_I64)]),
         "ogp_conj_screen": (ctypes.c_int, [_P, _P, _P]),
         "ogp_conj_num_pairs": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
         "ogp_conj_trajectory": (ctypes.c_int, [_P, _P, ctypes.POINTER(_I64)]),

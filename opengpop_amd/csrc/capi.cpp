@@ -20,6 +20,9 @@
 
 using namespace ogp;
 
+static_assert(kNoAllocMaxCandidates == OGP_NOALLOC_MAX_CANDIDATES,
+    "the header's limit is the scratch array's");
+
 namespace {
 
 thread_local std::string g_error;
@@ -189,6 +192,7 @@ struct ogp_context {
     }
 
     int simds = 1024; // SIMDs of the device (4 per CU), read at creation
+    void* noalloc = nullptr; // device address of the allocation-free search's scratch array
 
     // Waves per 64-object group: the override, else the tuned value for this size, else enough
     // waves for about 1.2 resident per SIMD, which hides most of the latency of the scalar
@@ -665,6 +669,7 @@ int ogp_create(const ogp_config* cfg, ogp_context** out)
         if (ndev < 1)
             throw ApiError(OGP_ERR_DEVICE, "no HIP device");
         HIPC(hipSetDevice(0));
+        hip_check(noalloc_scratch_setup(0, &c.noalloc), "hipGetSymbolAddress(scratch)");
         int cus = 0;
         HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
         c.simds = std::max(4, 4 * cus);
@@ -1861,7 +1866,7 @@ int64_t conj_screen(ogp_context* ctx, double threshold, bool all_pairs)
         ctx->cj_b.reserve(sizeof(uint32_t) * cap, &ctx->dev_bytes);
         ctx->cj_k.reserve(sizeof(int32_t) * cap, &ctx->dev_bytes);
         ctx->cj_d.reserve(sizeof(double) * cap, &ctx->dev_bytes);
-        const uint32_t zero = 0;
+        const unsigned long long zero = 0;
         HIPC(hipMemcpyAsync(ctx->cj_cnt.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
         int64_t pair0 = 0;
         for (size_t s = 0; s + 1 < off.size(); ++s) {
@@ -1869,19 +1874,20 @@ int64_t conj_screen(ogp_context* ctx, double threshold, bool all_pairs)
             const int64_t b0 = off[s + 1], nb = n - off[s + 1];
             launch_screen(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps, a0, na, b0, nb, pair0,
                 all_pairs ? 0.0 : threshold, all_pairs ? ctx->cj_mind.as<double>() : nullptr,
-                all_pairs ? ctx->cj_mink.as<int32_t>() : nullptr, ctx->cj_cnt.as<uint32_t>(), cap,
+                all_pairs ? ctx->cj_mink.as<int32_t>() : nullptr,
+                ctx->cj_cnt.as<unsigned long long>(), cap,
                 ctx->cj_a.as<uint32_t>(), ctx->cj_b.as<uint32_t>(), ctx->cj_k.as<int32_t>(),
                 ctx->cj_d.as<double>());
             ctx->check_launch();
             pair0 += na * nb;
         }
-        uint32_t cnt = 0;
+        unsigned long long cnt = 0;
         HIPC(hipMemcpyAsync(&cnt, ctx->cj_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
         ctx->sync();
         if (all_pairs)
             return np;
-        if ((int64_t)cnt <= cap)
-            return cnt;
+        if (cnt <= (unsigned long long)cap)
+            return (int64_t)cnt;
         cap = (int64_t)cnt; // too many candidates for the buffer: grow once and rescreen
     }
     throw ApiError(OGP_ERR_DEVICE, "candidate buffer overflow");
@@ -1992,6 +1998,107 @@ int ogp_conj_find(ogp_context* ctx, double threshold_m, int64_t* count)
         ctx->cj_timing.pair_steps = (double)np * (double)ctx->cj_steps;
         ctx->cj_timing.candidates = (double)nc;
         *count = nc;
+    });
+}
+
+int ogp_conj_find_noalloc(ogp_context* ctx, double threshold_m, int64_t capacity, uint32_t* a,
+    uint32_t* b, int32_t* step, double* distance, double* time, double* pos_a, double* pos_b,
+    int64_t* count)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    return guarded(ctx, [&] {
+        if (ctx->cj_steps <= 0)
+            throw ApiError(OGP_ERR_STATE, "FindClosest before Propagate");
+        arg_check(count != nullptr, "null count");
+        arg_check(std::isfinite(threshold_m) && threshold_m > 0, "threshold must be positive");
+        arg_check(capacity >= 0, "negative capacity");
+        arg_check(capacity == 0 || (a && b && step && distance && time && pos_a && pos_b),
+            "null output array");
+        if (!ctx->noalloc)
+            throw ApiError(OGP_ERR_DEVICE, "scratch array not set up on this device");
+        char* base = static_cast<char*>(ctx->noalloc);
+        auto* counter = reinterpret_cast<unsigned long long*>(base + kNoAllocCounter);
+        auto* ca = reinterpret_cast<uint32_t*>(base + kNoAllocA);
+        auto* cb = reinterpret_cast<uint32_t*>(base + kNoAllocB);
+        auto* ck = reinterpret_cast<int32_t*>(base + kNoAllocK);
+        auto* cd = reinterpret_cast<double*>(base + kNoAllocD);
+        auto* out = reinterpret_cast<double*>(base + kNoAllocOut);
+        const int64_t n = ctx->cj_n, nsteps = ctx->cj_steps;
+        const auto& off = ctx->cj_set_off;
+        launch_zero_counter(ctx->stream, counter);
+        int64_t pair0 = 0;
+        for (size_t s = 0; s + 1 < off.size(); ++s) {
+            const int64_t a0 = off[s], na = off[s + 1] - off[s];
+            const int64_t b0 = off[s + 1], nb = n - off[s + 1];
+            launch_screen(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps, a0, na, b0, nb, pair0,
+                threshold_m, nullptr, nullptr, counter, kNoAllocMaxCandidates, ca, cb, ck, cd);
+            pair0 += na * nb;
+        }
+        ctx->check_launch();
+        unsigned long long cnt = 0;
+        HIPC(hipMemcpyAsync(&cnt, counter, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        *count = (int64_t)cnt;
+        if (cnt > (unsigned long long)kNoAllocMaxCandidates)
+            throw ApiError(OGP_ERR_ARGUMENT, std::to_string(cnt) + " candidates exceed the "
+                    + std::to_string(kNoAllocMaxCandidates) + " of the allocation-free search "
+                    "(lower the threshold, or use ogp_conj_find)");
+        if ((int64_t)cnt > capacity)
+            throw ApiError(OGP_ERR_ARGUMENT, std::to_string(cnt) + " candidates exceed the "
+                    "capacity of the output arrays (" + std::to_string(capacity) + ")");
+        const int64_t nc = (int64_t)cnt;
+        if (nc == 0)
+            return;
+        launch_refine(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps, ctx->cj_h, 0.0, nc, ca,
+            cb, ck, cd, out, kNoAllocMaxCandidates);
+        ctx->check_launch();
+        const int64_t S = kNoAllocMaxCandidates;
+        HIPC(hipMemcpyAsync(a, ca, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(b, cb, sizeof(uint32_t) * nc, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(step, ck, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(distance, out, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(time, out + S, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(pos_a, out + 2 * S, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost,
+            ctx->stream));
+        HIPC(hipMemcpyAsync(pos_b, out + 5 * S, sizeof(double) * 3 * nc, hipMemcpyDeviceToHost,
+            ctx->stream));
+        ctx->sync();
+        // order by pair (a, b) in place (the queue is filled by atomics in no fixed order):
+        // heapsort over the caller's arrays, which needs no memory of its own
+        auto less = [&](int64_t x, int64_t y) {
+            return a[x] != a[y] ? a[x] < a[y] : b[x] < b[y];
+        };
+        auto swap = [&](int64_t x, int64_t y) {
+            std::swap(a[x], a[y]);
+            std::swap(b[x], b[y]);
+            std::swap(step[x], step[y]);
+            std::swap(distance[x], distance[y]);
+            std::swap(time[x], time[y]);
+            for (int c = 0; c < 3; ++c) {
+                std::swap(pos_a[3 * x + c], pos_a[3 * y + c]);
+                std::swap(pos_b[3 * x + c], pos_b[3 * y + c]);
+            }
+        };
+        auto sift = [&](int64_t root, int64_t end) {
+            for (;;) {
+                int64_t child = 2 * root + 1;
+                if (child >= end)
+                    return;
+                if (child + 1 < end && less(child, child + 1))
+                    ++child;
+                if (!less(root, child))
+                    return;
+                swap(root, child);
+                root = child;
+            }
+        };
+        for (int64_t i = nc / 2 - 1; i >= 0; --i)
+            sift(i, nc);
+        for (int64_t end = nc - 1; end > 0; --end) {
+            swap(0, end);
+            sift(0, end);
+        }
     });
 }
 

@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "conjunction.hpp"
 
 namespace ogp {
@@ -92,7 +94,7 @@ __attribute__((amdgpu_waves_per_eu(OGP_SCREEN_WAVES_PER_EU)))
 screen_kernel(const double* __restrict__ traj,
     int64_t N, int64_t nsteps, int64_t a0, int64_t na, int64_t b0, int64_t nb, int64_t pair0,
     int64_t row_len, double threshold, double* __restrict__ min_dist, int32_t* __restrict__ min_step,
-    uint32_t* __restrict__ cand_count, int64_t cand_cap, uint32_t* __restrict__ cand_a,
+    unsigned long long* __restrict__ cand_count, int64_t cand_cap, uint32_t* __restrict__ cand_a,
     uint32_t* __restrict__ cand_b, int32_t* __restrict__ cand_k, double* __restrict__ cand_d)
 {
     constexpr int SPAN = STAGE * WIN; // samples per decision window
@@ -244,8 +246,8 @@ screen_kernel(const double* __restrict__ traj,
                 min_step[p] = k;
             }
             if (cand) {
-                const uint32_t slot = atomicAdd(cand_count, 1u);
-                if (slot < cand_cap) {
+                const unsigned long long slot = atomicAdd(cand_count, 1ull);
+                if (slot < (unsigned long long)cand_cap) {
                     cand_a[slot] = (uint32_t)ia;
                     cand_b[slot] = (uint32_t)jb;
                     cand_k[slot] = k;
@@ -257,7 +259,7 @@ screen_kernel(const double* __restrict__ traj,
 
 void launch_screen(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, int64_t a0,
     int64_t na, int64_t b0, int64_t nb, int64_t pair0, double threshold, double* min_dist,
-    int32_t* min_step, uint32_t* cand_count, int64_t cand_cap, uint32_t* cand_a,
+    int32_t* min_step, unsigned long long* cand_count, int64_t cand_cap, uint32_t* cand_a,
     uint32_t* cand_b, int32_t* cand_k, double* cand_d)
 {
     if (na <= 0 || nb <= 0 || nsteps <= 0)
@@ -337,24 +339,35 @@ __device__ double approach(const Hermite* A, const Hermite* B, double t)
 
 } // namespace
 
+// Output of candidate q: distance, time, position of a, position of b. field_stride == 0: eight
+// consecutive doubles per candidate; field_stride = S > 0: field blocks (distance[S], time[S],
+// pos_a[S][3], pos_b[S][3]).
+struct RefineOut {
+    double* out;
+    int64_t S;
+    __device__ double& dist(int64_t q) const { return S ? out[q] : out[8 * q]; }
+    __device__ double& time(int64_t q) const { return S ? out[S + q] : out[8 * q + 1]; }
+    __device__ double& pa(int64_t q, int c) const { return S ? out[2 * S + 3 * q + c] : out[8 * q + 2 + c]; }
+    __device__ double& pb(int64_t q, int c) const { return S ? out[5 * S + 3 * q + c] : out[8 * q + 5 + c]; }
+};
+
 __global__ void refine_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps, double h,
     double t0, int64_t n, const uint32_t* __restrict__ ia, const uint32_t* __restrict__ ib,
-    const int32_t* __restrict__ ik, const double* __restrict__ id, double* __restrict__ out)
+    const int32_t* __restrict__ ik, const double* __restrict__ id, RefineOut o)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n)
         return;
     const int64_t a = ia[q], b = ib[q];
     const int64_t k = ik[q];
-    double* o = out + 8 * q; // distance, time, posA[3], posB[3]
     if (k <= 0 || k >= nsteps - 1) {
         // The reference reports time 0 (first sample) or total * h (last sample) and leaves
         // the positions unset; the sampled positions are returned here.
-        o[0] = id[q];
-        o[1] = t0 + (k <= 0 ? 0.0 : (double)nsteps * h);
+        o.dist(q) = id[q];
+        o.time(q) = t0 + (k <= 0 ? 0.0 : (double)nsteps * h);
         for (int c = 0; c < 3; ++c) {
-            o[2 + c] = traj[((k < 0 ? 0 : k) * 9 + c) * N + a];
-            o[5 + c] = traj[((k < 0 ? 0 : k) * 9 + c) * N + b];
+            o.pa(q, c) = traj[((k < 0 ? 0 : k) * 9 + c) * N + a];
+            o.pb(q, c) = traj[((k < 0 ? 0 : k) * 9 + c) * N + b];
         }
         return;
     }
@@ -398,22 +411,61 @@ __global__ void refine_kernel(const double* __restrict__ traj, int64_t N, int64_
         pb[c] = HB[c].val(t);
     }
     const double dx = pa[0] - pb[0], dy = pa[1] - pb[1], dz = pa[2] - pb[2];
-    o[0] = sqrt(dx * dx + dy * dy + dz * dz);
-    o[1] = t0 + t + (double)k * h;
+    o.dist(q) = sqrt(dx * dx + dy * dy + dz * dz);
+    o.time(q) = t0 + t + (double)k * h;
     for (int c = 0; c < 3; ++c) {
-        o[2 + c] = pa[c];
-        o[5 + c] = pb[c];
+        o.pa(q, c) = pa[c];
+        o.pb(q, c) = pb[c];
     }
 }
 
 void launch_refine(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, double h,
     double t0, int64_t n, const uint32_t* a, const uint32_t* b, const int32_t* k, const double* d,
-    double* out)
+    double* out, int64_t field_stride)
 {
     if (n <= 0)
         return;
     hipLaunchKernelGGL(refine_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, traj, N,
-        nsteps, h, t0, n, a, b, k, d, out);
+        nsteps, h, t0, n, a, b, k, d, RefineOut { out, field_stride });
+}
+
+// ------------------------------------------------------------------------------------------
+// Scratch of the allocation-free candidate search (ogp_conj_find_noalloc): the candidate
+// counter, the candidate queue and the refinement results in one file-scope device array.
+// Host code never passes `scratch` itself to a kernel (in host code the name is only a stand-in
+// for the symbol): noalloc_scratch_setup asks the runtime for its device address, once per
+// device, and the launches take that address.
+// ------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void zero_counter_kernel(unsigned long long* counter) { counter[threadIdx.x] = 0; }
+} // namespace
+
+hipError_t noalloc_scratch_setup(int device, void** out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+void launch_zero_counter(hipStream_t st, unsigned long long* counter)
+{
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, st, counter);
 }
 
 } // namespace ogp

@@ -12,13 +12,31 @@ namespace ogp {
 // candidate arrays (at most cand_cap; *cand_count counts all).
 void launch_screen(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, int64_t a0,
     int64_t na, int64_t b0, int64_t nb, int64_t pair0, double threshold, double* min_dist,
-    int32_t* min_step, uint32_t* cand_count, int64_t cand_cap, uint32_t* cand_a,
+    int32_t* min_step, unsigned long long* cand_count, int64_t cand_cap, uint32_t* cand_a,
     uint32_t* cand_b, int32_t* cand_k, double* cand_d);
 
-// Refines n candidates (a, b, sampled minimum step k and distance d). out: 8 doubles per
-// candidate = distance, time (t0 + seconds), position of a, position of b at the TCA.
+// Refines n candidates (a, b, sampled minimum step k and distance d). out: distance, time
+// (t0 + seconds), position of a, position of b at the TCA; field_stride = 0: 8 doubles per
+// candidate; field_stride = S: blocks distance[S], time[S], pos_a[S][3], pos_b[S][3].
 void launch_refine(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, double h,
     double t0, int64_t n, const uint32_t* a, const uint32_t* b, const int32_t* k, const double* d,
-    double* out);
+    double* out, int64_t field_stride = 0);
+
+// Allocation-free candidate search (ogp_conj_find_noalloc): one file-scope device array per
+// device holds, at these byte offsets, the candidate counter, kNoAllocMaxCandidates candidates
+// (a, b, step, sampled distance) and their refinement results (field blocks of that stride).
+constexpr int64_t kNoAllocMaxCandidates = 1 << 16;
+constexpr size_t kNoAllocCounter = 0;
+constexpr size_t kNoAllocA = 256;
+constexpr size_t kNoAllocB = kNoAllocA + 4 * kNoAllocMaxCandidates;
+constexpr size_t kNoAllocK = kNoAllocB + 4 * kNoAllocMaxCandidates;
+constexpr size_t kNoAllocD = kNoAllocK + 4 * kNoAllocMaxCandidates;
+constexpr size_t kNoAllocOut = kNoAllocD + 8 * kNoAllocMaxCandidates;
+constexpr size_t kNoAllocBytes = kNoAllocOut + 8 * 8 * kNoAllocMaxCandidates;
+static_assert(kNoAllocD % 8 == 0 && kNoAllocOut % 8 == 0, "8-byte fields");
+// The array's device address on `device` (which must be the current device): looked up with
+// hipGetSymbolAddress on the first call for that device and kept.
+hipError_t noalloc_scratch_setup(int device, void** out);
+void launch_zero_counter(hipStream_t st, unsigned long long* counter);
 
 } // namespace ogp

@@ -409,6 +409,23 @@ class ClosestApproach:
                                                  ptr(t), ptr(pa), ptr(pb)))
         return ClosestApproachData(a, b, k, d, t, pa, pb)
 
+    def find_closest_noalloc(self, threshold, capacity=65536):
+        """ogp_conj_find_noalloc: the threshold search of find_closest with no device
+        allocation (at most 65,536 candidates)."""
+        a = np.zeros(capacity, np.uint32)
+        b = np.zeros(capacity, np.uint32)
+        k = np.zeros(capacity, np.int32)
+        d = np.zeros(capacity)
+        t = np.zeros(capacity)
+        pa = np.zeros((capacity, 3))
+        pb = np.zeros((capacity, 3))
+        cnt = C._I64()
+        check(self.ctx.h, lib().ogp_conj_find_noalloc(self.ctx.h, threshold, capacity, ptr(a),
+                                                      ptr(b), ptr(k), ptr(d), ptr(t), ptr(pa),
+                                                      ptr(pb), ctypes.byref(cnt)))
+        n = cnt.value
+        return ClosestApproachData(a[:n], b[:n], k[:n], d[:n], t[:n], pa[:n], pb[:n])
+
     def screen(self):
         np_ = C._I64()
         check(self.ctx.h, lib().ogp_conj_num_pairs(self.ctx.h, ctypes.byref(np_)))
