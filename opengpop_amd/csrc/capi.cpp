@@ -117,7 +117,7 @@ struct ogp_context {
 
     GravityModel grav;
     FrameData frames;
-    DevBuf d_rec, d_recf, d_diag, d_parts, d_eop, d_leap_mjd, d_leap_sec, d_nut[4];
+    DevBuf d_rec, d_recf, d_diag, d_parts, d_part_diag, d_eop, d_leap_mjd, d_leap_sec, d_nut[4];
     GravityDev gdev {};
     FrameDev fdev {};
     int kind = kGravity;  // kernel variant of the force
@@ -691,6 +691,18 @@ int ogp_create(const ogp_config* cfg, ogp_context** out)
             column_parts(c.grav.L, parts.data());
             upload(&c, c.d_parts, parts);
             c.gdev.parts = c.d_parts.as<int>();
+            // D_m = prod_{j=1..m} diag[j] at each part's first column (the closed-form start of
+            // the diagonal recurrence; parts past the last column have no columns to start)
+            std::vector<double> part_diag(kParts, 0.0);
+            for (int q = 0; q < kParts; ++q) {
+                const int m0 = std::min(parts[q], c.grav.L);
+                double D = 1.0;
+                for (int j = 1; j <= m0; ++j)
+                    D *= c.grav.diag_ratio[j];
+                part_diag[q] = D;
+            }
+            upload(&c, c.d_part_diag, part_diag);
+            c.gdev.part_diag = c.d_part_diag.as<double>();
             c.gdev.L = c.grav.L;
             c.gdev.nsplit = cfg->precision == OGP_PRECISION_SPLIT ? std::max(cfg->split_degree, 1)
                                                                   : c.grav.L;

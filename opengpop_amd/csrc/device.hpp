@@ -31,6 +31,7 @@ struct GravityDev {
     const float* recf;  // the same records in FP32
     const double* diag; // Pbar_mm / Pbar_{m-1,m-1}
     const int* parts;   // kParts + 1 column bounds: part q = columns [parts[q], parts[q + 1])
+    const double* part_diag; // per part q: D_m = prod_{j=1..m} diag[j] at m = parts[q] (D_0 = 1)
     int L;              // degree and order
     int nsplit;         // split precision: degrees <= nsplit in FP64, above in FP32
     double gm, radius;
@@ -161,42 +162,72 @@ __device__ __forceinline__ void grav_assemble(V x, V y, V z, const GravPointT<V>
     az = coef * fmat(Lam * z, p.ir, -s.H);
 }
 
+// The column recurrences' state at the first column m0 of a part, in closed form:
+// (cos m0 lambda, sin m0 lambda) and (cos (m0-1) lambda, sin (m0-1) lambda) as powers of
+// (U1, W1) by repeated squaring, and the scaled diagonal rho (cos(phi) rho)^(m0-1) D_m0. Every
+// part starts from these values, whatever part precedes it and whichever wave sums it, so the
+// sums do not depend on the launch shape; a wave reaches its first part in O(log m0)
+// operations and no table loads.
+template <typename V>
+__device__ __forceinline__ void part_start(const GravPointT<V>& p, int m0, double Dm0, V& Um,
+    V& Wm, V& Um1, V& Wm1, V& diagP)
+{
+    if (m0 == 0) {
+        Um = Um1 = 1.0;
+        Wm = Wm1 = 0.0;
+        diagP = 1.0;
+        return;
+    }
+    V pu = 1.0, pw = 0.0, cr = 1.0;                   // z^(m0-1), (cphi rho)^(m0-1)
+    V bu = p.U1, bw = p.W1, cb = p.cphi * p.rho;      // z^(2^i), (cphi rho)^(2^i)
+    for (int e = m0 - 1; e > 0; e >>= 1) {           // e is wave-uniform
+        if (e & 1) {
+            const V nu = fmat(pu, bu, -pw * bw);
+            pw = fmat(pw, bu, pu * bw);
+            pu = nu;
+            cr = cr * cb;
+        }
+        if (e > 1) {
+            const V nu = fmat(bu, bu, -bw * bw);
+            bw = fmat(bw, bu, bu * bw);
+            bu = nu;
+            cb = cb * cb;
+        }
+    }
+    Um1 = pu;
+    Wm1 = pw;
+    Um = fmat(pu, p.U1, -pw * p.W1);
+    Wm = fmat(pw, p.U1, pu * p.W1);
+    diagP = p.rho * cr * Dm0;
+}
+
 // Sums of parts [q0, q1) (q0 < q1, wave-uniform); sink(q, sums) receives the sums of part q, in
-// part order. The longitude terms and the scaled diagonal of the first column are reached by
-// running their recurrences from m = 0 (a few operations per skipped column).
+// part order. Each part starts from part_start's closed-form values.
 template <bool SPLIT, typename V, class Sink>
 __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const GravityDev& g, int q0,
     int q1, Sink&& sink)
 {
     const V er = p.er, rr2 = p.rr2, cphi = p.cphi, rho = p.rho, U1 = p.U1, W1 = p.W1;
     const float erf_ = tofloat(er), rr2f = tofloat(rr2);
-    V Um = 1.0, Wm = 0.0;   // cos(m lambda), sin(m lambda)
-    V Um1 = 1.0, Wm1 = 0.0; // (m - 1)
-    V diagP = 1.0;          // scaled diagonal cos(phi)^(m-1) (R/r)^m Pbar_mm
+    V Um, Wm;   // cos(m lambda), sin(m lambda)
+    V Um1, Wm1; // (m - 1)
+    V diagP;    // scaled diagonal cos(phi)^(m-1) (R/r)^m Pbar_mm
     const uniform_dptr diag = (uniform_dptr)g.diag;
     const uniform_iptr parts = (uniform_iptr)g.parts;
+    const uniform_dptr part_diag = (uniform_dptr)g.part_diag;
     const int L = g.L;
-    const int m0 = parts[q0];
-    int kcol = 0; // first record of column m
-    for (int m = 0; m < m0; ++m) {
-        if (m == 1)
-            diagP = rho * diag[1];
-        else if (m >= 2)
-            diagP *= cphi * rho * diag[m];
-        Um1 = Um;
-        Wm1 = Wm;
-        const V un = fmat(Um, U1, -Wm * W1);
-        Wm = fmat(Wm, U1, Um * W1);
-        Um = un;
-        kcol += L - m + 1;
-    }
-    int m = m0;
     for (int q = q0; q < q1; ++q) {
+        const int mb = parts[q];
+        part_start(p, mb, part_diag[q], Um, Wm, Um1, Wm1, diagP);
+        int m = mb;
+        int kcol = m * (L + 1) - m * (m - 1) / 2; // first record of column m
         V G = 0.0, H = 0.0, J = 0.0, K = 0.0;
         for (const int mend = parts[q + 1]; m < mend; ++m) {
-            if (m == 1)
+            if (m == mb)
+                ; // the part's first diagonal comes from part_start
+            else if (m == 1)
                 diagP = rho * diag[1];
-            else if (m >= 2)
+            else
                 diagP *= cphi * rho * diag[m];
             const int ncol = L - m + 1;
             V A1, A2, B1, B2, E1, E2;

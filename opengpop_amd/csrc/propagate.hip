@@ -365,8 +365,8 @@ void column_parts(int L, int* parts)
 {
     // Contiguous column ranges of about equal cost: about 9 vector operations per record
     // (column m has L - m + 1 records) plus about 25 per column for its set-up and epilogue; a
-    // wave whose first part starts at column m also runs the longitude and diagonal recurrences
-    // over the m columns before it (about 7 operations each).
+    // part that starts at column m > 0 first computes the recurrences' state there in closed
+    // form (device.hpp, part_start: about 20 operations per halving of m).
     std::vector<double> C(L + 2, 0.0);
     for (int m = 0; m <= L; ++m)
         C[m + 1] = C[m] + 9.0 * (L - m + 1) + 25.0;
@@ -382,18 +382,23 @@ void column_parts(int L, int* parts)
     // At low degree a column is a large share of a part and the equal-cost boundaries leave the
     // waves unbalanced. Every launch shape uses these parts, so they are adjusted to balance the
     // slowest wave against the mean for 2, 4 and 8 waves at once: boundaries move by one column
-    // while the worst ratio drops (degree 20: 6% faster at 4 waves, 2% at 8). At high degree
-    // the equal-cost boundaries measured better (degree 70, 4 waves: 1.4% slower adjusted).
+    // while the worst ratio drops. At high degree the equal-cost boundaries measured better
+    // (degree 70, 4 waves: 1.4% slower adjusted).
     if (L > 30)
         return;
-    constexpr double kSkip = 7.0;
+    auto start_cost = [](int m) {
+        int bits = 0;
+        for (int e = m - 1; e > 0; e >>= 1)
+            ++bits;
+        return m > 0 ? 10.0 + 12.0 * bits : 0.0;
+    };
     auto worst = [&](const int* p) {
         double r = 0.0;
         for (int W = 2; W <= kParts; W *= 2) {
             double mx = 0.0, sum = 0.0;
             for (int w = 0; w < W; ++w) {
                 const int a = p[w * kParts / W], b = p[(w + 1) * kParts / W];
-                const double work = C[b] - C[a] + kSkip * a;
+                const double work = C[b] - C[a] + start_cost(a);
                 mx = std::max(mx, work);
                 sum += work;
             }
