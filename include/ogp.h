@@ -229,7 +229,9 @@ int ogp_get_tuning(const ogp_context* ctx, int64_t n, int* waves, int64_t* chunk
 /* Object slices per step-size class (no counterpart in the reference): each class is cut into
  * `slices` contiguous slices whose GJ8 chunk sequences run concurrently on streams of their own,
  * so the end of one launch overlaps the launches of other slices. 0 = automatic (2 for a class
- * of at least 4096 objects, else 1), at most 16. Results are bit-identical for every choice. */
+ * of at least 4096 objects, unless it is the call's only class and has no more 64-object groups
+ * than the device has compute units; else 1), at most 16. Results are bit-identical for every
+ * choice. */
 int ogp_set_slices(ogp_context* ctx, int slices);
 
 /* GpuPropagator (gpu_propagator.cuh:26-59, gpu_propagator.cu:28-154): a stepwise propagator
