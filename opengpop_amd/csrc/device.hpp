@@ -781,9 +781,11 @@ __device__ __forceinline__ ForceCtx object_ctx(const ForceCtx& f, int64_t i)
     return o;
 }
 
-// Per-workgroup exchange area of the column-split evaluation (unused when W == 1).
+// Per-workgroup exchange areas of the column-split evaluation (unused when W == 1): the part
+// sums, and the lead's geometry of the evaluation point (lead/helper form).
 struct Exchange {
     double* red;
+    double* pos = nullptr;
 };
 
 // ForceModel::ComputeGenerationAcceleration (force_model.cu:59-89): RK8 stage k of startup
@@ -820,6 +822,29 @@ __device__ __forceinline__ void force_prop(const ForceCtx& f, const double* x, c
         const double t = (double)(node - 4) * f.h;
         analytic_force<KIND>(x, v, t, a);
     }
+}
+
+// force_gen for the lead wave of a lead/helper workgroup (gravity kinds, W > 1); bit-identical
+// to force_gen<KIND, W>.
+template <int KIND, int W>
+__device__ __forceinline__ void force_gen_lead(const ForceCtx& f, const double* x, const double* v,
+    int s, int k, double* a, double* red, double* pos)
+{
+    static_assert(KIND == kGravity || KIND == kGravitySplit, "gravity kinds only");
+    const int idx = ((s > 0 ? s - 1 : s) + 4) * 10 + k;
+    gravity_gcrs_lead<KIND == kGravitySplit, W>(f.gen_M + 9 * idx, x, f.grav, red, pos, a);
+    add_sources(f.src, f.gen_tb + 6 * idx, x, v, a);
+}
+
+// The largest x over the 64 lanes of the wave, returned in every lane; every lane must be active.
+__device__ __forceinline__ int64_t wave_max(int64_t x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int64_t y = __shfl_xor(x, off, 64);
+        x = x > y ? x : y;
+    }
+    return x;
 }
 
 // force_prop for the lead wave of a lead/helper workgroup (gravity kinds, W > 1); bit-identical

@@ -2,14 +2,16 @@
 //
 // Layout: a workgroup of W waves (W = 1, 2, 4 or 8) holds one group of 64 objects; lane l of
 // every wave owns object 64 * blockIdx.x + l. The spherical-harmonic columns of each force
-// evaluation are divided between the W waves and their partial sums meet in LDS.
-//   * GJ8 steps (gj8_chunk_kernel): wave 0 alone runs the integrator, its global loads and
-//     stores, the fold of the parts and the corrector; it hands the geometry of each
-//     evaluation point to the other waves through LDS, and they only sum their parts
-//     (device.hpp, gravity_gcrs_lead).
-//   * Startup (startup_kernel): every wave runs the RK8 and startup-corrector arithmetic
-//     redundantly and bit-identically (device.hpp, gravity_itrs_split), so no wave reads a
-//     value written by another wave through global memory.
+// evaluation are divided between the W waves and their partial sums meet in LDS. Wave 0 (the
+// lead) alone runs the integrator, its global loads and stores, the fold of the parts and the
+// corrector; it hands the geometry of each evaluation point to the other waves through LDS, and
+// they only sum their parts (device.hpp, gravity_gcrs_lead / gravity_helper). Only the lead
+// reads or writes the state and the startup work areas in global memory.
+//
+// Every workgroup barrier sits in workgroup-uniform control flow: the loops run to the group's
+// last step (the same in every wave, which hold the same objects), and a lane whose span has
+// ended, or that lies past the launch's objects, takes part in the evaluations on a state it
+// does not store.
 #include "gj8.hpp"
 #include "kernels.hpp"
 
@@ -22,11 +24,40 @@ namespace ogp {
 
 namespace {
 
+// One object's rows of a startup work area ([slot][component][object]). Stores happen only for
+// lanes that run (`on`); the others read and discard.
 struct Slots {
     double* base;
     int64_t N, i;
-    __device__ double& operator()(int slot, int c) const { return base[(slot * 3 + c) * N + i]; }
+    bool on;
+    __device__ double get(int slot, int c) const { return base[(slot * 3 + c) * N + i]; }
+    __device__ void set(int slot, int c, double x) const
+    {
+        if (on)
+            base[(slot * 3 + c) * N + i] = x;
+    }
 };
+
+// Force evaluations of the startup: the lead form for gravity kinds with W > 1 (the helpers
+// run gravity_helper for each), the one-wave form otherwise.
+template <int KIND, int W>
+__device__ __forceinline__ void eval_gen(const ForceCtx& f, const double* x, const double* v,
+    int s, int k, double* a, Exchange& ex)
+{
+    if constexpr (W > 1 && (KIND == kGravity || KIND == kGravitySplit))
+        force_gen_lead<KIND, W>(f, x, v, s, k, a, ex.red, ex.pos);
+    else
+        force_gen<KIND, W>(f, x, v, s, k, a, ex);
+}
+template <int KIND, int W>
+__device__ __forceinline__ void eval_prop(const ForceCtx& f, const double* x, const double* v,
+    int64_t node, double* a, Exchange& ex)
+{
+    if constexpr (W > 1 && (KIND == kGravity || KIND == kGravitySplit))
+        force_prop_lead<KIND, W>(f, x, v, node, a, ex.red, ex.pos);
+    else
+        force_prop<KIND, W>(f, x, v, node, a, ex);
+}
 
 // RK8::step (gpu_startup.cu:108-144, reset_temp_vectors.cu, device_functions.cu,
 // copy_prev_state.cu): stage states from the previous node, stage accelerations into Kv,
@@ -40,16 +71,14 @@ __device__ void rk8_step(const ForceCtx& f, const double (*a)[9], double h, cons
     double x0[3], v0[3], tX[3], tV[3], out[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        x0[c] = nr(prev, c);
-        v0[c] = nv(prev, c);
+        x0[c] = nr.get(prev, c);
+        v0[c] = nv.get(prev, c);
     }
-    force_gen<KIND, W>(f, x0, v0, step, 0, out, ex);
-    // Kx(0) is written only after the stage-0 evaluation (and its workgroup barrier): another
-    // wave of the group may still be reading the previous step's Kx(0) before that barrier.
+    eval_gen<KIND, W>(f, x0, v0, step, 0, out, ex);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        Kx(0, c) = v0[c];
-        Kv(0, c) = out[c];
+        Kx.set(0, c, v0[c]);
+        Kv.set(0, c, out[c]);
     }
 #pragma unroll 1
     for (int k = 1; k <= 9; ++k) {
@@ -58,28 +87,28 @@ __device__ void rk8_step(const ForceCtx& f, const double (*a)[9], double h, cons
             double sx = 0.0, sv = 0.0;
 #pragma unroll 1
             for (int j = 0; j < k; ++j) {
-                sx = fma(Kx(j, c), a[k - 1][j], sx);
-                sv = fma(Kv(j, c), a[k - 1][j], sv);
+                sx = fma(Kx.get(j, c), a[k - 1][j], sx);
+                sv = fma(Kv.get(j, c), a[k - 1][j], sv);
             }
             tX[c] = fma(hs, sx, x0[c]);
             tV[c] = fma(hs, sv, v0[c]);
         }
-        force_gen<KIND, W>(f, tX, tV, step, k, out, ex);
+        eval_gen<KIND, W>(f, tX, tV, step, k, out, ex);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            Kv(k, c) = out[c];
-            Kx(k, c) = tV[c];
+            Kv.set(k, c, out[c]);
+            Kx.set(k, c, tV[c]);
         }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         double sv = 0.0, sx = 0.0;
         for (int k = 0; k < 10; ++k) {
-            sv = fma(Kv(k, c), c_rk8b[k], sv);
-            sx = fma(Kx(k, c), c_rk8b[k], sx);
+            sv = fma(Kv.get(k, c), c_rk8b[k], sv);
+            sx = fma(Kx.get(k, c), c_rk8b[k], sx);
         }
-        nv(4 + step, c) = fma(hs, sv, v0[c]);
-        nr(4 + step, c) = fma(hs, sx, x0[c]);
+        nv.set(4 + step, c, fma(hs, sv, v0[c]));
+        nr.set(4 + step, c, fma(hs, sx, x0[c]));
     }
 }
 
@@ -95,13 +124,13 @@ __device__ void acc_matrix(const ForceCtx& f, const Slots& nr, const Slots& nv, 
         double x[3], v[3], o[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            x[c] = nr(j, c);
-            v[c] = nv(j, c);
+            x[c] = nr.get(j, c);
+            v[c] = nv.get(j, c);
         }
-        force_prop<KIND, W>(f, x, v, j, o, ex);
+        eval_prop<KIND, W>(f, x, v, j, o, ex);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            acc(j, c) = o[c];
+            acc.set(j, c, o[c]);
     }
 }
 
@@ -116,27 +145,27 @@ __device__ void startup_correct(const ForceCtx& f, double h, const Slots& nr, co
     for (int c = 0; c < 3; ++c) {
         double ss = 0.0, sS = 0.0;
         for (int k = 0; k < 9; ++k) {
-            ss = fma(acc(k, c), c_sa[4][k], ss);
-            sS = fma(acc(k, c), c_gj[4][k], sS);
+            ss = fma(acc.get(k, c), c_sa[4][k], ss);
+            sS = fma(acc.get(k, c), c_gj[4][k], sS);
         }
-        sm(4, c) = fma(invh, nv(4, c), -ss);
-        Sm(4, c) = fma(invh2, nr(4, c), -sS);
+        sm.set(4, c, fma(invh, nv.get(4, c), -ss));
+        Sm.set(4, c, fma(invh2, nr.get(4, c), -sS));
         for (int i = 1; i <= 4; ++i) {
-            sm(4 + i, c) = sm(4 + i - 1, c) + (acc(4 + i - 1, c) + acc(4 + i, c)) / 2;
-            sm(4 - i, c) = sm(4 - i + 1, c) - (acc(4 - i + 1, c) + acc(4 - i, c)) / 2;
-            Sm(4 + i, c) = sm(4 + i - 1, c) + Sm(4 + i - 1, c) + acc(4 + i - 1, c) / 2;
-            Sm(4 - i, c) = -sm(4 - i + 1, c) + Sm(4 - i + 1, c) + acc(4 - i + 1, c) / 2;
+            sm.set(4 + i, c, sm.get(4 + i - 1, c) + (acc.get(4 + i - 1, c) + acc.get(4 + i, c)) / 2);
+            sm.set(4 - i, c, sm.get(4 - i + 1, c) - (acc.get(4 - i + 1, c) + acc.get(4 - i, c)) / 2);
+            Sm.set(4 + i, c, sm.get(4 + i - 1, c) + Sm.get(4 + i - 1, c) + acc.get(4 + i - 1, c) / 2);
+            Sm.set(4 - i, c, -sm.get(4 - i + 1, c) + Sm.get(4 - i + 1, c) + acc.get(4 - i + 1, c) / 2);
         }
         for (int j = 0; j < 9; ++j) {
             if (j == 4)
                 continue;
             double sv = 0.0, sp = 0.0;
             for (int k = 0; k < 9; ++k) {
-                sv = fma(acc(k, c), c_sa[j][k], sv);
-                sp = fma(acc(k, c), c_gj[j][k], sp);
+                sv = fma(acc.get(k, c), c_sa[j][k], sv);
+                sp = fma(acc.get(k, c), c_gj[j][k], sp);
             }
-            nv(j, c) = (sm(j, c) + sv) * h;
-            nr(j, c) = (Sm(j, c) + sp) * hh;
+            nv.set(j, c, (sm.get(j, c) + sv) * h);
+            nr.set(j, c, (Sm.get(j, c) + sp) * hh);
         }
     }
     acc_matrix<KIND, W>(f, nr, nv, acc, false, ex);
@@ -146,11 +175,24 @@ template <int W> struct ExchangeArea {
     static constexpr int size = W > 1 ? kParts * 4 * 64 : 1;
 };
 
+// Lane l's object. Lanes past the launch's objects read object obj0 (a valid index) and count
+// as having no steps.
+struct Lane {
+    bool valid;
+    int64_t i;
+};
+__device__ __forceinline__ Lane lane_object(int64_t obj0, int64_t nobj)
+{
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
+    return Lane { t < nobj, obj0 + (t < nobj ? t : 0) };
+}
+
 } // namespace
 
 // GpuStartup::Startup (gpu_startup.cu:55-68): 4 forward and 4 backward RK8 steps, the
 // acceleration matrix and ncorr corrections; then the hand-over to the GJ8 steps
-// (gpu_propagator.cu:31-53, copy_s.cu:19-30).
+// (gpu_propagator.cu:31-53, copy_s.cu:19-30). A lane with no step (no GpuPropagator::Step call)
+// leaves its state untouched.
 template <int KIND, int W>
 __global__ void __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(2))) // at most 256 VGPRs + AGPRs: 2 resident waves per SIMD
@@ -158,23 +200,36 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     StartupScratch sc, int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps)
 {
     __shared__ double red[ExchangeArea<W>::size];
-    Exchange ex { red };
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
-    if (t >= nobj)
-        return;
-    const int64_t i = obj0 + t;
+    __shared__ double pos[W > 1 ? kPointLen * 64 : 1]; // the lead's geometry (gravity_gcrs_lead)
+    Exchange ex { red, pos };
+    const Lane ln = lane_object(obj0, nobj);
+    const int64_t i = ln.i;
     const ForceCtx f = object_ctx(f0, i);
     const int64_t N = ps.N;
-    const int64_t ns = ps.nsteps[i];
-    if (ns <= 0)
-        return; // no GpuPropagator::Step call: the state is left untouched
-    const Slots nr { sc.node_r, N, i }, nv { sc.node_v, N, i }, acc { sc.acc, N, i };
-    const Slots sm { sc.sm, N, i }, Sm { sc.Sm, N, i }, Kx { sc.Kx, N, i }, Kv { sc.Kv, N, i };
+    const int64_t ns = ln.valid ? ps.nsteps[i] : 0;
+    if (__builtin_amdgcn_readfirstlane((int)std::min<int64_t>(wave_max(ns), 1)) <= 0)
+        return; // no lane of the group runs (the same in every wave of the workgroup)
+    const bool act = ns > 0;
+    if constexpr (W > 1 && (KIND == kGravity || KIND == kGravitySplit)) {
+        const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+        if (w > 0) {
+            // one part sum per force evaluation of the lead: 8 RK8 steps of 10 stages, the
+            // acceleration matrix (9 nodes) and ncorr corrections of 8 nodes each
+            const int nevals = 8 * 10 + 9 + 8 * ncorr;
+#pragma unroll 1
+            for (int e = 0; e < nevals; ++e)
+                gravity_helper<KIND == kGravitySplit, W>(f.grav, red, pos, w);
+            return;
+        }
+    }
+    const Slots nr { sc.node_r, N, i, act }, nv { sc.node_v, N, i, act }, acc { sc.acc, N, i, act };
+    const Slots sm { sc.sm, N, i, act }, Sm { sc.Sm, N, i, act }, Kx { sc.Kx, N, i, act },
+        Kv { sc.Kv, N, i, act };
     const double h = f.h;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        nr(4, c) = ps.r[c * N + i];
-        nv(4, c) = ps.v[c * N + i];
+        nr.set(4, c, ps.r[c * N + i]);
+        nv.set(4, c, ps.v[c * N + i]);
     }
     const double(*a)[9] = c_rk8a[rk8_variant];
 #pragma unroll 1
@@ -186,16 +241,17 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
 #pragma unroll 1
     for (int it = 0; it < ncorr; ++it)
         startup_correct<KIND, W>(f, h, nr, nv, acc, sm, Sm, ex);
-
+    if (!act)
+        return;
     const int last = (int)(ns < 4 ? ns : 4);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        ps.s[c * N + i] = sm(8, c);
-        ps.S[c * N + i] = Sm(8, c);
-        ps.r[c * N + i] = nr(4 + last, c);
-        ps.v[c * N + i] = nv(4 + last, c);
+        ps.s[c * N + i] = sm.get(8, c);
+        ps.S[c * N + i] = Sm.get(8, c);
+        ps.r[c * N + i] = nr.get(4 + last, c);
+        ps.v[c * N + i] = nv.get(4 + last, c);
         for (int j = -4; j <= 4; ++j)
-            ps.hist[(ring_slot(j) * 3 + c) * N + i] = acc(j + 4, c);
+            ps.hist[(ring_slot(j) * 3 + c) * N + i] = acc.get(j + 4, c);
     }
     if (traj) {
         // steps 1-4 output the startup nodes; the acceleration handed to the conjunction
@@ -203,9 +259,9 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
         for (int k = 1; k <= last && k <= traj_steps; ++k)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                traj[((int64_t)(k - 1) * 9 + c) * N + i] = nr(4 + k, c);
-                traj[((int64_t)(k - 1) * 9 + 3 + c) * N + i] = nv(4 + k, c);
-                traj[((int64_t)(k - 1) * 9 + 6 + c) * N + i] = acc(8, c);
+                traj[((int64_t)(k - 1) * 9 + c) * N + i] = nr.get(4 + k, c);
+                traj[((int64_t)(k - 1) * 9 + 3 + c) * N + i] = nv.get(4 + k, c);
+                traj[((int64_t)(k - 1) * 9 + 6 + c) * N + i] = acc.get(8, c);
             }
     }
 }
@@ -225,28 +281,25 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
 {
     __shared__ double red[ExchangeArea<W>::size];
     __shared__ double pos[W > 1 ? kPointLen * 64 : 1]; // the lead's geometry (gravity_gcrs_lead)
-    Exchange ex { red };
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
-    if (t >= nobj)
-        return;
-    const int64_t i = obj0 + t;
+    Exchange ex { red, pos };
+    const Lane ln = lane_object(obj0, nobj);
+    const int64_t i = ln.i;
     const ForceCtx f = object_ctx(f0, i);
     const int64_t N = ps.N;
-    const int64_t ns = ps.nsteps[i];
-    if (ns < k0)
+    const int64_t ns = ln.valid ? ps.nsteps[i] : -1;
+    // the group's last step of the chunk, the same in every wave of the workgroup
+    const int64_t kend = __builtin_amdgcn_readfirstlane(
+        (int)std::min<int64_t>(k1, wave_max(ns) + 1));
+    if (kend <= k0)
         return;
     if constexpr (W > 1) {
-        // Waves 1 .. W-1 only sum their harmonic parts; wave 0 integrates. Every wave of the
-        // group holds the same objects in its lanes, so all of them take the same iterations
-        // below and pass the same barriers (two per evaluation).
+        // Waves 1 .. W-1 only sum their harmonic parts; wave 0 integrates. Two barriers per
+        // evaluation, in every wave for every step up to kend.
         const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
         if (w > 0) {
-            for (int64_t k = k0; k < k1; ++k) {
-                if (k > ns)
-                    continue;
+            for (int64_t k = k0; k < kend; ++k)
                 for (int it = 0; it < ncorr; ++it)
                     gravity_helper<KIND == kGravitySplit, W>(f.grav, red, pos, w);
-            }
             return;
         }
     }
@@ -259,21 +312,30 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
         s[c] = ps.s[c * N + i];
         S[c] = ps.S[c * N + i];
     }
-    for (int64_t k = k0; k < k1; ++k) {
-        if (k > ns)
-            continue; // this object's span has ended; its workgroup continues for the others
+    auto store_state = [&]() {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ps.r[c * N + i] = r[c];
+            ps.v[c * N + i] = v[c];
+            ps.s[c * N + i] = s[c];
+            ps.S[c * N + i] = S[c];
+        }
+    };
+    for (int64_t k = k0; k < kend; ++k) {
+        // Every lane runs every step up to kend; a lane past its span (k > ns) integrates on
+        // a history it no longer updates and stores nothing: its final state was stored at its
+        // last step.
         GJ8Sums q;
         gj8_sums(ps.hist, N, i, k, q);
         double sprev[3], anew[3];
         gj8_predict(q, h, s, S, r, v, sprev);
         // P E (E C)^ncorr: the predictor's evaluation is the first corrector evaluation.
         for (int it = 0; it < ncorr; ++it) {
-            if constexpr (W > 1)
-                force_prop_lead<KIND, W>(f, r, v, k + 4, anew, red, pos);
-            else
-                force_prop<KIND, W>(f, r, v, k + 4, anew, ex);
+            eval_prop<KIND, W>(f, r, v, k + 4, anew, ex);
             gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         }
+        if (k > ns)
+            continue;
         const int wslot = ring_slot(k);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -287,14 +349,11 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
                 traj[(row * 9 + 6 + c) * N + i] = anew[c];
             }
         }
+        if (k == ns)
+            store_state();
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        ps.r[c * N + i] = r[c];
-        ps.v[c * N + i] = v[c];
-        ps.s[c * N + i] = s[c];
-        ps.S[c * N + i] = S[c];
-    }
+    if (ns >= kend)
+        store_state();
 }
 
 // Instantiation over force kinds (analytic test forces use W = 1 only).
