@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "gj8.hpp"
 #include "kernels.hpp"
 
@@ -216,26 +218,27 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
 {
     __shared__ double red[W > 1 ? kJacExchange : 1];
     __shared__ double pos[W > 1 ? 3 * 64 : 1]; // the lead's ITRS positions
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
+    // Every barrier sits in workgroup-uniform control flow: the loops run to the group's last
+    // step (the same in every wave, which hold the same objects); a lane past its span, or past
+    // the objects, integrates on a history it does not update and stores nothing.
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
     const int64_t n = ps.N;
-    if (i >= n)
-        return;
-    const int64_t ns = ps.nsteps[i];
-    if (ns < k0)
+    const bool valid = t < n;
+    const int64_t i = valid ? t : 0;
+    const int64_t ns = valid ? ps.nsteps[i] : -1;
+    const int64_t kend = __builtin_amdgcn_readfirstlane(
+        (int)std::min<int64_t>(k1, wave_max(ns) + 1));
+    if (kend <= k0)
         return;
     if constexpr (W > 1) {
         // The W waves of a workgroup hold the same 64 objects and share the harmonic columns.
         // Waves 1 .. W-1 only sum their Dual3 parts (gravity_jac_helper); wave 0 integrates,
-        // records the Jacobians and writes the ring and the state. All waves take the same
-        // iterations, so they pass the same barriers (two per evaluation).
+        // records the Jacobians and writes the ring and the state. Two barriers per evaluation.
         const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
         if (w > 0) {
-            for (int64_t k = k0; k < k1; ++k) {
-                if (k > ns)
-                    break;
+            for (int64_t k = k0; k < kend; ++k)
                 for (int it = 0; it < ncorr; ++it)
                     gravity_jac_helper<W>(f.grav, red, pos, w);
-            }
             return;
         }
     }
@@ -247,9 +250,18 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         r[c] = ps.r[c * n + i];
         v[c] = ps.v[c * n + i];
     }
-    for (int64_t k = k0; k < k1; ++k) {
-        if (k > ns)
-            break;
+    // the state after the segment (or at the object's last step): the record pass can then
+    // serve as the forward pass
+    auto store_state = [&]() {
+        for (int c = 0; c < 3; ++c) {
+            ps.s[c * n + i] = s[c];
+            ps.S[c * n + i] = S[c];
+            ps.r[c * n + i] = r[c];
+            ps.v[c * n + i] = v[c];
+        }
+    };
+    for (int64_t k = k0; k < kend; ++k) {
+        const bool act = k <= ns;
         GJ8Sums q;
         gj8_sums(ps.hist, n, i, k, q);
         double sprev[3], anew[3];
@@ -260,22 +272,23 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
                 gravity_gcrs_jac_lead<W>(f.prop_M + 9 * (k + 4), r, f.grav, red, pos, anew, G);
             else
                 gravity_gcrs_jac<W>(f.prop_M + 9 * (k + 4), r, f.grav, anew, G, red);
-            const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
-            for (int e = 0; e < 6; ++e)
-                Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
+            if (act) {
+                const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
+                for (int e = 0; e < 6; ++e)
+                    Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
+            }
             gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         }
+        if (!act)
+            continue;
         const int wslot = ring_slot(k);
         for (int c = 0; c < 3; ++c)
             ps.hist[(wslot * 3 + c) * n + i] = anew[c];
+        if (k == ns)
+            store_state();
     }
-    // the state after the segment: the record pass can then serve as the forward pass
-    for (int c = 0; c < 3; ++c) {
-        ps.s[c * n + i] = s[c];
-        ps.S[c * n + i] = S[c];
-        ps.r[c * n + i] = r[c];
-        ps.v[c * n + i] = v[c];
-    }
+    if (ns >= kend)
+        store_state();
 }
 
 // Reverse pass over steps k1-1 .. k0 for R cotangent rows. lam[(rho * 33 + e) * n + o] holds the
