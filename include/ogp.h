@@ -219,12 +219,21 @@ int ogp_last_timing(const ogp_context* ctx, ogp_timing* t);
  * 256-thread blocks per helper kernel): `waves` = waves per group of 64 objects that share one
  * object group's harmonic columns (1, 2, 4, 8; 0 = autotuned), chunk_steps = GJ8 steps per
  * launch (0 = 64). Results are bit-identical for every choice. With waves = 0 the context times
- * one chunk per candidate the first time it meets a group count (per power of two) and keeps the
- * fastest; ogp_set_tuning clears that table. A shape that gives fewer waves than the device has
+ * two chunks per candidate (in mirrored order) the first time it meets a group count (per power
+ * of two) and keeps the fastest; ogp_set_tuning clears that table. A shape that gives fewer waves than the device has
  * SIMDs is not a candidate unless it is the widest, so small populations take 8 waves without
  * trials. ogp_get_tuning reports the choice for n objects. */
 int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps);
 int ogp_get_tuning(const ogp_context* ctx, int64_t n, int* waves, int64_t* chunk_steps);
+
+/* Object groups per workgroup of the GJ8 chunks (no counterpart in the reference): 2 runs two
+ * groups of 64 objects in lockstep in one workgroup (with 4 or 8 waves per group), so the waves
+ * that sum the same harmonic columns for different groups read the same table lines at about
+ * the same time; 1 (and 0, the default) runs one. With 2 the tuner chooses the waves among the
+ * paired shapes. Results are bit-identical for every choice. ogp_get_lockstep reports the
+ * choice for n objects. */
+int ogp_set_lockstep(ogp_context* ctx, int groups);
+int ogp_get_lockstep(const ogp_context* ctx, int64_t n, int* groups);
 
 /* Object slices per step-size class (no counterpart in the reference): each class is cut into
  * `slices` contiguous slices whose GJ8 chunk sequences run concurrently on streams of their own,

@@ -175,10 +175,15 @@ struct ogp_context {
     void sync() { HIPC(hipStreamSynchronize(stream)); }
     void check_launch() { HIPC(hipGetLastError()); }
 
-    // Autotuned waves per 64-object group, keyed by floor(log2(number of groups)). Results do
-    // not depend on the choice (the harmonic parts are summed in a fixed order), so the tuner
-    // may switch between chunks of one propagation.
-    std::map<int, int> tuned_waves;
+    // Launch shape of the GJ8 chunks: W waves per 64-object group and G groups per workgroup
+    // in lockstep, encoded as W | (G - 1) << 4. Autotuned per floor(log2(number of groups)).
+    // Results do not depend on the shape (the harmonic parts are summed in a fixed order), so
+    // the tuner may switch between chunks of one propagation.
+    static int make_shape(int W, int G) { return W | ((G - 1) << 4); }
+    static int shape_waves(int shape) { return shape & 15; }
+    static int shape_groups(int shape) { return (shape >> 4) + 1; }
+    std::map<int, int> tuned_shape;
+    int lockstep_override = 0; // groups per workgroup; 0 = tuned (1 when waves are fixed)
 
     static int group_bucket(int64_t n)
     {
@@ -194,43 +199,49 @@ struct ogp_context {
     int simds = 1024; // SIMDs of the device (4 per CU), read at creation
     void* noalloc = nullptr; // device address of the allocation-free search's scratch array
 
-    // Waves per 64-object group: the override, else the tuned value for this size, else enough
-    // waves for about 1.2 resident per SIMD, which hides most of the latency of the scalar
-    // coefficient loads.
-    int waves_for(int64_t n) const
+    // Shape of the chunk launches at n objects: the overrides, else the tuned value for this
+    // size, else enough waves for about 1.2 resident per SIMD (which hides most of the latency
+    // of the scalar coefficient loads) and one group per workgroup.
+    int shape_for(int64_t n) const
     {
         if (kind != kGravity && kind != kGravitySplit)
-            return 1;
+            return make_shape(1, 1);
         if (waves_override > 0)
-            return waves_override;
-        auto it = tuned_waves.find(group_bucket(n));
-        if (it != tuned_waves.end())
+            return make_shape(waves_override,
+                waves_override >= 4 && lockstep_override == 2 ? 2 : 1);
+        auto it = tuned_shape.find(group_bucket(n));
+        if (it != tuned_shape.end())
             return it->second;
         const int64_t groups = (n + 63) / 64;
         int w = 1;
         while (w < kMaxWaves && groups * w < simds + simds / 5)
             w *= 2;
-        return w;
+        return make_shape(w, lockstep_override == 2 && w >= 4 ? 2 : 1);
     }
+    // Waves per 64-object group (every kernel with the column split; only the chunks take
+    // lockstep groups).
+    int waves_for(int64_t n) const { return shape_waves(shape_for(n)); }
 
     // Shapes the tuner tries at n objects. One that cannot give every SIMD a wave
     // (groups * W < simds) leaves most of the chip idle and is not tried, unless it is the
-    // widest. Since simds and the bucket bounds are powers of two, the set is the same for every
-    // size in a tuning bucket.
+    // widest. Lockstep pairs of groups are tried only when asked for (ogp_set_lockstep(2)): on
+    // the ragged mixed 100k workload they measured 12% slower (1,343 against 1,185 ms), and as
+    // candidates they only added noise to the choice. Since simds and the bucket bounds are
+    // powers of two, the set is the same for every size in a tuning bucket.
     std::vector<int> tuning_candidates(int64_t n) const
     {
         const int64_t groups = (n + 63) / 64;
         std::vector<int> c;
         for (int w = 1; w <= kMaxWaves; w *= 2)
             if (groups * w >= simds || w == kMaxWaves)
-                c.push_back(w);
+                c.push_back(make_shape(w, lockstep_override == 2 && w >= 4 ? 2 : 1));
         return c;
     }
 
     bool needs_tuning(int64_t n) const
     {
         return (kind == kGravity || kind == kGravitySplit) && waves_override == 0 && n > 0
-            && tuned_waves.find(group_bucket(n)) == tuned_waves.end();
+            && tuned_shape.find(group_bucket(n)) == tuned_shape.end();
     }
 
     // Acceleration sources added to gravity (ogp_set_sources) and the per-object drag
@@ -475,12 +486,12 @@ struct ogp_context {
         // every launch is bracketed by events on its own stream, for its own duration
         std::vector<std::pair<std::unique_ptr<Event>, std::unique_ptr<Event>>> spans;
         auto launch = [&](hipStream_t st, size_t q, int64_t first, int64_t active, int64_t k0,
-                          int W) {
+                          int shape) {
             const int64_t k1 = std::min<int64_t>(k0 + chunk, groups[q].max_steps + 1);
             spans.emplace_back(std::make_unique<Event>(), std::make_unique<Event>());
             HIPC(hipEventRecord(spans.back().first->e, st));
             launch_chunk(st, kind, fctx[q], ps, first, active, k0, k1, cfg.n_corrections, traj, 1,
-                traj_steps, W);
+                traj_steps, shape_waves(shape), shape_groups(shape));
             check_launch();
             HIPC(hipEventRecord(spans.back().second->e, st));
             launches += 1;
@@ -514,16 +525,16 @@ struct ogp_context {
                     continue;
                 more = true;
                 const int64_t active = active_in(g.first, g.count, k0[q]);
-                // one chunk of every slice of the class with W waves per group; returns the
-                // steps done and appends (launch index, object-steps) of each launch to `out`
-                auto class_chunk = [&](int W, std::vector<std::pair<size_t, double>>* out) {
+                // one chunk of every slice of the class with the given launch shape; returns
+                // the steps done and appends (launch index, object-steps) of each launch to `out`
+                auto class_chunk = [&](int shape, std::vector<std::pair<size_t, double>>* out) {
                     int64_t steps = 0;
                     for (const Lane& l : lanes) {
                         if (l.q != q)
                             continue;
                         const int64_t a = active_in(l.first, l.count, k0[q]);
                         if (a > 0) {
-                            steps = launch(l.st, q, l.first, a, k0[q], W);
+                            steps = launch(l.st, q, l.first, a, k0[q], shape);
                             if (out)
                                 out->emplace_back(spans.size() - 1, (double)a * (double)steps);
                         }
@@ -533,15 +544,19 @@ struct ogp_context {
                 const std::vector<int> cand = needs_tuning(active) ? tuning_candidates(active)
                                                                    : std::vector<int>();
                 if (cand.size() == 1)
-                    tuned_waves[group_bucket(active)] = cand[0];
-                if (cand.size() > 1 && g.max_steps - k0[q] + 1 >= (int64_t)cand.size() * chunk) {
-                    // the next chunks of the class, one per candidate, run as every other chunk
-                    // does (the slices concurrently on their streams); a candidate's rate is the
-                    // sum over slices of object-steps / launch duration, and the fastest is kept.
-                    // The chunks do real work (results do not depend on W).
+                    tuned_shape[group_bucket(active)] = cand[0];
+                if (cand.size() > 1
+                    && g.max_steps - k0[q] + 1 >= 2 * (int64_t)cand.size() * chunk) {
+                    // the next chunks of the class, two per candidate in mirrored order (A B C
+                    // C B A, which cancels a steady drift), run as every other chunk does (the
+                    // slices concurrently on their streams); a candidate's rate is the sum over
+                    // its launches of object-steps / launch duration, and the fastest is kept.
+                    // The chunks do real work (results do not depend on the shape).
                     const int nc = (int)cand.size();
-                    std::vector<std::pair<size_t, double>> trial[kMaxWaves];
+                    std::vector<std::vector<std::pair<size_t, double>>> trial(nc);
                     for (int c = 0; c < nc; ++c)
+                        class_chunk(cand[c], &trial[c]);
+                    for (int c = nc - 1; c >= 0; --c)
                         class_chunk(cand[c], &trial[c]);
                     int best = 0;
                     double best_rate = 0.0;
@@ -559,10 +574,10 @@ struct ogp_context {
                             best = cand[c];
                         }
                     }
-                    tuned_waves[group_bucket(active)] = best;
+                    tuned_shape[group_bucket(active)] = best;
                     continue;
                 }
-                class_chunk(waves_for(active), nullptr);
+                class_chunk(shape_for(active), nullptr);
             }
         }
         join(stream, used);
@@ -1234,7 +1249,26 @@ int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps)
         return fail(ctx, OGP_ERR_ARGUMENT, "waves must be 0, 1, 2, 4 or 8; chunk_steps >= 0");
     ctx->waves_override = waves;
     ctx->chunk = chunk_steps > 0 ? chunk_steps : 64;
-    ctx->tuned_waves.clear();
+    ctx->tuned_shape.clear();
+    return OGP_OK;
+}
+
+int ogp_set_lockstep(ogp_context* ctx, int groups)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    if (groups < 0 || groups > 2)
+        return fail(ctx, OGP_ERR_ARGUMENT, "groups must be 0 (tuned), 1 or 2");
+    ctx->lockstep_override = groups;
+    ctx->tuned_shape.clear();
+    return OGP_OK;
+}
+
+int ogp_get_lockstep(const ogp_context* ctx, int64_t n, int* groups)
+{
+    if (!ctx || !groups)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null argument");
+    *groups = ogp_context::shape_groups(ctx->shape_for(n));
     return OGP_OK;
 }
 

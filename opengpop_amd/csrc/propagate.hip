@@ -175,15 +175,15 @@ template <int W> struct ExchangeArea {
     static constexpr int size = W > 1 ? kParts * 4 * 64 : 1;
 };
 
-// Lane l's object. Lanes past the launch's objects read object obj0 (a valid index) and count
-// as having no steps.
+// Lane l's object in object group `group` of the launch. Lanes past the launch's objects read
+// object obj0 (a valid index) and count as having no steps.
 struct Lane {
     bool valid;
     int64_t i;
 };
-__device__ __forceinline__ Lane lane_object(int64_t obj0, int64_t nobj)
+__device__ __forceinline__ Lane lane_object(int64_t obj0, int64_t nobj, int64_t group)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x % 64;
+    const int64_t t = group * 64 + threadIdx.x % 64;
     return Lane { t < nobj, obj0 + (t < nobj ? t : 0) };
 }
 
@@ -202,7 +202,7 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     __shared__ double red[ExchangeArea<W>::size];
     __shared__ double pos[W > 1 ? kPointLen * 64 : 1]; // the lead's geometry (gravity_gcrs_lead)
     Exchange ex { red, pos };
-    const Lane ln = lane_object(obj0, nobj);
+    const Lane ln = lane_object(obj0, nobj, blockIdx.x);
     const int64_t i = ln.i;
     const ForceCtx f = object_ctx(f0, i);
     const int64_t N = ps.N;
@@ -269,37 +269,52 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
 // GJ8 steps k0 .. k1-1 (each k >= 5), persistent over the chunk: Predictor::Predict
 // (prediction.cu:49-74, dev_functions.cu:20-50) and the corrector (gpu_propagator.cu:86-119,
 // dev_functions.cu:36-69). The acceleration history is a ring of 9 slots in HBM.
+// G object groups share a workgroup in lockstep (G x W waves, one exchange area per group): the
+// barriers are the workgroup's, so the waves summing the same harmonic part of different groups
+// read the same table lines at about the same time. The results do not depend on G.
 #ifndef OGP_CHUNK_WAVES_PER_EU
 #define OGP_CHUNK_WAVES_PER_EU 4 // resident waves per SIMD the register allocation aims for
 #endif
-template <int KIND, int W>
-__global__ void __launch_bounds__(64 * W)
+template <int KIND, int W, int G>
+__global__ void __launch_bounds__(64 * W * G)
 __attribute__((amdgpu_waves_per_eu(OGP_CHUNK_WAVES_PER_EU)))
 gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj, int64_t traj_first,
     int64_t traj_steps)
 {
-    __shared__ double red[ExchangeArea<W>::size];
-    __shared__ double pos[W > 1 ? kPointLen * 64 : 1]; // the lead's geometry (gravity_gcrs_lead)
-    Exchange ex { red, pos };
-    const Lane ln = lane_object(obj0, nobj);
+    static_assert(G == 1 || W > 1, "lockstep groups need the lead/helper form");
+    __shared__ double red[G * ExchangeArea<W>::size];
+    __shared__ double pos[W > 1 ? G * kPointLen * 64 : 1]; // the leads' geometry
+    __shared__ int64_t group_end[G];
+    const int wg = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+    const int g = wg / W, w = wg % W; // object group of the workgroup, wave within the group
+    double* gred = red + (W > 1 ? g * ExchangeArea<W>::size : 0);
+    double* gpos = pos + (W > 1 ? g * kPointLen * 64 : 0);
+    Exchange ex { gred, gpos };
+    const Lane ln = lane_object(obj0, nobj, (int64_t)blockIdx.x * G + g);
     const int64_t i = ln.i;
     const ForceCtx f = object_ctx(f0, i);
     const int64_t N = ps.N;
     const int64_t ns = ln.valid ? ps.nsteps[i] : -1;
-    // the group's last step of the chunk, the same in every wave of the workgroup
-    const int64_t kend = __builtin_amdgcn_readfirstlane(
-        (int)std::min<int64_t>(k1, wave_max(ns) + 1));
+    // the last step of the chunk over the workgroup's groups, the same in every wave
+    int64_t kend = std::min<int64_t>(k1, wave_max(ns) + 1);
+    if constexpr (G > 1) {
+        if (w == 0 && threadIdx.x % 64 == 0)
+            group_end[g] = kend;
+        __syncthreads();
+        for (int q = 0; q < G; ++q)
+            kend = std::max<int64_t>(kend, group_end[q]);
+    }
+    kend = __builtin_amdgcn_readfirstlane((int)kend);
     if (kend <= k0)
         return;
     if constexpr (W > 1) {
-        // Waves 1 .. W-1 only sum their harmonic parts; wave 0 integrates. Two barriers per
-        // evaluation, in every wave for every step up to kend.
-        const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+        // Waves 1 .. W-1 of a group only sum their harmonic parts; its wave 0 integrates. Two
+        // barriers per evaluation, in every wave for every step up to kend.
         if (w > 0) {
             for (int64_t k = k0; k < kend; ++k)
                 for (int it = 0; it < ncorr; ++it)
-                    gravity_helper<KIND == kGravitySplit, W>(f.grav, red, pos, w);
+                    gravity_helper<KIND == kGravitySplit, W>(f.grav, gred, gpos, w);
             return;
         }
     }
@@ -407,16 +422,31 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
 
 void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
     int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
-    int64_t traj_first, int64_t traj_steps, int waves)
+    int64_t traj_first, int64_t traj_steps, int waves, int lockstep)
 {
     if (nobj <= 0 || k1 <= k0)
         return;
-    const dim3 grid((unsigned)((nobj + 63) / 64));
     const int W_ = effective_waves(kind, waves);
+    const int G_ = W_ >= 4 && lockstep == 2 ? 2 : 1;
+    const int64_t ngroups = (nobj + 63) / 64;
+    const dim3 grid((unsigned)((ngroups + G_ - 1) / G_));
 #define L_(K, Wv)                                                                                  \
-    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv>), grid, dim3(64 * (Wv)), 0, st, f, ps, obj0, nobj, \
-        k0, k1, ncorr, traj, traj_first, traj_steps)
-    OGP_DISPATCH(kind, W_, L_);
+    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, 1>), grid, dim3(64 * (Wv)), 0, st, f, ps, obj0,   \
+        nobj, k0, k1, ncorr, traj, traj_first, traj_steps)
+#define L2_(K, Wv)                                                                                 \
+    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, 2>), grid, dim3(64 * 2 * (Wv)), 0, st, f, ps,      \
+        obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps)
+    if (G_ == 2) {
+        const bool split_ = kind == kGravitySplit;
+        if (W_ == 4) {
+            if (split_) L2_(kGravitySplit, 4); else L2_(kGravity, 4);
+        } else {
+            if (split_) L2_(kGravitySplit, 8); else L2_(kGravity, 8);
+        }
+    } else {
+        OGP_DISPATCH(kind, W_, L_);
+    }
+#undef L2_
 #undef L_
 }
 

@@ -218,6 +218,15 @@ class Context:
         check(self.h, lib().ogp_get_sources(self.h, ctypes.byref(s)))
         return s.mask, s.drag_bc
 
+    def set_lockstep(self, groups=0):
+        """Object groups per workgroup of the GJ8 chunks: 0 tuned, 1 or 2."""
+        check(self.h, lib().ogp_set_lockstep(self.h, groups))
+
+    def get_lockstep(self, n):
+        g = ctypes.c_int()
+        check(self.h, lib().ogp_get_lockstep(self.h, n, ctypes.byref(g)))
+        return g.value
+
     def set_slices(self, slices=0):
         """Object slices per step-size class run on concurrent streams (0 = automatic)."""
         check(self.h, lib().ogp_set_slices(self.h, slices))

@@ -261,18 +261,24 @@ def test_launch_shape_does_not_change_results(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])
-def test_ragged_spans_every_wave_count(precision):
+@pytest.mark.parametrize("n", [330, 300])
+def test_ragged_spans_every_wave_count(precision, n):
     # Spans 0 .. 140 spread over every group, so objects end in the startup, at every point of
     # the three 64-step chunks, and next to objects that run on. With 2..8 waves per group only
-    # the first wave integrates, and the others must follow it step for step: the result
-    # equals the one-wave launch bit for bit, and a span of 0 leaves the state unchanged.
-    d = scenarios.mixed(330, seed=17)
-    steps = (np.arange(330) * 37) % 141
+    # the first wave integrates, and the others must follow it step for step to the group's
+    # last step; with two groups in lockstep per workgroup (330 objects: 6 groups, 300: 5, the
+    # last workgroup then holding one group past the objects) every wave follows the
+    # workgroup's last step. The result equals the one-wave launch bit for bit, and a span of 0
+    # leaves the state unchanged.
+    d = scenarios.mixed(n, seed=17)
+    steps = (np.arange(n) * 37) % 141
     with ogp.Context(degree=20, precision=precision) as c:
         out = []
-        for w in (1, 2, 4, 8):
+        for w, g in ((1, 1), (2, 1), (4, 1), (8, 1), (4, 2), (8, 2)):
             c.set_tuning(w, 64)
+            c.set_lockstep(g)
             out.append(c.propagate(d["r"], d["v"], d["step"], steps))
+            assert c.get_tuning(n)[0] == w and c.get_lockstep(n) == g
         for r, v in out[1:]:
             assert np.array_equal(r, out[0][0]) and np.array_equal(v, out[0][1])
         zero = steps == 0
@@ -312,8 +318,9 @@ def test_concurrent_slices_do_not_change_results():
 def test_autotuner_keeps_results_and_records_choice():
     d = scenarios.config0()
     # 640 objects (10 groups): only the widest shape can give every SIMD a wave, so it is taken
-    # without trials; 16,640 objects (260 groups): 4 and 8 waves are timed against each other
-    for reps, choices in ((10, (8,)), (260, (4, 8))):
+    # without trials; 16,640 objects (260 groups): 4 and 8 waves are timed against each other;
+    # 33,280 objects (520 groups): 2, 4 and 8 waves
+    for reps, choices in ((10, (8,)), (260, (4, 8)), (520, (2, 4, 8))):
         r0, v0 = np.tile(d["r"], (reps, 1)), np.tile(d["v"], (reps, 1))
         with ogp.Context(degree=20) as c:
             c.set_tuning(1, 16)
