@@ -224,7 +224,8 @@ def leg_pairs(ogp, scenarios):
     pair_steps = t["pair_steps"]
     gpu = dict(a=np.asarray(res.a), b=np.asarray(res.b), step=np.asarray(res.step),
                distance=np.asarray(res.distance), time=np.asarray(res.time))
-    return dict(_oracle_sample=dict(c4=c4, gpu=gpu),
+    shapes = leg_pairs_shapes(ogp, c4)
+    return dict(_oracle_sample=dict(c4=c4, gpu=gpu), set_layouts=shapes,
         workload="20,000 objects (1,000 forced pairs), set 0 = first 2,000, 3 d at 60 s, "
                  "JGM3 20x20, 10 km screen",
         pairs=int(t["pairs"]), pair_steps=pair_steps, candidates=int(t["candidates"]),
@@ -237,6 +238,40 @@ def leg_pairs(ogp, scenarios):
                              achieved=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12,
                              peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                              frac=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS))
+
+
+def leg_pairs_shapes(ogp, c4):
+    """The same 20,000 objects and 3-day window with other set layouts: one object against the
+    other 19,999, and three sets (8, 1,992 and 18,000 objects). The screen with square tiles,
+    with narrow tiles and with the tuner's choice per set rectangle; the candidate sets must be
+    equal."""
+    out = {}
+    n = len(c4["r"])
+    for name, sizes in (("1_vs_19999", (1, n - 1)), ("sets_8_1992_18000", (8, 1992, n - 2000))):
+        sets = np.repeat(np.arange(len(sizes), dtype=np.uint32), sizes)
+        with ogp.Context(degree=20) as ctx:
+            ca = ogp.ClosestApproach(ctx, c4["step"])
+            ca.initialize(c4["r"], c4["v"], sets)
+            ca.propagate(c4["steps"])
+            rec = {}
+            cands = []
+            for label, shape in (("square", 1), ("narrow", 2), ("tuned", 0)):
+                ctx.set_screen_shape(shape)
+                ca.find_closest(threshold=10e3)  # the tuner's trial, allocations
+                res = ca.find_closest(threshold=10e3)
+                t = ca.timing()
+                rec[label] = dict(screen_ms=t["screen_ms"],
+                                  pair_steps_per_s=t["pair_steps"] / (t["screen_ms"] * 1e-3))
+                cands.append((np.asarray(res.a).tobytes(), np.asarray(res.b).tobytes(),
+                              np.asarray(res.step).tobytes()))
+            rec["tuned_shape_first_set"] = "narrow" if ctx.get_screen_shape(sizes[0]) == 2 else "square"
+            rec["pairs"] = int(t["pairs"])
+            rec["candidates"] = int(t["candidates"])
+            rec["same_candidates"] = all(c == cands[0] for c in cands)
+        out[name] = rec
+        log(f"pairs leg, {name}: square {rec['square']['screen_ms']:.2f} ms, narrow "
+            f"{rec['narrow']['screen_ms']:.2f} ms, tuned {rec['tuned']['screen_ms']:.2f} ms")
+    return out
 
 
 def leg_cpu(sample, threads):

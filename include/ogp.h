@@ -334,6 +334,17 @@ int ogp_conj_results(ogp_context* ctx, uint32_t* obj_a, uint32_t* obj_b, int32_t
 int ogp_conj_find_noalloc(ogp_context* ctx, double threshold_m, int64_t capacity,
     uint32_t* obj_a, uint32_t* obj_b, int32_t* step, double* distance, double* time,
     double* pos_a, double* pos_b, int64_t* count);
+/* Tile shape of the conjunction screen (no counterpart in the reference, which runs one
+ * 256-thread block per pair range): 1 = square tiles of 64 x 64 pairs, 2 = narrow tiles of 4
+ * objects of the lower set against 256 of the higher ones, which stream the higher set's
+ * trajectory once per 4 lower objects (for a small lower set, e.g. a few objects against a
+ * catalogue). 0 (the default) = tuned: the first time a context screens a lower set of at most
+ * 64 objects of a size bucket (per power of two), both shapes screen it against a prefix of the
+ * higher set and the faster one is kept; larger lower sets take square tiles. Results are
+ * bit-identical for every choice. ogp_get_screen_shape reports the shape for a lower set of na
+ * objects. */
+int ogp_set_screen_shape(ogp_context* ctx, int shape);
+int ogp_get_screen_shape(const ogp_context* ctx, int64_t na, int* shape);
 /* Sampled screening minimum of every cross-set pair (pair index order). */
 int ogp_conj_screen(ogp_context* ctx, double* min_dist, int32_t* min_step);
 int ogp_conj_num_pairs(const ogp_context* ctx, int64_t* num_pairs);

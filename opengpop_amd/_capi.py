@@ -99,6 +99,8 @@ def lib():
         "ogp_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, _I64]),
         "ogp_set_slices": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_set_lockstep": (ctypes.c_int, [_P, ctypes.c_int]),
+        "ogp_set_screen_shape": (ctypes.c_int, [_P, ctypes.c_int]),
+        "ogp_get_screen_shape": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int)]),
         "ogp_get_lockstep": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int)]),
         "ogp_set_derivative_memory": (ctypes.c_int, [_P, _I64]),
         "ogp_set_sources": (ctypes.c_int, [_P, ctypes.POINTER(OgpSources)]),

@@ -155,6 +155,33 @@ struct ogp_context {
         ad_lam, ad_seeds, ad_out;
     ogp_adjoint_timing ad_timing {};
 
+    // Screen tile shape per set-rectangle (conjunction.hpp: kScreenSquare, kScreenNarrow),
+    // tuned per ceil(log2(set-A size)) for set A of at most 64 objects (one square tile); the
+    // override of ogp_set_screen_shape (0 = tuned).
+    std::map<int, int> tuned_screen;
+    int screen_override = 0;
+    static int screen_bucket(int64_t na)
+    {
+        int b = 0;
+        while ((int64_t(1) << b) < na)
+            ++b;
+        return b;
+    }
+    // The shape for a rectangle with set A of na objects: a set A of more than 64 objects fills
+    // square tiles (a narrow tile would stream set B once per 4 A objects); below that the tuned
+    // choice, else narrow up to 8 objects.
+    int screen_shape(int64_t na) const
+    {
+        if (screen_override > 0)
+            return screen_override;
+        if (na > 64)
+            return kScreenSquare;
+        auto it = tuned_screen.find(screen_bucket(na));
+        if (it != tuned_screen.end())
+            return it->second;
+        return na <= 8 ? kScreenNarrow : kScreenSquare;
+    }
+
     // Streams of the concurrent GJ8 chunk sequences (object slices), default priority.
     static constexpr int kLaneStreams = 4;
     hipStream_t lane_streams[kLaneStreams] = {};
@@ -1253,6 +1280,25 @@ int ogp_set_tuning(ogp_context* ctx, int waves, int64_t chunk_steps)
     return OGP_OK;
 }
 
+int ogp_set_screen_shape(ogp_context* ctx, int shape)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    if (shape < 0 || shape > 2)
+        return fail(ctx, OGP_ERR_ARGUMENT, "shape must be 0 (tuned), 1 (square) or 2 (narrow)");
+    ctx->screen_override = shape;
+    ctx->tuned_screen.clear();
+    return OGP_OK;
+}
+
+int ogp_get_screen_shape(const ogp_context* ctx, int64_t na, int* shape)
+{
+    if (!ctx || !shape)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null argument");
+    *shape = ctx->screen_shape(na);
+    return OGP_OK;
+}
+
 int ogp_set_lockstep(ogp_context* ctx, int groups)
 {
     if (!ctx)
@@ -1901,6 +1947,32 @@ int ogp_conj_num_pairs(const ogp_context* ctx, int64_t* num_pairs)
 
 namespace {
 
+// The screen tuner: the first time the context meets a set A of this size bucket (at most 64
+// objects, beside at least 1,024 B objects), it screens A against a prefix of set B (up to 16,384
+// objects) with each tile shape, writing nothing, times both on the context stream and keeps the
+// faster. The shapes give the same bits, so the choice only changes the speed.
+void tune_screen(ogp_context* ctx, int64_t a0, int64_t na, int64_t b0, int64_t nb)
+{
+    if (ctx->screen_override > 0 || na > 64 || nb < 1024
+        || ctx->tuned_screen.count(ogp_context::screen_bucket(na)))
+        return;
+    const int64_t pb = std::min<int64_t>(nb, 16384);
+    const int shapes[2] = { kScreenSquare, kScreenNarrow };
+    float ms[2] = { 0, 0 };
+    for (int q = 0; q < 2; ++q) {
+        Event e0, e1;
+        HIPC(hipEventRecord(e0.e, ctx->stream));
+        launch_screen(ctx->stream, ctx->cj_traj.as<double>(), ctx->cj_n, ctx->cj_steps, a0, na, b0,
+            pb, 0, 0.0, nullptr, nullptr, ctx->cj_cnt.as<unsigned long long>(), 0, nullptr,
+            nullptr, nullptr, nullptr, shapes[q]);
+        ctx->check_launch();
+        HIPC(hipEventRecord(e1.e, ctx->stream));
+        HIPC(hipEventSynchronize(e1.e));
+        HIPC(hipEventElapsedTime(&ms[q], e0.e, e1.e));
+    }
+    ctx->tuned_screen[ogp_context::screen_bucket(na)] = ms[1] < ms[0] ? kScreenNarrow : kScreenSquare;
+}
+
 // Screens every set rectangle; all_pairs writes the per-pair minima, threshold collects
 // candidates. Returns the number of candidates.
 int64_t conj_screen(ogp_context* ctx, double threshold, bool all_pairs)
@@ -1926,12 +1998,13 @@ int64_t conj_screen(ogp_context* ctx, double threshold, bool all_pairs)
         for (size_t s = 0; s + 1 < off.size(); ++s) {
             const int64_t a0 = off[s], na = off[s + 1] - off[s];
             const int64_t b0 = off[s + 1], nb = n - off[s + 1];
+            tune_screen(ctx, a0, na, b0, nb);
             launch_screen(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps, a0, na, b0, nb, pair0,
                 all_pairs ? 0.0 : threshold, all_pairs ? ctx->cj_mind.as<double>() : nullptr,
                 all_pairs ? ctx->cj_mink.as<int32_t>() : nullptr,
                 ctx->cj_cnt.as<unsigned long long>(), cap,
                 ctx->cj_a.as<uint32_t>(), ctx->cj_b.as<uint32_t>(), ctx->cj_k.as<int32_t>(),
-                ctx->cj_d.as<double>());
+                ctx->cj_d.as<double>(), ctx->screen_shape(na));
             ctx->check_launch();
             pair0 += na * nb;
         }
@@ -2086,7 +2159,8 @@ int ogp_conj_find_noalloc(ogp_context* ctx, double threshold_m, int64_t capacity
             const int64_t a0 = off[s], na = off[s + 1] - off[s];
             const int64_t b0 = off[s + 1], nb = n - off[s + 1];
             launch_screen(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps, a0, na, b0, nb, pair0,
-                threshold_m, nullptr, nullptr, counter, kNoAllocMaxCandidates, ca, cb, ck, cd);
+                threshold_m, nullptr, nullptr, counter, kNoAllocMaxCandidates, ca, cb, ck, cd,
+                ctx->screen_shape(na));
             pair0 += na * nb;
         }
         ctx->check_launch();

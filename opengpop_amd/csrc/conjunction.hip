@@ -257,22 +257,189 @@ screen_kernel(const double* __restrict__ traj,
         }
 }
 
+// Narrow tiles for a small set A: thread t of block (x, y) screens B object b0 + 256 x + t
+// against the NA objects a0 + NA y .. a0 + NA y + NA - 1. Each B object's positions are read
+// once per A tile, straight from HBM into the thread's registers (the next stage's loads fly
+// while the current one is screened); the NA objects' positions are staged through LDS. Where
+// set A holds a few objects, a 64 x 64 tile would leave most of its pairs empty; here the
+// kernel streams set B's trajectory about once. The decision rule and the squared distance
+// are screen_kernel's, so the minima and steps are the same bits.
+constexpr int NA = 4, NB_THREADS = 256;
+template <int WIN>
+__global__ void __launch_bounds__(NB_THREADS)
+screen_narrow_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps, int64_t a0,
+    int64_t na, int64_t b0, int64_t nb, int64_t pair0, int64_t row_len, double threshold,
+    double* __restrict__ min_dist, int32_t* __restrict__ min_step,
+    unsigned long long* __restrict__ cand_count, int64_t cand_cap, uint32_t* __restrict__ cand_a,
+    uint32_t* __restrict__ cand_b, int32_t* __restrict__ cand_k, double* __restrict__ cand_d)
+{
+    constexpr int SPAN = STAGE * WIN;
+    __shared__ double sA[STAGE][3][NA];
+    const int tid = threadIdx.x;
+    const int64_t ta0 = a0 + (int64_t)blockIdx.y * NA;
+    const int64_t alim = a0 + na, blim = b0 + nb;
+    const int64_t jb = b0 + (int64_t)blockIdx.x * NB_THREADS + tid;
+    const bool bvalid = jb < blim;
+    const int64_t jl = bvalid ? jb : b0; // lanes past set B read object b0 and write nothing
+    // A stage element e (< STAGE * 3 * NA): step e / (3 NA), component (e / NA) % 3, object e % NA
+    auto a_load = [&](int64_t k0, int e) {
+        const int64_t k = k0 + e / (3 * NA);
+        const int c = (e / NA) % 3;
+        const int64_t ia = ta0 + e % NA;
+        return k < nsteps && ia < alim ? traj[(k * 9 + c) * N + ia] : 0.0;
+    };
+    auto b_load = [&](int64_t k0, double* dst) {
+#pragma unroll
+        for (int st = 0; st < STAGE; ++st)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int64_t k = k0 + st;
+                dst[3 * st + c] = k < nsteps ? traj[(k * 9 + c) * N + jl] : 0.0;
+            }
+    };
+    double mn[NA], wm[NA];
+    int32_t sk[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        mn[i] = __builtin_inf();
+        sk[i] = -1;
+    }
+    double cur[3 * STAGE], pre[3 * STAGE];
+    double apre = tid < STAGE * 3 * NA ? a_load(0, tid) : 0.0;
+    b_load(0, pre);
+    for (int64_t k0 = 0; k0 < nsteps; k0 += STAGE) {
+        __syncthreads(); // the previous stage has been screened by every thread
+        if (tid < STAGE * 3 * NA)
+            (&sA[0][0][0])[tid] = apre;
+#pragma unroll
+        for (int q = 0; q < 3 * STAGE; ++q)
+            cur[q] = pre[q];
+        __syncthreads();
+        if (k0 + STAGE < nsteps) { // flies while this stage is screened
+            if (tid < STAGE * 3 * NA)
+                apre = a_load(k0 + STAGE, tid);
+            b_load(k0 + STAGE, pre);
+        }
+        if (k0 % SPAN == 0) {
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+                wm[i] = __builtin_inf();
+        }
+        const int smax = (int)((nsteps - k0) < STAGE ? (nsteps - k0) : STAGE);
+        auto screen_step = [&](int st) {
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {
+                const double dx = sA[st][0][i] - cur[3 * st];
+                const double dy = sA[st][1][i] - cur[3 * st + 1];
+                const double dz = sA[st][2][i] - cur[3 * st + 2];
+                const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                wm[i] = min_f64(wm[i], d2);
+            }
+        };
+        if (smax == STAGE) {
+#pragma unroll
+            for (int st = 0; st < STAGE; ++st)
+                screen_step(st);
+        } else {
+#pragma unroll
+            for (int st = 0; st < STAGE; ++st)
+                if (st < smax)
+                    screen_step(st);
+        }
+        if ((k0 + STAGE) % SPAN == 0 || k0 + STAGE >= nsteps) { // end of a window
+            const int32_t win = (int32_t)(k0 / SPAN);
+            bool unsure = false;
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {
+                const bool drop = wm[i] < mn[i] * kShrink;
+                unsure |= (wm[i] < mn[i]) & !drop;
+                sk[i] = drop ? win : sk[i];
+            }
+            if (unsure) {
+#pragma unroll
+                for (int i = 0; i < NA; ++i)
+                    if (wm[i] < mn[i] && __builtin_sqrt(wm[i]) < __builtin_sqrt(mn[i]))
+                        sk[i] = win;
+            }
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+                mn[i] = min_f64(mn[i], wm[i]);
+        }
+    }
+    if (!bvalid)
+        return;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        const int64_t ia = ta0 + i;
+        if (ia >= alim)
+            continue;
+        const double d = sk[i] < 0 ? 1.7976931348623157e308 : __builtin_sqrt(mn[i]);
+        const bool cand = threshold > 0.0 && d < threshold;
+        if (!min_dist && !cand)
+            continue;
+        int32_t k = -1;
+        if (sk[i] >= 0) { // first sample of that window whose root is the minimum
+            const int64_t kb = (int64_t)sk[i] * SPAN;
+            const int64_t ke = kb + SPAN < nsteps ? kb + SPAN : nsteps;
+            for (int64_t kk = kb; kk < ke; ++kk) {
+                const double* p = traj + kk * 9 * N;
+                const double dx = p[ia] - p[jb];
+                const double dy = p[N + ia] - p[N + jb];
+                const double dz = p[2 * N + ia] - p[2 * N + jb];
+                const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                if (__builtin_sqrt(d2) == d) {
+                    k = (int32_t)kk;
+                    break;
+                }
+            }
+        }
+        const int64_t p = pair0 + (ia - a0) * row_len + (jb - b0);
+        if (min_dist) {
+            min_dist[p] = d;
+            min_step[p] = k;
+        }
+        if (cand) {
+            const unsigned long long slot = atomicAdd(cand_count, 1ull);
+            if (slot < (unsigned long long)cand_cap) {
+                cand_a[slot] = (uint32_t)ia;
+                cand_b[slot] = (uint32_t)jb;
+                cand_k[slot] = k;
+                cand_d[slot] = d;
+            }
+        }
+    }
+}
+
 void launch_screen(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, int64_t a0,
     int64_t na, int64_t b0, int64_t nb, int64_t pair0, double threshold, double* min_dist,
     int32_t* min_step, unsigned long long* cand_count, int64_t cand_cap, uint32_t* cand_a,
-    uint32_t* cand_b, int32_t* cand_k, double* cand_d)
+    uint32_t* cand_b, int32_t* cand_k, double* cand_d, int shape, int64_t row_len)
 {
     if (na <= 0 || nb <= 0 || nsteps <= 0)
         return;
-    const dim3 grid((unsigned)((nb + TB - 1) / TB), (unsigned)((na + TA - 1) / TA));
+    if (row_len <= 0)
+        row_len = nb;
     // every pair written out: short windows; candidates only: long ones
+    if (shape == kScreenNarrow) {
+        const dim3 grid((unsigned)((nb + NB_THREADS - 1) / NB_THREADS), (unsigned)((na + NA - 1) / NA));
+        if (min_dist)
+            hipLaunchKernelGGL(screen_narrow_kernel<1>, grid, dim3(NB_THREADS), 0, st, traj, N,
+                nsteps, a0, na, b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count,
+                cand_cap, cand_a, cand_b, cand_k, cand_d);
+        else
+            hipLaunchKernelGGL(screen_narrow_kernel<4>, grid, dim3(NB_THREADS), 0, st, traj, N,
+                nsteps, a0, na, b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count,
+                cand_cap, cand_a, cand_b, cand_k, cand_d);
+        return;
+    }
+    const dim3 grid((unsigned)((nb + TB - 1) / TB), (unsigned)((na + TA - 1) / TA));
     if (min_dist)
         hipLaunchKernelGGL(screen_kernel<1>, grid, dim3(THREADS), 0, st, traj, N, nsteps, a0, na,
-            b0, nb, pair0, nb, threshold, min_dist, min_step, cand_count, cand_cap, cand_a,
+            b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count, cand_cap, cand_a,
             cand_b, cand_k, cand_d);
     else
         hipLaunchKernelGGL(screen_kernel<4>, grid, dim3(THREADS), 0, st, traj, N, nsteps, a0, na,
-            b0, nb, pair0, nb, threshold, min_dist, min_step, cand_count, cand_cap, cand_a,
+            b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count, cand_cap, cand_a,
             cand_b, cand_k, cand_d);
 }
 

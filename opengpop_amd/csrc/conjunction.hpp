@@ -7,13 +7,17 @@ namespace ogp {
 
 // Screens the pair rectangle A = [a0, a0 + na) x B = [b0, b0 + nb) over steps [0, nsteps) of
 // the stored trajectory traj[step][9][N] (components 0-2 are positions). Pair (ia, jb) gets
-// index pair0 + (ia - a0) * nb + (jb - b0) (partition_parameters.cuh:58-77). min_dist /
-// min_step may be null; with threshold > 0 pairs below the threshold are appended to the
-// candidate arrays (at most cand_cap; *cand_count counts all).
+// index pair0 + (ia - a0) * row_len + (jb - b0) (partition_parameters.cuh:58-77; row_len <= 0
+// means nb). min_dist / min_step may be null; with threshold > 0 pairs below the threshold are
+// appended to the candidate arrays (at most cand_cap; *cand_count counts all).
+// shape: kScreenSquare (64 x 64 pair tiles) or kScreenNarrow (4 A objects x 256 B objects);
+// the results are the same bits.
+enum { kScreenSquare = 1, kScreenNarrow = 2 };
 void launch_screen(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, int64_t a0,
     int64_t na, int64_t b0, int64_t nb, int64_t pair0, double threshold, double* min_dist,
     int32_t* min_step, unsigned long long* cand_count, int64_t cand_cap, uint32_t* cand_a,
-    uint32_t* cand_b, int32_t* cand_k, double* cand_d);
+    uint32_t* cand_b, int32_t* cand_k, double* cand_d, int shape = kScreenSquare,
+    int64_t row_len = 0);
 
 // Refines n candidates (a, b, sampled minimum step k and distance d). out: distance, time
 // (t0 + seconds), position of a, position of b at the TCA; field_stride = 0: 8 doubles per

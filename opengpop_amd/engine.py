@@ -218,6 +218,15 @@ class Context:
         check(self.h, lib().ogp_get_sources(self.h, ctypes.byref(s)))
         return s.mask, s.drag_bc
 
+    def set_screen_shape(self, shape=0):
+        """Conjunction screen tiles: 0 tuned, 1 square (64 x 64), 2 narrow (4 x 256)."""
+        check(self.h, lib().ogp_set_screen_shape(self.h, shape))
+
+    def get_screen_shape(self, na):
+        g = ctypes.c_int()
+        check(self.h, lib().ogp_get_screen_shape(self.h, na, ctypes.byref(g)))
+        return g.value
+
     def set_lockstep(self, groups=0):
         """Object groups per workgroup of the GJ8 chunks: 0 tuned, 1 or 2."""
         check(self.h, lib().ogp_set_lockstep(self.h, groups))

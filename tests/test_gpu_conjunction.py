@@ -193,3 +193,49 @@ def test_candidate_pairs_bit_exact_vs_oracle_end_to_end(po):
         g = gpu.get(a, b)
         assert abs(g["distance"] - ref["distance"]) <= 1.0
         assert abs(g["time"] - ref["time"]) <= 1e-3
+
+
+@pytest.mark.parametrize("sizes", [(1, 1199), (3, 597, 600), (5, 40, 1155), (70, 1130)])
+def test_screen_tile_shapes_bit_identical(sizes, po):
+    # square (64 x 64) and narrow (4 x 256) tiles on the same trajectory: every pair's sampled
+    # minimum and step equal between the shapes and to the oracle's per-pair screen, and the
+    # threshold search returns the same records; a one-object set, small sets and three sets
+    n = sum(sizes)
+    r, v = scenarios.leo(n, seed=31)
+    r[1:4] = r[0] + [[50.0, 0, 0], [0, 80.0, 0], [0, 0, 120.0]]  # near pairs across the sets
+    sets = np.repeat(np.arange(len(sizes), dtype=np.uint32), sizes)
+    out = {}
+    with ogp.Context(degree=20) as c:
+        ca = ogp.ClosestApproach(c, 60.0)
+        ca.initialize(r, v, sets)
+        ca.propagate(77)  # ends in a partial stage of 8 steps
+        traj = ca.trajectory()
+        for shape in (1, 2):
+            c.set_screen_shape(shape)
+            assert c.get_screen_shape(sizes[0]) == shape
+            out[shape] = (ca.screen(), ca.find_closest(threshold=2e6))
+    (d1, k1), f1 = out[1]
+    (d2, k2), f2 = out[2]
+    assert np.array_equal(d1, d2) and np.array_equal(k1, k2)
+    assert len(f1) == len(f2) > 0
+    for a in ("a", "b", "step", "distance", "time", "pos_a", "pos_b"):
+        assert np.array_equal(getattr(f1, a), getattr(f2, a)), a
+    ref_d, ref_k = po.Oracle(degree=2).screen(traj, sets)
+    assert np.array_equal(d1, ref_d) and np.array_equal(k1, ref_k)
+
+
+def test_screen_tuner_records_choice():
+    # a one-object set against 2,047: the tuner times both shapes once and keeps one; the
+    # results do not depend on it
+    r, v = scenarios.leo(2048, seed=5)
+    sets = np.ones(2048, np.uint32)
+    sets[0] = 0
+    with ogp.Context(degree=20) as c:
+        ca = ogp.ClosestApproach(c, 60.0)
+        ca.initialize(r, v, sets)
+        ca.propagate(40)
+        tuned = ca.screen()
+        assert c.get_screen_shape(1) in (1, 2)
+        c.set_screen_shape(1)
+        square = ca.screen()
+    assert np.array_equal(tuned[0], square[0]) and np.array_equal(tuned[1], square[1])
