@@ -1,7 +1,7 @@
 """Benchmark of the propagation and conjunction hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--legs fp64,split,pairs,adjoint,cpu,refpoint,sources,perturb,copy]
+                    [--legs fp64,split,pairs,layouts,adjoint,cpu,refpoint,sources,perturb,copy]
 
 Prints ONE JSON line. The headline `value` is FP64 object-steps/s of the mixed-regime workload:
 100,000 objects (80% LEO at 30 s, 15% MEO and 5% GEO at 300 s, spans U(6, 24) h), JGM3
@@ -11,6 +11,8 @@ complete propagation of that population. Side legs (same line):
             the split path's final-position error against the FP64 GPU path for all objects
   pairs     20,000 objects with 1,000 forced encounters, set 0 = first 2,000, 3 days at 60 s,
             degree 20: propagation, screen (10 km) and refinement; pair-steps/s of the screen
+  layouts   the same objects as one object against the other 19,999 and as three sets (8,
+            1,992, 18,000): the screen with square tiles, narrow tiles and the tuned choice
   adjoint   10,000 LEO, 60 s, 24 h, degree 20: the 6x6 d(final)/d(initial) per object by the
             discrete adjoint and by the variational equations; agreement and cost vs the forward
             propagation
@@ -44,7 +46,8 @@ METRIC = "object-steps/s at deg-70 gravity: FP64 vs split-prec (≤1 m @7 d); pa
 CPU_SAMPLE_STRIDE = 40  # cpu leg: every 40th object of the headline workload (~12 s on 16 threads)
 CPU_SINGLE_STRIDE = 800  # single-core rate: every 800th object (~8 s on one thread)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (equal to its FP64 matrix figure)
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3  # packed FP32 (v_pk_fma_f32), which these builds never emit
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # non-packed FP32 issues at the FP64 rate
 PEAK_HBM_GBS = 8000.0
 
 
@@ -63,17 +66,32 @@ def flops_step(L, k=7):
     return k * flops_eval(L) + 228 + 33 * k
 
 
-def measured_traffic_per_object_step():
-    """HBM bytes per GJ8 object-step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (calibrated against a copy of known size), or None when it is absent."""
-    path = os.path.join(ROOT, "profiles", "gj8_chunk_mixed100k_deg70_hbm_bytes.md")
+def profile_value(name, key):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/, or None."""
     try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_object_step:"):
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic_per_object_step():
+    """HBM bytes per GJ8 object-step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated against a copy of known size), or None when it is absent."""
+    return profile_value("gj8_chunk_mixed100k_deg70_hbm_bytes.md", "traffic_bytes_per_object_step")
+
+
+def hw_rate(profile, key, units, seconds, peak):
+    """Issued FLOP rate from the hardware counters: the committed summary's FLOPs per unit
+    (SQ_INSTS_VALU_FLOPS_* x 64 lanes over a profiled run of the same workload) times the units
+    this run processed, over this run's time."""
+    per = profile_value(profile, key)
+    if per is None or seconds <= 0:
+        return None
+    rate = per * units / seconds / 1e12
+    return dict(flop_per_unit=per, tflops=rate, frac=rate / peak, source="profiles/" + profile)
 
 
 def parse():
@@ -81,7 +99,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--legs", default="fp64,split,pairs,adjoint,cpu,refpoint,sources,perturb")
+    p.add_argument("--legs",
+                   default="fp64,split,pairs,layouts,adjoint,cpu,refpoint,sources,perturb")
     p.add_argument("--cpu-threads", type=int, default=0,
                    help="0 = min(16, CPUs in the affinity mask)")
     return p.parse_args()
@@ -148,7 +167,10 @@ def leg_fp64(ogp, scenarios, torch, args):
                                      "(= GJ8-phase FLOPs / GJ8-phase elapsed time)",
                       object_steps_per_launch=gj / launches,
                       traffic_unit="bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
-                      algorithmic_bytes_per_launch=9.75 * gj / launches),
+                      algorithmic_bytes_per_launch=9.75 * gj / launches,
+                      hw_fp64=hw_rate("gj8_chunk_mixed100k_deg70_counters.md",
+                                      "hw_fp64_flop_per_object_step", gj, steps_ms * 1e-3,
+                                      PEAK_FP64_TFLOPS)),
         object_steps=obj_steps,
         tuning={str(b): ctx.get_tuning(64 * 2 ** b)[0] for b in range(0, 11)},
     )
@@ -193,10 +215,30 @@ def leg_split(ogp, scenarios, args):
     per_launch = s["gj"] / s["launches"]
     bound_s = (f32 / (PEAK_FP32_TFLOPS * 1e12) + f64 / (PEAK_FP64_TFLOPS * 1e12)) * per_launch
     # chip-level: the GJ8 phase's bound time over its elapsed time (slices run concurrently)
+    prof = "gj8_chunk_split_leo50k_7d_deg70_counters.md"
+    hw32 = hw_rate(prof, "hw_fp32_flop_per_object_step_split", s["gj"], s["gj8_ms"] * 1e-3,
+                   PEAK_FP32_TFLOPS)
+    hw64 = hw_rate(prof, "hw_fp64_flop_per_object_step_split", s["gj"], s["gj8_ms"] * 1e-3,
+                   PEAK_FP64_TFLOPS)
+    # non-packed FP32 (the only FP32 form these builds emit) issues at the FP64 rate
+    nonpacked = None
+    if hw32 and hw64:
+        nonpacked = (hw32["tflops"] + hw64["tflops"]) / PEAK_FP32_NONPACKED_TFLOPS
     out["split_roofline"] = dict(frac=bound_s / t_launch, fp32_flops_per_object_step=f32,
                                  fp64_flops_per_object_step=f64,
                                  elapsed_ms_per_launch=t_launch * 1e3,
-                                 avg_launch_ms=s["launch_ms_sum"] / s["launches"])
+                                 avg_launch_ms=s["launch_ms_sum"] / s["launches"],
+                                 frac_basis="algorithmic FP32 at the packed 157.3 TFLOP/s peak, "
+                                            "FP64 at 78.6",
+                                 algorithmic_frac_nonpacked_fp32=(f32 + f64) / (
+                                     PEAK_FP32_NONPACKED_TFLOPS * 1e12) * per_launch / t_launch,
+                                 hw_fp32=hw32, hw_fp64=hw64,
+                                 hw_issue_frac_nonpacked=nonpacked,
+                                 ceiling=("non-packed FP32 issues at the FP64 rate and the split "
+                                          "path issues as many FLOPs per object-step as the "
+                                          "FP64 path, so its ceiling is the FP64 path's time "
+                                          "without waits: about 1/0.76 = 1.3-1.4x the FP64 "
+                                          "path, not 3x"))
     # 32 objects for the cpu leg's check against the oracle
     k = np.arange(0, n, n // 32)[:32]
     out["_oracle_sample"] = dict(r0=c2["r"][k], v0=c2["v"][k], step=c2["step"][k],
@@ -222,10 +264,11 @@ def leg_pairs(ogp, scenarios):
         t2 = time.perf_counter()
         t = ca.timing()
     pair_steps = t["pair_steps"]
+    prop_gj = float(len(c4["r"])) * (c4["steps"] - 4)
     gpu = dict(a=np.asarray(res.a), b=np.asarray(res.b), step=np.asarray(res.step),
                distance=np.asarray(res.distance), time=np.asarray(res.time))
-    shapes = leg_pairs_shapes(ogp, c4)
-    return dict(_oracle_sample=dict(c4=c4, gpu=gpu), set_layouts=shapes,
+
+    return dict(_oracle_sample=dict(c4=c4, gpu=gpu),
         workload="20,000 objects (1,000 forced pairs), set 0 = first 2,000, 3 d at 60 s, "
                  "JGM3 20x20, 10 km screen",
         pairs=int(t["pairs"]), pair_steps=pair_steps, candidates=int(t["candidates"]),
@@ -237,7 +280,18 @@ def leg_pairs(ogp, scenarios):
         screen_roofline=dict(bound="valu",
                              achieved=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12,
                              peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                             frac=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS))
+                             frac=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                             hw_fp64=hw_rate("screen_pairs20k_3d_counters.md",
+                                             "hw_fp64_flop_per_pair_step", pair_steps,
+                                             t["screen_ms"] * 1e-3, PEAK_FP64_TFLOPS)),
+        propagate_roofline=dict(
+            bound="valu", unit="TFLOP/s", peak=PEAK_FP64_TFLOPS,
+            flops_per_object_step=flops_step(20),
+            achieved=flops_step(20) * prop_gj / (t["propagate_ms"] * 1e-3) / 1e12,
+            frac=flops_step(20) * prop_gj / (t["propagate_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+            hw_fp64=hw_rate("gj8_chunk_conj20k_3d_deg20_counters.md",
+                            "hw_fp64_flop_per_object_step", prop_gj, t["propagate_ms"] * 1e-3,
+                            PEAK_FP64_TFLOPS)))
 
 
 def leg_pairs_shapes(ogp, c4):
@@ -436,6 +490,7 @@ def leg_adjoint(ogp, scenarios):
         _, _, Phi = ctx.stm(r0, v0, h, steps)
         stm_ms = ctx.adjoint_timing()["stm_ms"]
     rel = np.linalg.norm(J - Phi, axis=(1, 2)) / np.linalg.norm(Phi, axis=(1, 2))
+    gj = float(np.maximum(steps - 4, 0).sum())
     out = dict(workload="10,000 LEO, 60 s, 1,440 steps (24 h), JGM3 20x20, 7 corrector "
                         "evaluations per step; Jacobians recorded in the forward pass",
                forward_ms=fwd_ms,
@@ -447,7 +502,18 @@ def leg_adjoint(ogp, scenarios):
                jacobian_cost_vs_forward=(ta["forward_ms"] + ta["adjoint_ms"]) / fwd_ms,
                vjp_cost_vs_forward=(tv["forward_ms"] + tv["adjoint_ms"]) / fwd_ms,
                stm_cost_vs_forward=stm_ms / fwd_ms,
-               adjoint_vs_stm_rel=dict(max=float(rel.max()), median=float(np.median(rel))))
+               adjoint_vs_stm_rel=dict(max=float(rel.max()), median=float(np.median(rel))),
+               forward_roofline=dict(
+                   bound="valu", unit="TFLOP/s", peak=PEAK_FP64_TFLOPS,
+                   achieved=flops_step(20) * gj / (fwd_ms * 1e-3) / 1e12,
+                   frac=flops_step(20) * gj / (fwd_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                   basis="algorithmic FLOPs of the GJ8 steps over the whole propagation "
+                         "(startup included)"),
+               # the Jacobian's forward pass is the record pass over the whole span (its
+               # variational startup included in the time)
+               record_hw_fp64=hw_rate("gj8_record_adjoint10k_24h_deg20_counters.md",
+                                      "hw_fp64_flop_per_record_object_step", gj,
+                                      ta["forward_ms"] * 1e-3, PEAK_FP64_TFLOPS))
     # two objects for the cpu leg's central differences of the oracle's propagation
     k = [0, 5000]
     out["_oracle_sample"] = dict(r0=r0[k], v0=v0[k], h=h, steps=steps[k], J=J[k])
@@ -564,6 +630,8 @@ def main():
         line["pairs"] = leg_pairs(ogp, scenarios)
         pairs_sample = line["pairs"].pop("_oracle_sample")
         log(f"pairs leg: {line['pairs']['pair_steps_per_s']:.3e} pair-steps/s")
+    if "layouts" in legs:
+        line.setdefault("pairs", {})["set_layouts"] = leg_pairs_shapes(ogp, scenarios.config4())
     if "adjoint" in legs:
         line["adjoint"] = leg_adjoint(ogp, scenarios)
         adjoint_sample = line["adjoint"].pop("_oracle_sample")
