@@ -731,9 +731,18 @@ int ogp_create(const ogp_config* cfg, ogp_context** out)
             c.kind = cfg->precision == OGP_PRECISION_SPLIT ? kGravitySplit : kGravity;
             c.grav = load_gravity(c.data_dir + "/JGM3.gfc", cfg->degree);
             c.frames = load_frames(c.data_dir);
-            upload(&c, c.d_rec, c.grav.records);
-            upload(&c, c.d_recf, c.grav.records_f);
-            upload(&c, c.d_diag, c.grav.diag_ratio);
+            // zero records after each table (column_terms fetches a column's last batch
+            // whole) and zero entries after the diagonal ratios (read one column ahead, also
+            // by an empty part that starts at column L + 1)
+            std::vector<double> rec = c.grav.records;
+            std::vector<float> recf = c.grav.records_f;
+            rec.resize(rec.size() + kRecordPad * kRecordLen, 0.0);
+            recf.resize(recf.size() + kRecordPad * kRecordLen, 0.0f);
+            std::vector<double> diag = c.grav.diag_ratio;
+            diag.resize(diag.size() + 2, 0.0);
+            upload(&c, c.d_rec, rec);
+            upload(&c, c.d_recf, recf);
+            upload(&c, c.d_diag, diag);
             c.gdev.rec = c.d_rec.as<double>();
             c.gdev.recf = c.d_recf.as<float>();
             c.gdev.diag = c.d_diag.as<double>();

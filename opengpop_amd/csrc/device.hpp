@@ -29,7 +29,7 @@ constexpr int kParts = 8;
 struct GravityDev {
     const double* rec;  // kRecordLen doubles per record, see datasets.hpp (GravityModel::records)
     const float* recf;  // the same records in FP32
-    const double* diag; // Pbar_mm / Pbar_{m-1,m-1}
+    const double* diag; // Pbar_mm / Pbar_{m-1,m-1}, m = 0..L, and two zero entries after
     const int* parts;   // kParts + 1 column bounds: part q = columns [parts[q], parts[q + 1])
     const double* part_diag; // per part q: D_m = prod_{j=1..m} diag[j] at m = parts[q] (D_0 = 1)
     int L;              // degree and order
@@ -52,33 +52,30 @@ __device__ __forceinline__ double fmat(double a, double b, double c) { return __
 #define OGP_RECORD_BATCH 4 // gravity records per scalar-load wait
 #endif
 
+// Zero records after the last one of each table: a column's last batch is fetched whole.
+constexpr int kRecordPad = OGP_RECORD_BATCH - 1;
+
 // One column m of records [k0, k1): k0 is the diagonal n = m (accumulated only), the rest run
 // the recursion U_n = (xi'_n * eps * rho) U_{n-1} - rho^2 U_{n-2} on the scaled, rescaled
 // functions (the rescaling constants are folded into the coefficients, see GravityModel).
 // Nine instructions per record: two multiplications and seven FMAs.
 // T: type of the table entries; V: type of the per-object values.
+//
+// Records are fetched BATCH at a time into scalar registers and then consumed, one wait for the
+// scalar loads per batch. The diagonal record opens the first batch, and the last batch is
+// fetched whole even when it holds fewer records (the loads past k1 read the next column's
+// records or the table's kRecordPad zero records, and their values are not used), so a column
+// of n records waits ceil(n / BATCH) times. At a few waves per SIMD each wait is exposed, and
+// the short columns of low degrees would otherwise wait once per record of their last batch.
 template <typename T, typename V>
 __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, V er, V rr2,
     V& P, V& Pp, V& A1, V& A2, V& B1, V& B2, V& E1, V& E2, bool first_is_diag)
 {
     constexpr int R = kRecordLen;
+    constexpr int BATCH = OGP_RECORD_BATCH;
     typedef const __attribute__((address_space(4))) T* tptr;
     tptr q = (tptr)table + R * (int64_t)k0;
-    int k = k0;
-    if (first_is_diag) {
-        A1 = fmat(P, q[1], A1);
-        A2 = fmat(P, q[2], A2);
-        B1 = fmat(P, q[3], B1);
-        B2 = fmat(P, q[4], B2);
-        E1 = fmat(P, q[5], E1);
-        E2 = fmat(P, q[6], E2);
-        q += R;
-        ++k;
-    }
-    auto step = [&](const T* r) {
-        const V Pn = fmat(r[0] * er, P, -(rr2 * Pp));
-        Pp = P;
-        P = Pn;
+    auto accumulate = [&](const T* r) {
         A1 = fmat(P, r[1], A1);
         A2 = fmat(P, r[2], A2);
         B1 = fmat(P, r[3], B1);
@@ -86,26 +83,52 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
         E1 = fmat(P, r[5], E1);
         E2 = fmat(P, r[6], E2);
     };
-    // Records are fetched BATCH at a time into scalar registers and then consumed: one wait for
-    // the scalar loads per batch instead of one per record.
-    constexpr int BATCH = OGP_RECORD_BATCH;
-    for (; k + BATCH <= k1; k += BATCH, q += R * BATCH) {
-        T rec[BATCH][R];
+    auto step = [&](const T* r) {
+        const V Pn = fmat(r[0] * er, P, -(rr2 * Pp));
+        Pp = P;
+        P = Pn;
+        accumulate(r);
+    };
+    auto fetch = [&](T (*rec)[R]) {
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
 #pragma unroll
             for (int e = 0; e < R; ++e)
                 rec[b][e] = q[R * b + e];
+    };
+    int k = k0;
+    if (first_is_diag) {
+        T rec[BATCH][R];
+        fetch(rec);
+        accumulate(rec[0]);
+        if (k1 - k0 >= BATCH) {
+#pragma unroll
+            for (int b = 1; b < BATCH; ++b)
+                step(rec[b]);
+        } else {
+#pragma unroll
+            for (int b = 1; b < BATCH - 1; ++b)
+                if (b < k1 - k0)
+                    step(rec[b]);
+        }
+        k += BATCH;
+        q += R * BATCH;
+    }
+    for (; k + BATCH <= k1; k += BATCH, q += R * BATCH) {
+        T rec[BATCH][R];
+        fetch(rec);
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
             step(rec[b]);
     }
-    for (; k < k1; ++k, q += R) {
-        T rec[R];
+    if (k < k1) {
+        T rec[BATCH][R];
+        fetch(rec);
+        step(rec[0]);
 #pragma unroll
-        for (int e = 0; e < R; ++e)
-            rec[e] = q[e];
-        step(rec);
+        for (int b = 1; b < BATCH - 1; ++b)
+            if (b < k1 - k)
+                step(rec[b]);
     }
 }
 
@@ -129,21 +152,22 @@ using GravPoint = GravPointT<double>;
 template <typename V>
 __device__ __forceinline__ GravPointT<V> grav_point(V x, V y, V z, double radius)
 {
+    // 1/r and 1/|(x, y)| by reciprocal square roots (drsqrt): this geometry is on the serial
+    // path of every evaluation, between the corrector and the harmonic sums
     GravPointT<V> p;
     const V pxy2 = fmat(y, y, x * x);
     const V r2 = fmat(z, z, pxy2);
-    const V r = dsqrt(r2);
-    p.ir = 1.0 / r;
+    p.ir = drsqrt(r2);
     p.eps = z * p.ir;
     p.rho = radius * p.ir;
     p.er = p.eps * p.rho;
     p.rr2 = p.rho * p.rho;
-    const V pxy = dsqrt(pxy2);
-    p.cphi = pxy * p.ir;
+    p.cphi = 0.0;
     p.U1 = 1.0;
     p.W1 = 0.0;
-    if (val(pxy) > 0.0) {
-        const V ip = 1.0 / pxy;
+    if (val(pxy2) > 0.0) {
+        const V ip = drsqrt(pxy2);
+        p.cphi = (pxy2 * ip) * p.ir;
         p.U1 = x * ip;
         p.W1 = y * ip;
     }
@@ -222,13 +246,17 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
         int m = mb;
         int kcol = m * (L + 1) - m * (m - 1) / 2; // first record of column m
         V G = 0.0, H = 0.0, J = 0.0, K = 0.0;
+        // the next column's diagonal ratio, loaded one column ahead so that its wait is the
+        // wait of the current column's first record batch
+        double ratio = diag[mb + 1];
         for (const int mend = parts[q + 1]; m < mend; ++m) {
-            if (m == mb)
-                ; // the part's first diagonal comes from part_start
-            else if (m == 1)
-                diagP = rho * diag[1];
-            else
-                diagP *= cphi * rho * diag[m];
+            if (m != mb) { // the part's first diagonal comes from part_start
+                if (m == 1)
+                    diagP = rho * ratio;
+                else
+                    diagP *= cphi * rho * ratio;
+                ratio = diag[m + 1];
+            }
             const int ncol = L - m + 1;
             V A1, A2, B1, B2, E1, E2;
             if constexpr (!SPLIT) {
@@ -280,18 +308,71 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
     }
 }
 
-// Adds the part sums in part order (the fold every launch shape performs).
+// The parts wave w of a W-wave group sums: [wave_part(w), wave_part(w) + kParts / W). The
+// hardware places the waves of a workgroup on the CU's 4 SIMDs in turn, so waves w and w + 4 of
+// an 8-wave group share a SIMD; they take parts 2s and 2s + 1 (s = w mod 4), the pair that a
+// 4-wave group gives its wave s, and the part boundaries balance both shapes at once. Fewer
+// waves take consecutive parts. Which wave sums a part does not change the result.
+template <int W> __device__ __forceinline__ int wave_part(int w)
+{
+    if constexpr (W == 8)
+        return (w & 3) * 2 + (w >> 2);
+    else
+        return w * (kParts / W);
+}
+
+// Adds the part sums, received in part order q = 0 .. kParts - 1, as
+// ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)): the fold every launch shape performs, three
+// dependent additions deep instead of seven.
 template <typename V> struct PartFoldT {
-    GravSumsT<V> t { 0.0, 0.0, 0.0, 0.0 };
-    __device__ __forceinline__ void operator()(int, const GravSumsT<V>& s)
+    GravSumsT<V> t { 0.0, 0.0, 0.0, 0.0 }, quad, pair;
+    static __device__ __forceinline__ void add(GravSumsT<V>& a, const GravSumsT<V>& b)
     {
-        t.G += s.G;
-        t.H += s.H;
-        t.J += s.J;
-        t.K += s.K;
+        a.G += b.G;
+        a.H += b.H;
+        a.J += b.J;
+        a.K += b.K;
+    }
+    __device__ __forceinline__ void operator()(int q, const GravSumsT<V>& s)
+    {
+        static_assert(kParts == 8, "the fold tree is written for 8 parts");
+        if (q % 2 == 0) {
+            pair = s;
+            return;
+        }
+        add(pair, s);
+        if (q % 4 == 1) {
+            quad = pair;
+            return;
+        }
+        add(quad, pair);
+        if (q == 3)
+            t = quad;
+        else
+            add(t, quad);
     }
 };
 using PartFold = PartFoldT<double>;
+
+// The kParts part sums of an LDS exchange (red[(q * 4 + c) * 64 + lane], c = G, H, J, K),
+// folded. The reads are issued four parts at a time, one LDS wait per four parts.
+__device__ __forceinline__ GravSums fold_exchange(const double* red, int lane)
+{
+    PartFold f;
+#pragma unroll
+    for (int h = 0; h < kParts; h += 4) {
+        GravSums s[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double* o = red + (h + j) * 4 * 64 + lane;
+            s[j] = GravSums { o[0], o[64], o[128], o[192] };
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            f(h + j, s[j]);
+    }
+    return f.t;
+}
 
 template <bool SPLIT>
 __device__ __forceinline__ void gravity_itrs(
@@ -329,7 +410,8 @@ __device__ __forceinline__ void gravity_itrs_jac(
 // part order, so value and Jacobian are bit-identical to gravity_itrs_jac for every W.
 constexpr int kJacExchange = kParts * 16 * 64;
 
-// Wave w's Dual3 part sums (parts [w kParts / W, (w + 1) kParts / W)) into their LDS slots.
+// Wave w's Dual3 part sums (parts [wave_part(w), wave_part(w) + kParts / W)) into their LDS
+// slots.
 template <int W>
 __device__ __forceinline__ void jac_parts_to_lds(const GravPointT<Dual3>& p, const GravityDev& g,
     double* red, int w)
@@ -342,7 +424,7 @@ __device__ __forceinline__ void jac_parts_to_lds(const GravPointT<Dual3>& p, con
         o[128] = d.d[1];
         o[192] = d.d[2];
     };
-    grav_parts<false>(p, g, w * (kParts / W), (w + 1) * (kParts / W),
+    grav_parts<false>(p, g, wave_part<W>(w), wave_part<W>(w) + kParts / W,
         [&](int q, const GravSumsT<Dual3>& s) {
             put(q, 0, s.G);
             put(q, 1, s.H);
@@ -394,10 +476,10 @@ __device__ __forceinline__ void gravity_itrs_jac_split(double x, double y, doubl
 }
 
 // Column-split evaluation for a workgroup of W waves that all hold the same 64 objects: wave w
-// sums parts [w kParts / W, (w + 1) kParts / W), every part sum goes to its LDS slot, and every
-// wave adds the kParts slots in part order, which is the fold of the one-wave evaluation, so the
-// result does not depend on W. red holds kParts * 4 * 64 doubles; the first barrier keeps the
-// writes of this evaluation behind the reads of the previous one (the waves arrive together:
+// sums parts [wave_part(w), wave_part(w) + kParts / W), every part sum goes to its LDS slot, and
+// every wave adds the kParts slots in part order, which is the fold of the one-wave evaluation,
+// so the result does not depend on W. red holds kParts * 4 * 64 doubles; the first barrier keeps
+// the writes of this evaluation behind the reads of the previous one (the waves arrive together:
 // they run the same integrator arithmetic between evaluations).
 template <bool SPLIT, int W>
 __device__ __forceinline__ void gravity_itrs_split(double x, double y, double z,
@@ -416,7 +498,7 @@ __device__ __forceinline__ void gravity_itrs_split(double x, double y, double z,
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
     const int lane = threadIdx.x % 64;
     __syncthreads();
-    grav_parts<SPLIT>(p, g, w * (kParts / W), (w + 1) * (kParts / W),
+    grav_parts<SPLIT>(p, g, wave_part<W>(w), wave_part<W>(w) + kParts / W,
         [&](int q, const GravSums& s) {
             red[(q * 4 + 0) * 64 + lane] = s.G;
             red[(q * 4 + 1) * 64 + lane] = s.H;
@@ -424,12 +506,7 @@ __device__ __forceinline__ void gravity_itrs_split(double x, double y, double z,
             red[(q * 4 + 3) * 64 + lane] = s.K;
         });
     __syncthreads();
-    PartFold f;
-#pragma unroll
-    for (int q = 0; q < kParts; ++q)
-        f(q, GravSums { red[(q * 4 + 0) * 64 + lane], red[(q * 4 + 1) * 64 + lane],
-                 red[(q * 4 + 2) * 64 + lane], red[(q * 4 + 3) * 64 + lane] });
-    grav_assemble(x, y, z, p, f.t, g.gm, ax, ay, az);
+    grav_assemble(x, y, z, p, fold_exchange(red, lane), g.gm, ax, ay, az);
 }
 
 // a_GCRS = M g(M^T r) with M the column-major ITRS->GCRS matrix of the node
@@ -503,12 +580,7 @@ __device__ __forceinline__ void gravity_gcrs_lead(const double* Mg, const double
         red[(q * 4 + 3) * 64 + lane] = s.K;
     });
     __syncthreads();
-    PartFold f;
-#pragma unroll
-    for (int q = 0; q < kParts; ++q)
-        f(q, GravSums { red[(q * 4 + 0) * 64 + lane], red[(q * 4 + 1) * 64 + lane],
-                 red[(q * 4 + 2) * 64 + lane], red[(q * 4 + 3) * 64 + lane] });
-    grav_assemble(ri[0], ri[1], ri[2], p, f.t, g.gm, gi[0], gi[1], gi[2]);
+    grav_assemble(ri[0], ri[1], ri[2], p, fold_exchange(red, lane), g.gm, gi[0], gi[1], gi[2]);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
@@ -522,7 +594,7 @@ __device__ __forceinline__ void gravity_helper(const GravityDev& g, double* red,
     const int lane = threadIdx.x % 64;
     __syncthreads();
     const GravPoint p = get_point(pos, lane);
-    grav_parts<SPLIT>(p, g, w * (kParts / W), (w + 1) * (kParts / W),
+    grav_parts<SPLIT>(p, g, wave_part<W>(w), wave_part<W>(w) + kParts / W,
         [&](int q, const GravSums& s) {
             red[(q * 4 + 0) * 64 + lane] = s.G;
             red[(q * 4 + 1) * 64 + lane] = s.H;

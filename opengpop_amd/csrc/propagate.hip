@@ -452,68 +452,87 @@ void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& 
 
 void column_parts(int L, int* parts)
 {
-    // Contiguous column ranges of about equal cost: about 9 vector operations per record
-    // (column m has L - m + 1 records) plus about 25 per column for its set-up and epilogue; a
-    // part that starts at column m > 0 first computes the recurrences' state there in closed
-    // form (device.hpp, part_start: about 20 operations per halving of m).
-    std::vector<double> C(L + 2, 0.0);
-    for (int m = 0; m <= L; ++m)
-        C[m + 1] = C[m] + 9.0 * (L - m + 1) + 25.0;
-    parts[0] = 0;
-    for (int q = 1; q < kParts; ++q) {
-        const double target = C[L + 1] * q / kParts;
-        int m = parts[q - 1];
-        while (m <= L && C[m] < target)
-            ++m;
-        parts[q] = m;
-    }
-    parts[kParts] = L + 1;
-    // At low degree a column is a large share of a part and the equal-cost boundaries leave the
-    // waves unbalanced. Every launch shape uses these parts, so they are adjusted to balance the
-    // slowest wave against the mean for 2, 4 and 8 waves at once: boundaries move by one column
-    // while the worst ratio drops. At high degree the equal-cost boundaries measured better
-    // (degree 70, 4 waves: 1.4% slower adjusted).
-    if (L > 30)
-        return;
-    auto start_cost = [](int m) {
-        int bits = 0;
-        for (int e = m - 1; e > 0; e >>= 1)
-            ++bits;
-        return m > 0 ? 10.0 + 12.0 * bits : 0.0;
+    static_assert(kParts == 8, "the pairs below assume 8 parts");
+    // Cost model, in tenths of a record, fitted to the measured time of each wave's parts on
+    // gfx950 (degrees 20 and 70): a record 10 (its nine FP64 instructions), the set-up and
+    // epilogue of a column 55, and the closed-form start of a part at column m > 0 10 plus 23
+    // per halving of m. One wave issues at most one FP64 instruction per ~7.5 cycles and a SIMD
+    // one per ~4.5, in favour of the older wave: while two waves share a SIMD the older one runs
+    // rho times as fast as the younger, rho = R / 10 measured 1.5 at degree 20 (short columns)
+    // and 1.2 at degree 70, and taken linear in between.
+    //
+    // Wave s of a 4-wave group sums parts 2s and 2s + 1, alone on its SIMD; in an 8-wave group
+    // the same two parts go to the older and the younger wave of SIMD s (device.hpp,
+    // wave_part), which finish together when the older one holds rho / (1 + rho) of the pair's
+    // work. So the columns are cut into 4 contiguous chunks (the pairs; every cut is tried) and
+    // each chunk into its two parts at the split that minimises the 8-wave time of the pair, and
+    // the chunk cut minimises a mix of the 8-, 4- and 2-wave times relative to their ideals. All
+    // arithmetic is on integers, so the parts of a degree are the same on every host.
+    const int64_t R = L <= 20 ? 15 : L >= 70 ? 12 : 15 - (3 * (L - 20) + 25) / 50;
+    const int N = L + 1;
+    std::vector<int64_t> pre(N + 1, 0);
+    for (int m = 0; m < N; ++m)
+        pre[m + 1] = pre[m] + 10 * (L - m + 1) + 55;
+    auto part_cost = [&](int a, int b) -> int64_t {
+        if (b <= a)
+            return 0;
+        int halvings = 0;
+        for (int e = a - 1; e > 0; e >>= 1)
+            ++halvings;
+        return pre[b] - pre[a] + (a > 0 ? 10 + 23 * halvings : 0);
     };
-    auto worst = [&](const int* p) {
-        double r = 0.0;
-        for (int W = 2; W <= kParts; W *= 2) {
-            double mx = 0.0, sum = 0.0;
-            for (int w = 0; w < W; ++w) {
-                const int a = p[w * kParts / W], b = p[(w + 1) * kParts / W];
-                const double work = C[b] - C[a] + start_cost(a);
-                mx = std::max(mx, work);
-                sum += work;
+    // best split of chunk [x, y): (R x 8-wave time, chunk cost, split), lexicographic minimum
+    struct Inner {
+        int64_t t8, cost;
+        int split;
+    };
+    std::vector<Inner> inner((size_t)(N + 1) * (N + 1));
+    for (int x = 0; x <= N; ++x)
+        for (int y = x; y <= N; ++y) {
+            Inner best { 0, 0, -1 };
+            for (int sp = x; sp <= y; ++sp) {
+                const int64_t a = part_cost(x, sp), b = part_cost(sp, y);
+                const Inner t { R * a + std::max<int64_t>(0, R * b - 10 * a), a + b, sp };
+                if (best.split < 0 || t.t8 < best.t8 || (t.t8 == best.t8 && t.cost < best.cost))
+                    best = t;
             }
-            r = std::max(r, mx * W / sum);
+            inner[(size_t)x * (N + 1) + y] = best;
         }
-        return r;
-    };
-    double best = worst(parts);
-    for (bool moved = true; moved;) {
-        moved = false;
-        for (int q = 1; q < kParts; ++q)
-            for (int d = -1; d <= 1; d += 2) {
-                const int m = parts[q] + d;
-                if (m < parts[q - 1] || m > parts[q + 1])
-                    continue;
-                const int keep = parts[q];
-                parts[q] = m;
-                const double r = worst(parts);
-                if (r < best - 1e-12) {
-                    best = r;
-                    moved = true;
-                } else {
-                    parts[q] = keep;
+    // objective f / S (S the total cost): f = 10 (a8 + a4) + 3 a2 + 20 (max(a8, a4, a2) - S R^2)
+    // with a8, a4, a2 the slowest SIMD (8 waves), wave (4 waves) and wave (2 waves) scaled so
+    // that S R^2 is a perfect balance
+    int64_t best_f = -1, best_S = 1;
+    int bounds[3] = { 0, 0, 0 };
+    for (int b2 = 0; b2 <= N; ++b2)
+        for (int b4 = b2; b4 <= N; ++b4)
+            for (int b6 = b4; b6 <= N; ++b6) {
+                const int cut[5] = { 0, b2, b4, b6, N };
+                int64_t sums[4], S = 0, M8 = 0, M4 = 0;
+                for (int c = 0; c < 4; ++c) {
+                    const Inner& t = inner[(size_t)cut[c] * (N + 1) + cut[c + 1]];
+                    sums[c] = t.cost;
+                    S += t.cost;
+                    M8 = std::max(M8, t.t8);
+                    M4 = std::max(M4, t.cost);
+                }
+                const int64_t M2 = std::max(sums[0] + sums[1], sums[2] + sums[3]);
+                const int64_t a8 = 4 * (R + 10) * M8, a4 = 4 * M4 * R * R, a2 = 2 * M2 * R * R;
+                const int64_t f = 10 * (a8 + a4) + 3 * a2
+                    + 20 * (std::max(a8, std::max(a4, a2)) - S * R * R);
+                if (S > 0 && (best_f < 0 || f * best_S < best_f * S)) {
+                    best_f = f;
+                    best_S = S;
+                    bounds[0] = b2;
+                    bounds[1] = b4;
+                    bounds[2] = b6;
                 }
             }
+    const int cut[5] = { 0, bounds[0], bounds[1], bounds[2], N };
+    for (int c = 0; c < 4; ++c) {
+        parts[2 * c] = cut[c];
+        parts[2 * c + 1] = inner[(size_t)cut[c] * (N + 1) + cut[c + 1]].split;
     }
+    parts[kParts] = N;
 }
 
 } // namespace ogp
