@@ -125,8 +125,8 @@ GravityModel load_gravity(const std::string& path, int degree)
                     zs = z * g.S[tri(n, m - 1)];
                 }
             }
-            const double rec[kRecordLen] = { xi, cn * c, cn * s, cn * kc, cn * ks, cn * zc,
-                cn * zs };
+            const double rec[kRecordLen] = { xi, m * (cn * c), m * (cn * s), cn * kc, cn * ks,
+                cn * zc, cn * zs };
             g.records.insert(g.records.end(), rec, rec + kRecordLen);
         }
     }

@@ -33,7 +33,7 @@ struct FrameData {
 // less per degree; c_n stays within [0.46, 1.13] up to degree 70). The c_n go into the
 // coefficients. Seven values per record:
 //   xi'     : xi'_n (unused on the diagonal n = m)
-//   c, s    : c_n Cbar_nm, c_n Sbar_nm                       (J and K sums)
+//   c, s    : m c_n Cbar_nm, m c_n Sbar_nm                   (J and K sums, which carry m)
 //   kc, ks  : c_n (1 + n + m) Cbar_nm, c_n (1 + n + m) Sbar_nm (Gamma sum)
 //   zc, zs  : c_n zeta_{n,m-1} Cbar_{n,m-1}, c_n zeta_{n,m-1} Sbar_{n,m-1} (H sum carried by
 //             column m)

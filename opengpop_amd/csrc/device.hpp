@@ -67,7 +67,9 @@ constexpr int kRecordPad = OGP_RECORD_BATCH - 1;
 // records or the table's kRecordPad zero records, and their values are not used), so a column
 // of n records waits ceil(n / BATCH) times. At a few waves per SIMD each wait is exposed, and
 // the short columns of low degrees would otherwise wait once per record of their last batch.
-template <typename T, typename V>
+// ZONAL: column m = 0, whose records feed the Gamma sum B1 only (its J, K and H terms carry the
+// factor m, and it has no sine terms): one FMA per record instead of six.
+template <typename T, typename V, bool ZONAL = false>
 __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, V er, V rr2,
     V& P, V& Pp, V& A1, V& A2, V& B1, V& B2, V& E1, V& E2, bool first_is_diag)
 {
@@ -76,12 +78,14 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
     typedef const __attribute__((address_space(4))) T* tptr;
     tptr q = (tptr)table + R * (int64_t)k0;
     auto accumulate = [&](const T* r) {
-        A1 = fmat(P, r[1], A1);
-        A2 = fmat(P, r[2], A2);
         B1 = fmat(P, r[3], B1);
-        B2 = fmat(P, r[4], B2);
-        E1 = fmat(P, r[5], E1);
-        E2 = fmat(P, r[6], E2);
+        if constexpr (!ZONAL) {
+            A1 = fmat(P, r[1], A1);
+            A2 = fmat(P, r[2], A2);
+            B2 = fmat(P, r[4], B2);
+            E1 = fmat(P, r[5], E1);
+            E2 = fmat(P, r[6], E2);
+        }
     };
     auto step = [&](const T* r) {
         const V Pn = fmat(r[0] * er, P, -(rr2 * Pp));
@@ -245,7 +249,8 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
         part_start(p, mb, part_diag[q], Um, Wm, Um1, Wm1, diagP);
         int m = mb;
         int kcol = m * (L + 1) - m * (m - 1) / 2; // first record of column m
-        V G = 0.0, H = 0.0, J = 0.0, K = 0.0;
+        // Gc: the Gamma sum of the columns m >= 1 before its factor cos(phi), applied once
+        V G = 0.0, Gc = 0.0, H = 0.0, J = 0.0, K = 0.0;
         // the next column's diagonal ratio, loaded one column ahead so that its wait is the
         // wait of the current column's first record batch
         double ratio = diag[mb + 1];
@@ -262,8 +267,12 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
             if constexpr (!SPLIT) {
                 V P = diagP, Pp = 0.0;
                 A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
-                column_terms<double>(g.rec, kcol, kcol + ncol, er, rr2, P, Pp, A1, A2, B1, B2, E1,
-                    E2, true);
+                if (m == 0)
+                    column_terms<double, V, true>(g.rec, kcol, kcol + ncol, er, rr2, P, Pp, A1, A2,
+                        B1, B2, E1, E2, true);
+                else
+                    column_terms<double>(g.rec, kcol, kcol + ncol, er, rr2, P, Pp, A1, A2, B1, B2,
+                        E1, E2, true);
             } else {
                 double P = diagP, Pp = 0.0;
                 A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
@@ -277,8 +286,12 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
                 if (k32 < kcol + ncol) {
                     float Pf = (float)P, Ppf = (float)Pp;
                     float a1 = 0.f, a2 = 0.f, b1 = 0.f, b2 = 0.f, e1 = 0.f, e2 = 0.f;
-                    column_terms<float>(g.recf, k32, kcol + ncol, erf_, rr2f, Pf, Ppf, a1, a2, b1,
-                        b2, e1, e2, k32 == kcol);
+                    if (m == 0)
+                        column_terms<float, float, true>(g.recf, k32, kcol + ncol, erf_, rr2f, Pf,
+                            Ppf, a1, a2, b1, b2, e1, e2, k32 == kcol);
+                    else
+                        column_terms<float>(g.recf, k32, kcol + ncol, erf_, rr2f, Pf, Ppf, a1, a2,
+                            b1, b2, e1, e2, k32 == kcol);
                     A1 += (double)a1;
                     A2 += (double)a2;
                     B1 += (double)b1;
@@ -288,13 +301,13 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
                 }
             }
             if (m == 0) {
-                G += B1;
+                G = B1;
             } else {
-                const double dm = (double)m;
-                G += cphi * fmat(Um, B1, Wm * B2);
-                J += dm * fmat(Um1, A1, Wm1 * A2);
-                K -= dm * fmat(Wm1, A1, -Um1 * A2);
-                H += fmat(Um1, E1, Wm1 * E2);
+                // A1 and A2 carry the factor m (GravityModel::records)
+                Gc = fmat(Um, B1, fmat(Wm, B2, Gc));
+                J = fmat(Um1, A1, fmat(Wm1, A2, J));
+                K = fmat(Um1, A2, fmat(-Wm1, A1, K));
+                H = fmat(Um1, E1, fmat(Wm1, E2, H));
             }
             // advance the longitude terms: (U + iW)_{m+1} = (U + iW)_m (U1 + i W1)
             Um1 = Um;
@@ -304,7 +317,7 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
             Um = un;
             kcol += ncol;
         }
-        sink(q, GravSumsT<V> { G, H, J, K });
+        sink(q, GravSumsT<V> { fmat(cphi, Gc, G), H, J, K });
     }
 }
 

@@ -454,9 +454,9 @@ void column_parts(int L, int* parts)
 {
     static_assert(kParts == 8, "the pairs below assume 8 parts");
     // Cost model, in tenths of a record, fitted to the measured time of each wave's parts on
-    // gfx950 (degrees 20 and 70): a record 10 (its nine FP64 instructions), the set-up and
-    // epilogue of a column 55, and the closed-form start of a part at column m > 0 10 plus 23
-    // per halving of m. One wave issues at most one FP64 instruction per ~7.5 cycles and a SIMD
+    // gfx950 (degrees 20 and 70): a record 10 (its nine FP64 instructions; 5 in column 0, which
+    // feeds one sum), the set-up and epilogue of a column 55, and the closed-form start of a
+    // part at column m > 0 10 plus 23 per halving of m. One wave issues at most one FP64 instruction per ~7.5 cycles and a SIMD
     // one per ~4.5, in favour of the older wave: while two waves share a SIMD the older one runs
     // rho times as fast as the younger, rho = R / 10 measured 1.5 at degree 20 (short columns)
     // and 1.2 at degree 70, and taken linear in between.
@@ -472,7 +472,7 @@ void column_parts(int L, int* parts)
     const int N = L + 1;
     std::vector<int64_t> pre(N + 1, 0);
     for (int m = 0; m < N; ++m)
-        pre[m + 1] = pre[m] + 10 * (L - m + 1) + 55;
+        pre[m + 1] = pre[m] + (m == 0 ? 5 : 10) * (L - m + 1) + 55;
     auto part_cost = [&](int a, int b) -> int64_t {
         if (b <= a)
             return 0;
