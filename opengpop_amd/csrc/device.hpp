@@ -208,19 +208,23 @@ __device__ __forceinline__ void part_start(const GravPointT<V>& p, int m0, doubl
     }
     V pu = 1.0, pw = 0.0, cr = 1.0;                   // z^(m0-1), (cphi rho)^(m0-1)
     V bu = p.U1, bw = p.W1, cb = p.cphi * p.rho;      // z^(2^i), (cphi rho)^(2^i)
-    for (int e = m0 - 1; e > 0; e >>= 1) {           // e is wave-uniform
+    for (int e = m0 - 1; e > 0;) {                   // e is wave-uniform
         if (e & 1) {
+            // a branch, not both products and selects: e is the same in every lane
+            __asm__ volatile("");
             const V nu = fmat(pu, bu, -pw * bw);
             pw = fmat(pw, bu, pu * bw);
             pu = nu;
             cr = cr * cb;
         }
-        if (e > 1) {
-            const V nu = fmat(bu, bu, -bw * bw);
-            bw = fmat(bw, bu, bu * bw);
-            bu = nu;
-            cb = cb * cb;
-        }
+        e >>= 1;
+        if (e == 0)
+            break;
+        // double angle on the unit circle: cos 2x = 2 cos^2 x - 1, sin 2x = 2 sin x cos x
+        const V twob = bu + bu;
+        bw = twob * bw;
+        bu = fmat(twob, bu, -1.0);
+        cb = cb * cb;
     }
     Um1 = pu;
     Wm1 = pw;

@@ -453,13 +453,15 @@ void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& 
 void column_parts(int L, int* parts)
 {
     static_assert(kParts == 8, "the pairs below assume 8 parts");
-    // Cost model, in tenths of a record, fitted to the measured time of each wave's parts on
-    // gfx950 (degrees 20 and 70): a record 10 (its nine FP64 instructions; 5 in column 0, which
-    // feeds one sum), the set-up and epilogue of a column 55, and the closed-form start of a
-    // part at column m > 0 10 plus 23 per halving of m. One wave issues at most one FP64 instruction per ~7.5 cycles and a SIMD
-    // one per ~4.5, in favour of the older wave: while two waves share a SIMD the older one runs
-    // rho times as fast as the younger, rho = R / 10 measured 1.5 at degree 20 (short columns)
-    // and 1.2 at degree 70, and taken linear in between.
+    // Cost model, in tenths of a full record (nine FP64 instructions), fitted to the time of
+    // each part measured on gfx950, alone on a CU (degrees 10 to 30) and in 4- and 8-wave groups
+    // of the 20,000-object propagations at degrees 20 and 70: a column's set-up and epilogue 68,
+    // a part's closed-form start at column m > 0 38 plus 19 per halving of m, and a record of
+    // the zonal column 10 at degree 20 rising to 14 at degree 70 (it issues fewer instructions
+    // than a full record but waits as long for each batch of records). One wave issues at most
+    // one FP64 instruction per ~7.5 cycles and a SIMD one per ~4.5, in favour of the older
+    // wave: while two waves share a SIMD the older one runs rho times as fast as the younger,
+    // rho = R / 10 measured about 1.3 at degree 20 and 1.15 at degree 70.
     //
     // Wave s of a 4-wave group sums parts 2s and 2s + 1, alone on its SIMD; in an 8-wave group
     // the same two parts go to the older and the younger wave of SIMD s (device.hpp,
@@ -468,18 +470,19 @@ void column_parts(int L, int* parts)
     // each chunk into its two parts at the split that minimises the 8-wave time of the pair, and
     // the chunk cut minimises a mix of the 8-, 4- and 2-wave times relative to their ideals. All
     // arithmetic is on integers, so the parts of a degree are the same on every host.
-    const int64_t R = L <= 20 ? 15 : L >= 70 ? 12 : 15 - (3 * (L - 20) + 25) / 50;
+    const int64_t R = L <= 20 ? 13 : L >= 70 ? 12 : 13 - (L - 20 + 25) / 50;
+    const int64_t Z = L <= 20 ? 10 : L >= 70 ? 14 : 10 + (4 * (L - 20) + 25) / 50;
     const int N = L + 1;
     std::vector<int64_t> pre(N + 1, 0);
     for (int m = 0; m < N; ++m)
-        pre[m + 1] = pre[m] + (m == 0 ? 5 : 10) * (L - m + 1) + 55;
+        pre[m + 1] = pre[m] + (m == 0 ? Z : 10) * (L - m + 1) + 68;
     auto part_cost = [&](int a, int b) -> int64_t {
         if (b <= a)
             return 0;
         int halvings = 0;
         for (int e = a - 1; e > 0; e >>= 1)
             ++halvings;
-        return pre[b] - pre[a] + (a > 0 ? 10 + 23 * halvings : 0);
+        return pre[b] - pre[a] + (a > 0 ? 38 + 19 * halvings : 0);
     };
     // best split of chunk [x, y): (R x 8-wave time, chunk cost, split), lexicographic minimum
     struct Inner {
