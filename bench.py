@@ -236,8 +236,9 @@ def leg_split(ogp, scenarios, args):
                                  hw_issue_frac_nonpacked=nonpacked,
                                  ceiling=("non-packed FP32 issues at the FP64 rate and the split "
                                           "path issues as many FLOPs per object-step as the "
-                                          "FP64 path, so its ceiling is the FP64 path's time "
-                                          "without waits: about 1/0.76 = 1.3-1.4x the FP64 "
+                                          "FP64 path (9% more instructions), so its ceiling is "
+                                          "its own issue time without waits: with the FP64 "
+                                          "path's VALU about 68% busy, 1.35-1.4x the FP64 "
                                           "path, not 3x"))
     # 32 objects for the cpu leg's check against the oracle
     k = np.arange(0, n, n // 32)[:32]
