@@ -102,7 +102,7 @@ def parse():
     p.add_argument("--legs",
                    default="fp64,split,pairs,layouts,adjoint,cpu,refpoint,sources,perturb")
     p.add_argument("--cpu-threads", type=int, default=0,
-                   help="0 = min(16, CPUs in the affinity mask)")
+                   help="0 = every CPU in the affinity mask, at most 16")
     return p.parse_args()
 
 
@@ -358,8 +358,8 @@ def leg_cpu(sample, threads):
                                         f"{int(work1)} object-steps, 1 thread, {dt1:.1f} s"),
                 cpu_model=_cpu_model(), nproc=os.cpu_count(),
                 affinity_cpus=len(os.sched_getaffinity(0)),
-                threads_rule="min(16, CPUs in this process's affinity mask): the box's CPU share "
-                             "for one GPU is 16"), \
+                threads_rule="every CPU in this process's affinity mask, at most 16 (the CPU "
+                             "share of one GPU on the GPU box, whose mask holds 16)"), \
         dict(objects=len(idx), max_dr_m=float(dr.max()), max_dv_mps=float(dv.max()))
 
 
