@@ -266,19 +266,31 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         gj8_sums(ps.hist, n, i, k, q);
         double sprev[3], anew[3];
         gj8_predict(q, h, s, S, r, v, sprev);
+        // the evaluation points and the one correction of gj8_chunk_kernel without sources, bit
+        // for bit: the predictor's point rotated into the ITRS, then hh (M^T (T2 + S) + g88 g_ITRS)
+        // after each evaluation (contexts with sources have no derivatives)
+        const uniform_dptr M = (uniform_dptr)(f.prop_M + 9 * (k + 4));
+        double ri[3], base[3], t[3];
+        to_itrs(M, r, ri);
+        for (int c = 0; c < 3; ++c)
+            t[c] = q.T2[c] + S[c];
+        to_itrs(M, t, base);
         for (int it = 0; it < ncorr; ++it) {
-            double G[3][3];
+            double gi[3], Gi[3][3], G[3][3];
             if constexpr (W > 1)
-                gravity_gcrs_jac_lead<W>(f.prop_M + 9 * (k + 4), r, f.grav, red, pos, anew, G);
+                gravity_itrs_jac_lead<W>(ri, f.grav, red, pos, gi, Gi);
             else
-                gravity_gcrs_jac<W>(f.prop_M + 9 * (k + 4), r, f.grav, anew, G, red);
+                gravity_itrs_jac(ri[0], ri[1], ri[2], f.grav, gi, Gi);
+            jac_to_gcrs(M, gi, Gi, anew, G);
             if (act) {
                 const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
                 for (int e = 0; e < 6; ++e)
                     Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
             }
-            gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+            for (int c = 0; c < 3; ++c)
+                ri[c] = hh * __builtin_fma(c_gj[8][8], gi[c], base[c]);
         }
+        gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         if (!act)
             continue;
         const int wslot = ring_slot(k);

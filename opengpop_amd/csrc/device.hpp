@@ -576,17 +576,29 @@ __device__ __forceinline__ GravPoint get_point(const double* pos, int lane)
     return p;
 }
 
-template <bool SPLIT, int W>
-__device__ __forceinline__ void gravity_gcrs_lead(const double* Mg, const double* r,
-    const GravityDev& g, double* red, double* pos, double* a)
+// r_ITRS = M^T r and a_GCRS = M g for the column-major ITRS->GCRS matrix M (wave-uniform), in
+// the operation order of gravity_gcrs.
+__device__ __forceinline__ void to_itrs(const uniform_dptr M, const double* r, double* ri)
 {
-    static_assert(W > 1 && kParts % W == 0, "waves must divide the part count");
-    const uniform_dptr M = (uniform_dptr)Mg;
-    const int lane = threadIdx.x % 64;
-    double ri[3], gi[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
+}
+__device__ __forceinline__ void to_gcrs(const uniform_dptr M, const double* gi, double* a)
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
+}
+
+// The lead wave's part of a lead/helper evaluation at the ITRS point ri: the geometry to pos,
+// the first barrier, its own parts, the second barrier, the fold and assembly into gi (ITRS).
+template <bool SPLIT, int W>
+__device__ __forceinline__ void gravity_itrs_lead(const double* ri, const GravityDev& g,
+    double* red, double* pos, double* gi)
+{
+    static_assert(W > 1 && kParts % W == 0, "waves must divide the part count");
+    const int lane = threadIdx.x % 64;
     const GravPoint p = grav_point(ri[0], ri[1], ri[2], g.radius);
     put_point(pos, lane, p);
     __syncthreads();
@@ -598,9 +610,17 @@ __device__ __forceinline__ void gravity_gcrs_lead(const double* Mg, const double
     });
     __syncthreads();
     grav_assemble(ri[0], ri[1], ri[2], p, fold_exchange(red, lane), g.gm, gi[0], gi[1], gi[2]);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        a[i] = __builtin_fma(M[6 + i], gi[2], __builtin_fma(M[3 + i], gi[1], M[i] * gi[0]));
+}
+
+template <bool SPLIT, int W>
+__device__ __forceinline__ void gravity_gcrs_lead(const double* Mg, const double* r,
+    const GravityDev& g, double* red, double* pos, double* a)
+{
+    const uniform_dptr M = (uniform_dptr)Mg;
+    double ri[3], gi[3];
+    to_itrs(M, r, ri);
+    gravity_itrs_lead<SPLIT, W>(ri, g, red, pos, gi);
+    to_gcrs(M, gi, a);
 }
 
 // The part of one gravity_gcrs_lead evaluation that wave w (1 <= w < W) runs.
@@ -661,25 +681,34 @@ __device__ __forceinline__ void gravity_gcrs_jac(const double* Mg, const double*
 // gravity_gcrs_lead: wave 0 leaves the ITRS position in pos before the first barrier, every wave
 // sums its Dual3 parts there, and wave 0 alone folds, assembles and rotates. Bit-identical to
 // gravity_gcrs_jac<W>; the barrier hazards are those of gravity_gcrs_lead.
+//
+// gravity_itrs_jac_lead is the lead wave's part at the ITRS point ri (the position to pos, the
+// Dual3 parts, the fold and assembly): gi and its Jacobian Gi in the ITRS.
 template <int W>
-__device__ __forceinline__ void gravity_gcrs_jac_lead(const double* Mg, const double* r,
-    const GravityDev& g, double* red, double* pos, double* a, double (*G)[3])
+__device__ __forceinline__ void gravity_itrs_jac_lead(const double* ri, const GravityDev& g,
+    double* red, double* pos, double* gi, double (*Gi)[3])
 {
     static_assert(W > 1 && kParts % W == 0, "waves must divide the part count");
-    const uniform_dptr M = (uniform_dptr)Mg;
     const int lane = threadIdx.x % 64;
-    double ri[3], gi[3], Gi[3][3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
+    for (int i = 0; i < 3; ++i)
         pos[i * 64 + lane] = ri[i];
-    }
     const Dual3 X = dvar(ri[0], 0), Y = dvar(ri[1], 1), Z = dvar(ri[2], 2);
     const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
     __syncthreads();
     jac_parts_to_lds<W>(p, g, red, 0);
     __syncthreads();
     jac_fold_assemble(X, Y, Z, p, g, red, gi, Gi);
+}
+
+template <int W>
+__device__ __forceinline__ void gravity_gcrs_jac_lead(const double* Mg, const double* r,
+    const GravityDev& g, double* red, double* pos, double* a, double (*G)[3])
+{
+    const uniform_dptr M = (uniform_dptr)Mg;
+    double ri[3], gi[3], Gi[3][3];
+    to_itrs(M, r, ri);
+    gravity_itrs_jac_lead<W>(ri, g, red, pos, gi, Gi);
     jac_to_gcrs(M, gi, Gi, a, G);
 }
 

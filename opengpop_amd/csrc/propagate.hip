@@ -345,9 +345,54 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
         double sprev[3], anew[3];
         gj8_predict(q, h, s, S, r, v, sprev);
         // P E (E C)^ncorr: the predictor's evaluation is the first corrector evaluation.
-        for (int it = 0; it < ncorr; ++it) {
-            eval_prop<KIND, W>(f, r, v, k + 4, anew, ex);
-            gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+        if constexpr (KIND == kGravity || KIND == kGravitySplit) {
+            // Every evaluation of the step is at node k + 4, one matrix M. Without sources the
+            // corrected position rotated into the ITRS is M^T r = hh (M^T (T2 + S) + g88 g_ITRS),
+            // so each evaluation point follows from the previous evaluation's ITRS gravity in two
+            // operations; and since a correction depends only on the latest acceleration (s, v
+            // and r from sprev, S and the sums), the step corrects once, after its last
+            // evaluation. The corrector and both rotations leave the serial path of the
+            // evaluations (gj8_record_kernel runs the same arithmetic, bit for bit). With
+            // sources, which need each corrected GCRS position and velocity, every evaluation
+            // is corrected and rotated.
+            const uniform_dptr M = (uniform_dptr)(f.prop_M + 9 * (k + 4));
+            auto gravity = [&](const double* ri, double* gi) {
+                if constexpr (W > 1)
+                    gravity_itrs_lead<KIND == kGravitySplit, W>(ri, f.grav, ex.red, ex.pos, gi);
+                else
+                    gravity_itrs<KIND == kGravitySplit>(ri[0], ri[1], ri[2], f.grav, gi[0], gi[1],
+                        gi[2]);
+            };
+            double ri[3], gi[3];
+            to_itrs(M, r, ri);
+            if (f.src.mask == 0) {
+                double base[3], t[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    t[c] = q.T2[c] + S[c];
+                to_itrs(M, t, base);
+                for (int it = 0; it < ncorr; ++it) {
+                    gravity(ri, gi);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        ri[c] = hh * __builtin_fma(c_gj[8][8], gi[c], base[c]);
+                }
+                to_gcrs(M, gi, anew);
+                gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+            } else {
+                for (int it = 0; it < ncorr; ++it) {
+                    gravity(ri, gi);
+                    to_gcrs(M, gi, anew);
+                    add_sources(f.src, f.prop_tb + 6 * (k + 4), r, v, anew);
+                    gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+                    to_itrs(M, r, ri);
+                }
+            }
+        } else {
+            for (int it = 0; it < ncorr; ++it) {
+                eval_prop<KIND, W>(f, r, v, k + 4, anew, ex);
+                gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+            }
         }
         if (k > ns)
             continue;
