@@ -460,7 +460,34 @@ struct ogp_context {
         for (const Group& g : groups)
             tabs.push_back(node_table(g.h, g.max_steps, t0));
         HIPC(hipEventRecord(e1.e, stream));
+        // A class with less than a third of the object-steps of the largest class starts late:
+        // its startup and chunks wait until the largest class's remaining object-steps fall to
+        // 3 times its own. It then fills the chip while the largest class runs down its longest
+        // spans (few objects, launches that leave CUs idle) instead of competing with it while
+        // it fills the chip (mixed 100k workload: 1,093 against 1,107 ms; starting at half or
+        // 0.3 of the largest class's objects still active measured 1,097 and 1,096 ms, at 0.2
+        // 1,112 ms). Every class integrates the same arithmetic whenever it starts.
+        std::vector<double> work(groups.size(), 0.0);
+        std::vector<char> started(groups.size(), 1);
+        size_t qbig = 0;
+        std::vector<double> big_prefix; // object-steps of the largest class's first j objects
+        if (ns && groups.size() > 1) {
+            for (size_t q = 0; q < groups.size(); ++q) {
+                for (int64_t j = 0; j < groups[q].count; ++j)
+                    work[q] += (double)ns[groups[q].first + j];
+                if (work[q] > work[qbig])
+                    qbig = q;
+            }
+            // (a class with no GJ8 step, only its startup, is never deferred)
+            for (size_t q = 0; q < groups.size(); ++q)
+                started[q] = q == qbig || 3.0 * work[q] >= work[qbig] || groups[q].max_steps < 5;
+            big_prefix.assign(groups[qbig].count + 1, 0.0);
+            for (int64_t j = 0; j < groups[qbig].count; ++j)
+                big_prefix[j + 1] = big_prefix[j] + (double)ns[groups[qbig].first + j];
+        }
         for (size_t q = 0; q < groups.size(); ++q) {
+            if (!started[q])
+                continue;
             const Group& g = groups[q];
             const int W = waves_for(g.count);
             launch_startup(stream, kind, force_ctx(tabs[q], g.h), rk8_variant, ps, scratch(),
@@ -551,6 +578,32 @@ struct ogp_context {
                 if (k0[q] > g.max_steps)
                     continue;
                 more = true;
+                if (!started[q]) {
+                    // the largest class's object-steps from k0 on: its active objects (a prefix,
+                    // spans descending) each have ns - k0 + 1 steps left
+                    const Group& gb = groups[qbig];
+                    const int64_t a = k0[qbig] <= gb.max_steps
+                        ? active_in(gb.first, gb.count, k0[qbig]) : 0;
+                    if (big_prefix[a] - (double)a * (double)(k0[qbig] - 1) > 3.0 * work[q])
+                        continue;
+                    // after the largest class's chunks issued so far: the startup on the first
+                    // stream of the class, then its chunks on all of them
+                    std::vector<hipStream_t> mine, big;
+                    for (const Lane& l : lanes) {
+                        if (l.q == q)
+                            mine.push_back(l.st);
+                        else if (l.q == qbig)
+                            big.push_back(l.st);
+                    }
+                    for (hipStream_t st : mine)
+                        join(st, big);
+                    launch_startup(mine[0], kind, fctx[q], rk8_variant, ps, scratch(), g.first,
+                        g.count, cfg.n_corrections, traj, traj_steps, waves_for(g.count));
+                    check_launch();
+                    for (hipStream_t st : mine)
+                        join(st, { mine[0] });
+                    started[q] = 1;
+                }
                 const int64_t active = active_in(g.first, g.count, k0[q]);
                 // one chunk of every slice of the class with the given launch shape; returns
                 // the steps done and appends (launch index, object-steps) of each launch to `out`

@@ -338,6 +338,24 @@ def test_concurrent_slices_do_not_change_results():
             c.set_slices(17)
 
 
+def test_late_started_class_matches_separate_runs():
+    # 1,600 LEO at 30 s for 300 steps, 400 MEO/GEO at 300 s for 72-288 steps (under a third of
+    # the LEO object-steps, so that class starts late, during the LEO class's last chunks) and
+    # 40 objects at 120 s with only startup steps (never started late): every class gives the
+    # bits of its own propagation
+    d = scenarios.mixed(2000, seed=17)
+    r0 = np.concatenate([d["r"], d["r"][:40]])
+    v0 = np.concatenate([d["v"], d["v"][:40]])
+    step = np.concatenate([d["step"], np.full(40, 120.0)])
+    steps = np.concatenate([np.minimum(d["steps"], 300), np.arange(40) % 5])
+    with ogp.Context(degree=20) as c:
+        r, v = c.propagate(r0, v0, step, steps)
+        for h in (30.0, 300.0, 120.0):
+            m = step == h
+            rs, vs = c.propagate(r0[m], v0[m], step[m], steps[m])
+            assert np.array_equal(r[m], rs) and np.array_equal(v[m], vs), h
+
+
 def test_autotuner_keeps_results_and_records_choice():
     d = scenarios.config0()
     # 640 objects (10 groups): only the widest shape can give every SIMD a wave, so it is taken
