@@ -193,9 +193,13 @@ __device__ __forceinline__ Lane lane_object(int64_t obj0, int64_t nobj, int64_t 
 // acceleration matrix and ncorr corrections; then the hand-over to the GJ8 steps
 // (gpu_propagator.cu:31-53, copy_s.cu:19-30). A lane with no step (no GpuPropagator::Step call)
 // leaves its state untouched.
+// Register budget: 128 VGPRs (4 resident waves per SIMD) for gravity, whose lead wave then
+// spills part of its RK8 work area (100,000 objects at degree 70: 18.7 against 25.0 ms with
+// 256 VGPRs and 2 waves per SIMD; 20,000: 5.4 against 8.6 ms); 256 for the tangent form, whose
+// dual numbers would spill heavily in 128, and for the analytic test forces.
 template <int KIND, int W>
 __global__ void __launch_bounds__(64 * W)
-__attribute__((amdgpu_waves_per_eu(2))) // at most 256 VGPRs + AGPRs: 2 resident waves per SIMD
+__attribute__((amdgpu_waves_per_eu(KIND == kGravity || KIND == kGravitySplit ? 4 : 2)))
 startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     StartupScratch sc, int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps)
 {
