@@ -66,6 +66,14 @@ def flops_step(L, k=7):
     return k * flops_eval(L) + 228 + 33 * k
 
 
+def flops_chunks(L, gj, evals):
+    """Algorithmic FLOPs of the GJ8 chunk kernels over gj object-steps in which the kernels ran
+    `evals` object-evaluations (ogp_timing.evaluations): a step's evaluations end at the
+    corrector's bitwise fixed point, so k varies per step; each evaluation costs F_E + 33 (its
+    corrector share) and each object-step 228 (predictor and history sums)."""
+    return evals * (flops_eval(L) + 33) + 228 * gj
+
+
 def profile_value(name, key):
     """A `key: value` line of a committed rocprofv3 summary under profiles/, or None."""
     try:
@@ -147,7 +155,8 @@ def leg_fp64(ogp, scenarios, torch, args):
     # in flight (= algorithmic FLOPs of the GJ8 phase / the phase's elapsed time)
     avg_launch_ms = sum(t["launch_ms_sum"] for t in tm) / len(tm) / launches
     concurrency = avg_launch_ms * launches / steps_ms
-    flops_launch = flops_step(70) * gj / launches
+    evals = tm[-1]["evaluations"]
+    flops_launch = flops_chunks(70, gj, evals) / launches
     per_launch = flops_launch / (avg_launch_ms * 1e-3) / 1e12
     achieved = per_launch * concurrency
     dev_ms = sum(t["total_ms"] for t in tm) / len(tm)
@@ -160,7 +169,11 @@ def leg_fp64(ogp, scenarios, torch, args):
                     gj8_ms=steps_ms, object_steps_per_s=obj_steps / (dev_ms * 1e-3)),
         roofline=dict(bound="valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                       frac=achieved / PEAK_FP64_TFLOPS, traffic=traffic,
-                      kernel="gj8_chunk_kernel<kGravity>", flops_per_object_step=flops_step(70),
+                      kernel="gj8_chunk_kernel<kGravity>",
+                      flops_per_object_step=flops_chunks(70, gj, evals) / gj,
+                      evaluations_per_object_step=evals / gj,
+                      flops_basis="F_E = 64,935 per executed evaluation + 33 corrector share, "
+                                  "228 per object-step; evaluations counted on the device",
                       avg_launch_ms=avg_launch_ms, launches_per_step=launches,
                       launches_in_flight=concurrency, per_launch_tflops=per_launch,
                       achieved_basis="per-launch FLOP rate x mean launches in flight "
@@ -174,6 +187,19 @@ def leg_fp64(ogp, scenarios, torch, args):
         object_steps=obj_steps,
         tuning={str(b): ctx.get_tuning(64 * 2 ** b)[0] for b in range(0, 11)},
     )
+    # the same propagation with every step running all 7 corrector evaluations (the reference's
+    # count): the fixed-point exit must leave every final state bit-identical
+    r_fp, v_fp = r.clone(), v.clone()
+    ctx.set_fixed_point_exit(False)
+    run()
+    t7 = ctx.timing()
+    ctx.set_fixed_point_exit(True)
+    out["all_corrections"] = dict(
+        device_ms=t7["total_ms"], object_steps_per_s=obj_steps / (t7["total_ms"] * 1e-3),
+        evaluations_per_object_step=t7["evaluations"] / t7["gj_object_steps"],
+        bit_identical=bool(torch.equal(r, r_fp) and torch.equal(v, v_fp)))
+    log(f"fp64 all 7 corrections: {t7['total_ms']:.1f} ms, bit-identical "
+        f"{out['all_corrections']['bit_identical']}")
     # keep the final states of the stratified sample for the cpu leg's comparison
     idx = np.arange(0, n, CPU_SAMPLE_STRIDE)
     out["_sample"] = dict(idx=idx, r=r.cpu().numpy().T[idx].copy(), v=v.cpu().numpy().T[idx].copy(),
@@ -197,6 +223,7 @@ def leg_split(ogp, scenarios, args):
             res[prec] = dict(r=r, v=v, dev_ms=t["total_ms"], wall=wall,
                              rate=float(c2["steps"].sum()) / (t["total_ms"] * 1e-3),
                              gj8_ms=t["steps_ms"], gj=t["gj_object_steps"],
+                             evals=t["evaluations"],
                              launches=t["step_launches"], launch_ms_sum=t["launch_ms_sum"])
             log(f"split leg {prec}: {t['total_ms']:.0f} ms device, {res[prec]['rate']:.3e} obj-steps/s")
     err = np.linalg.norm(res["split"]["r"] - res["fp64"]["r"], axis=1)
@@ -208,9 +235,10 @@ def leg_split(ogp, scenarios, args):
                                         median=float(np.median(err))))
     # split-path roofline: FP32 terms n >= 3 against the FP32 peak, the rest against FP64
     L, ns = 70, 2
-    f32 = 7 * 25 * (terms(L) - terms(ns))
-    f64 = flops_step(L) - f32
     s = res["split"]
+    k = s["evals"] / s["gj"]  # executed evaluations per object-step
+    f32 = k * 25 * (terms(L) - terms(ns))
+    f64 = flops_chunks(L, s["gj"], s["evals"]) / s["gj"] - f32
     t_launch = s["gj8_ms"] / s["launches"] * 1e-3
     per_launch = s["gj"] / s["launches"]
     bound_s = (f32 / (PEAK_FP32_TFLOPS * 1e12) + f64 / (PEAK_FP64_TFLOPS * 1e12)) * per_launch
@@ -226,6 +254,9 @@ def leg_split(ogp, scenarios, args):
         nonpacked = (hw32["tflops"] + hw64["tflops"]) / PEAK_FP32_NONPACKED_TFLOPS
     out["split_roofline"] = dict(frac=bound_s / t_launch, fp32_flops_per_object_step=f32,
                                  fp64_flops_per_object_step=f64,
+                                 evaluations_per_object_step=k,
+                                 fp64_path_evaluations_per_object_step=(
+                                     res["fp64"]["evals"] / res["fp64"]["gj"]),
                                  elapsed_ms_per_launch=t_launch * 1e3,
                                  avg_launch_ms=s["launch_ms_sum"] / s["launches"],
                                  frac_basis="algorithmic FP32 at the packed 157.3 TFLOP/s peak, "
@@ -261,6 +292,7 @@ def leg_pairs(ogp, scenarios):
         t0 = time.perf_counter()
         ca.propagate(c4["steps"])
         t1 = time.perf_counter()
+        prop_evals = ctx.timing()["evaluations"]
         res = ca.find_closest(threshold=10e3)
         t2 = time.perf_counter()
         t = ca.timing()
@@ -287,9 +319,11 @@ def leg_pairs(ogp, scenarios):
                                              t["screen_ms"] * 1e-3, PEAK_FP64_TFLOPS)),
         propagate_roofline=dict(
             bound="valu", unit="TFLOP/s", peak=PEAK_FP64_TFLOPS,
-            flops_per_object_step=flops_step(20),
-            achieved=flops_step(20) * prop_gj / (t["propagate_ms"] * 1e-3) / 1e12,
-            frac=flops_step(20) * prop_gj / (t["propagate_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+            flops_per_object_step=flops_chunks(20, prop_gj, prop_evals) / prop_gj,
+            evaluations_per_object_step=prop_evals / prop_gj,
+            achieved=flops_chunks(20, prop_gj, prop_evals) / (t["propagate_ms"] * 1e-3) / 1e12,
+            frac=flops_chunks(20, prop_gj, prop_evals) / (t["propagate_ms"] * 1e-3) / 1e12
+            / PEAK_FP64_TFLOPS,
             hw_fp64=hw_rate("gj8_chunk_conj20k_3d_deg20_counters.md",
                             "hw_fp64_flop_per_object_step", prop_gj, t["propagate_ms"] * 1e-3,
                             PEAK_FP64_TFLOPS)))
@@ -480,6 +514,7 @@ def leg_adjoint(ogp, scenarios):
         ctx.propagate(r0[:640], v0[:640], h, steps[:640])  # node tables, wave tuning
         ctx.propagate(r0, v0, h, steps)
         fwd_ms = ctx.timing()["total_ms"]
+        fwd_evals = ctx.timing()["evaluations"]
         _, _, J = ctx.adjoint_jacobian(r0, v0, h, steps)
         ta = ctx.adjoint_timing()
         _, _, J64 = ctx.adjoint_jacobian(r0, v0, h, steps, checkpoint_steps=64)
@@ -492,8 +527,10 @@ def leg_adjoint(ogp, scenarios):
         stm_ms = ctx.adjoint_timing()["stm_ms"]
     rel = np.linalg.norm(J - Phi, axis=(1, 2)) / np.linalg.norm(Phi, axis=(1, 2))
     gj = float(np.maximum(steps - 4, 0).sum())
-    out = dict(workload="10,000 LEO, 60 s, 1,440 steps (24 h), JGM3 20x20, 7 corrector "
-                        "evaluations per step; Jacobians recorded in the forward pass",
+    out = dict(workload="10,000 LEO, 60 s, 1,440 steps (24 h), JGM3 20x20, up to 7 corrector "
+                        "evaluations per step (the propagation stops at the corrector's fixed "
+                        "point, the record pass runs all 7); Jacobians recorded in the forward "
+                        "pass",
                forward_ms=fwd_ms,
                jacobian_ms=ta["forward_ms"] + ta["adjoint_ms"],
                jacobian_forward_ms=ta["forward_ms"], jacobian_adjoint_ms=ta["adjoint_ms"],
@@ -506,10 +543,12 @@ def leg_adjoint(ogp, scenarios):
                adjoint_vs_stm_rel=dict(max=float(rel.max()), median=float(np.median(rel))),
                forward_roofline=dict(
                    bound="valu", unit="TFLOP/s", peak=PEAK_FP64_TFLOPS,
-                   achieved=flops_step(20) * gj / (fwd_ms * 1e-3) / 1e12,
-                   frac=flops_step(20) * gj / (fwd_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-                   basis="algorithmic FLOPs of the GJ8 steps over the whole propagation "
-                         "(startup included)"),
+                   evaluations_per_object_step=fwd_evals / gj,
+                   achieved=flops_chunks(20, gj, fwd_evals) / (fwd_ms * 1e-3) / 1e12,
+                   frac=flops_chunks(20, gj, fwd_evals) / (fwd_ms * 1e-3) / 1e12
+                   / PEAK_FP64_TFLOPS,
+                   basis="algorithmic FLOPs of the GJ8 steps (executed evaluations) over the "
+                         "whole propagation (startup included)"),
                # the Jacobian's forward pass is the record pass over the whole span (its
                # variational startup included in the time)
                record_hw_fp64=hw_rate("gj8_record_adjoint10k_24h_deg20_counters.md",

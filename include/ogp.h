@@ -212,6 +212,8 @@ typedef struct ogp_timing {
     double gj_object_steps; /* object-steps executed by the GJ8 chunk kernels */
     double launch_ms_sum;   /* sum of the GJ8 chunk launches' own durations; launches of
                                concurrent object slices overlap, so this exceeds steps_ms */
+    double evaluations;     /* gravity evaluations the GJ8 chunk kernels ran, counted per object
+                               that stores the step (at most n_corrections per object-step) */
 } ogp_timing;
 int ogp_last_timing(const ogp_context* ctx, ogp_timing* t);
 
@@ -234,6 +236,15 @@ int ogp_get_tuning(const ogp_context* ctx, int64_t n, int* waves, int64_t* chunk
  * choice for n objects. */
 int ogp_set_lockstep(ogp_context* ctx, int groups);
 int ogp_get_lockstep(const ogp_context* ctx, int64_t n, int* groups);
+
+/* Corrector evaluations of a GJ8 step (no counterpart in the reference, which always runs
+ * n_corrections, gpu_propagator.cu:102-117). With on = 1 (the default) a step of a group of 64
+ * objects stops evaluating once the state to evaluate next is, for every object of the group that
+ * stores the step, bit for bit the state just evaluated: each further evaluation and correction
+ * would repeat the last one exactly, so every result is bit-identical to running all
+ * n_corrections evaluations (typically 2-4 of 7 are run). 0 always runs n_corrections. Lockstep
+ * groups (ogp_set_lockstep 2) always run n_corrections. */
+int ogp_set_fixed_point_exit(ogp_context* ctx, int on);
 
 /* Object slices per step-size class (no counterpart in the reference): each class is cut into
  * `slices` contiguous slices whose GJ8 chunk sequences run concurrently on streams of their own,

@@ -35,7 +35,8 @@ class OgpConfig(ctypes.Structure):
 class OgpTiming(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("total_ms", "nodes_ms", "startup_ms", "steps_ms", "h2d_ms", "d2h_ms",
-                 "step_launches", "object_steps", "gj_object_steps", "launch_ms_sum")]
+                 "step_launches", "object_steps", "gj_object_steps", "launch_ms_sum",
+                 "evaluations")]
 
 
 class OgpSources(ctypes.Structure):
@@ -98,6 +99,7 @@ def lib():
         "ogp_last_timing": (ctypes.c_int, [_P, ctypes.POINTER(OgpTiming)]),
         "ogp_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, _I64]),
         "ogp_set_slices": (ctypes.c_int, [_P, ctypes.c_int]),
+        "ogp_set_fixed_point_exit": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_set_lockstep": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_set_screen_shape": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_get_screen_shape": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int)]),

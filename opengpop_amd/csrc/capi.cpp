@@ -186,6 +186,10 @@ struct ogp_context {
     static constexpr int kLaneStreams = 4;
     hipStream_t lane_streams[kLaneStreams] = {};
     int slices_override = 0; // object slices per class; 0 = chosen from the class size
+    // GJ8 corrector evaluations end at a bitwise fixed point (ogp_set_fixed_point_exit), and the
+    // device count of the object-evaluations the chunk kernels ran in the last call
+    bool fp_exit = true;
+    DevBuf d_evals;
     // HBM for the gravity Jacobians recorded by the derivative passes (ogp_set_derivative_memory)
     double record_bytes = 16.0 * (1 << 30);
 
@@ -454,6 +458,8 @@ struct ogp_context {
     {
         Event e0, e1, e2, e3;
         const PropState ps = prop_state(N);
+        d_evals.reserve(sizeof(unsigned long long), &dev_bytes);
+        HIPC(hipMemsetAsync(d_evals.p, 0, sizeof(unsigned long long), stream));
         // nodes
         std::vector<NodeTable*> tabs;
         HIPC(hipEventRecord(e0.e, stream));
@@ -545,7 +551,8 @@ struct ogp_context {
             spans.emplace_back(std::make_unique<Event>(), std::make_unique<Event>());
             HIPC(hipEventRecord(spans.back().first->e, st));
             launch_chunk(st, kind, fctx[q], ps, first, active, k0, k1, cfg.n_corrections, traj, 1,
-                traj_steps, shape_waves(shape), shape_groups(shape));
+                traj_steps, shape_waves(shape), shape_groups(shape), fp_exit,
+                d_evals.as<unsigned long long>());
             check_launch();
             HIPC(hipEventRecord(spans.back().second->e, st));
             launches += 1;
@@ -679,6 +686,9 @@ struct ogp_context {
             sum += ms;
         }
         timing.launch_ms_sum = sum;
+        unsigned long long evals = 0;
+        HIPC(hipMemcpy(&evals, d_evals.p, sizeof(evals), hipMemcpyDeviceToHost));
+        timing.evaluations = (double)evals;
     }
 };
 
@@ -1390,6 +1400,16 @@ int ogp_set_slices(ogp_context* ctx, int slices)
     return OGP_OK;
 }
 
+int ogp_set_fixed_point_exit(ogp_context* ctx, int on)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    if (on != 0 && on != 1)
+        return fail(ctx, OGP_ERR_ARGUMENT, "fixed-point exit must be 0 or 1");
+    ctx->fp_exit = on != 0;
+    return OGP_OK;
+}
+
 int ogp_set_derivative_memory(ogp_context* ctx, int64_t bytes)
 {
     if (!ctx)
@@ -1578,7 +1598,8 @@ void propagator_step(ogp_propagator& p, double* host_r, double* host_v, double* 
     if (k <= 4) {
         p.last_row = k - 1;
     } else {
-        launch_chunk(st, ctx->kind, f, ps, 0, n, k, k + 1, ctx->cfg.n_corrections, traj, k, 1, W);
+        launch_chunk(st, ctx->kind, f, ps, 0, n, k, k + 1, ctx->cfg.n_corrections, traj, k, 1, W, 1,
+            ctx->fp_exit);
         ctx->check_launch();
         p.last_row = 0;
     }
@@ -2486,7 +2507,8 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
         for (int64_t q = 0; q < nseg; ++q) {
             const int64_t k0 = 5 + q * C, k1 = std::min<int64_t>(k0 + C, max_steps + 1);
             launch_checkpoint(ctx->stream, pc, ctx->ad_ck.as<double>() + q * 33 * n, true);
-            launch_chunk(ctx->stream, kGravity, f, pc, 0, n, k0, k1, ncorr, nullptr, 1, 0, W);
+            launch_chunk(ctx->stream, kGravity, f, pc, 0, n, k0, k1, ncorr, nullptr, 1, 0, W, 1,
+                ctx->fp_exit);
             ctx->check_launch();
         }
     }

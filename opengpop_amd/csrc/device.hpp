@@ -641,6 +641,32 @@ __device__ __forceinline__ void gravity_helper(const GravityDev& g, double* red,
     __syncthreads();
 }
 
+// Wave w's share of every evaluation the lead runs, for as many evaluations as the lead decides:
+// after each first barrier the wave reads the lead's control word `more` (LDS), and leaves when
+// it is 0. The lead sets more = 1 before its first evaluation and more = 0, followed by one
+// barrier, after its last; it never rewrites `more` between a first barrier and the following
+// second barrier, where the helpers read it.
+template <bool SPLIT, int W>
+__device__ __forceinline__ void gravity_helper_loop(const GravityDev& g, double* red,
+    const double* pos, const int* more, int w)
+{
+    const int lane = threadIdx.x % 64;
+    for (;;) {
+        __syncthreads();
+        if (__builtin_amdgcn_readfirstlane(*more) == 0)
+            return;
+        const GravPoint p = get_point(pos, lane);
+        grav_parts<SPLIT>(p, g, wave_part<W>(w), wave_part<W>(w) + kParts / W,
+            [&](int q, const GravSums& s) {
+                red[(q * 4 + 0) * 64 + lane] = s.G;
+                red[(q * 4 + 1) * 64 + lane] = s.H;
+                red[(q * 4 + 2) * 64 + lane] = s.J;
+                red[(q * 4 + 3) * 64 + lane] = s.K;
+            });
+        __syncthreads();
+    }
+}
+
 // a = M gi and G = M Gi M^T for the column-major ITRS->GCRS matrix M.
 __device__ __forceinline__ void jac_to_gcrs(const uniform_dptr M, const double* gi,
     const double (*Gi)[3], double* a, double (*G)[3])

@@ -80,10 +80,15 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
 // GJ8 steps k0 .. k1-1 (k0 >= 5) of objects [obj0, obj0 + nobj). With traj != null, steps
 // traj_first .. traj_first + traj_steps - 1 are stored at rows 0.. of traj[row][9][N].
 // lockstep = 2 puts two object groups in each workgroup (gravity kinds with 4 or 8 waves; else
-// one).
+// one). fixed_point_exit (gravity kinds, one group per workgroup): a step of a group of 64
+// objects ends its corrector evaluations once the point to evaluate next is, for every object
+// that stores the step, bit for bit the point just evaluated (every further evaluation would
+// repeat the last one, so the results are those of all ncorr evaluations). evals (optional)
+// accumulates, over the steps, the evaluations of each step times the objects that store it.
 void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
     int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
-    int64_t traj_first, int64_t traj_steps, int waves, int lockstep = 1);
+    int64_t traj_first, int64_t traj_steps, int waves, int lockstep = 1,
+    bool fixed_point_exit = false, unsigned long long* evals = nullptr);
 
 // The kParts balanced contiguous column ranges of the harmonic sums (fills kParts + 1 bounds,
 // parts[0] = 0, parts[kParts] = L + 1, non-decreasing).

@@ -175,6 +175,15 @@ template <int W> struct ExchangeArea {
     static constexpr int size = W > 1 ? kParts * 4 * 64 : 1;
 };
 
+// Equal bit patterns (unlike ==, tells +0 from -0).
+__device__ __forceinline__ bool same_bits(double a, double b)
+{
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+// pred holds in every lane of the wave (the same answer in every lane).
+__device__ __forceinline__ bool wave_all(bool pred) { return __ballot(!pred) == 0; }
+
 // Lane l's object in object group `group` of the launch. Lanes past the launch's objects read
 // object obj0 (a valid index) and count as having no steps.
 struct Lane {
@@ -284,12 +293,13 @@ __global__ void __launch_bounds__(64 * W * G)
 __attribute__((amdgpu_waves_per_eu(OGP_CHUNK_WAVES_PER_EU)))
 gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj, int64_t traj_first,
-    int64_t traj_steps)
+    int64_t traj_steps, int fixed_point_exit, unsigned long long* evals)
 {
     static_assert(G == 1 || W > 1, "lockstep groups need the lead/helper form");
     __shared__ double red[G * ExchangeArea<W>::size];
     __shared__ double pos[W > 1 ? G * kPointLen * 64 : 1]; // the leads' geometry
     __shared__ int64_t group_end[G];
+    __shared__ int more; // one group: the lead's control word for its helpers
     const int wg = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
     const int g = wg / W, w = wg % W; // object group of the workgroup, wave within the group
     double* gred = red + (W > 1 ? g * ExchangeArea<W>::size : 0);
@@ -312,16 +322,31 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     kend = __builtin_amdgcn_readfirstlane((int)kend);
     if (kend <= k0)
         return;
+    // One group per workgroup: a step may end its corrector evaluations at a fixed point (below),
+    // so the helpers follow the lead's control word. Lockstep groups pass common barriers and
+    // run every step's ncorr evaluations.
+    const bool fp_exit = G == 1 && fixed_point_exit != 0;
     if constexpr (W > 1) {
         // Waves 1 .. W-1 of a group only sum their harmonic parts; its wave 0 integrates. Two
-        // barriers per evaluation, in every wave for every step up to kend.
+        // barriers per evaluation in every wave, and with one group a last barrier after which
+        // the helpers read more == 0.
         if (w > 0) {
-            for (int64_t k = k0; k < kend; ++k)
-                for (int it = 0; it < ncorr; ++it)
-                    gravity_helper<KIND == kGravitySplit, W>(f.grav, gred, gpos, w);
+            if constexpr (G == 1) {
+                gravity_helper_loop<KIND == kGravitySplit, W>(f.grav, gred, gpos, &more, w);
+            } else {
+                for (int64_t k = k0; k < kend; ++k)
+                    for (int it = 0; it < ncorr; ++it)
+                        gravity_helper<KIND == kGravitySplit, W>(f.grav, gred, gpos, w);
+            }
             return;
         }
+        if constexpr (G == 1) {
+            if (threadIdx.x == 0)
+                more = 1;
+        }
     }
+    // Evaluations of the live objects of the group (lanes that store step k), for the FLOP count.
+    unsigned long long done = 0;
     const double h = f.h, hh = h * h;
     double r[3], v[3], s[3], S[3];
 #pragma unroll
@@ -344,6 +369,8 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
         // Every lane runs every step up to kend; a lane past its span (k > ns) integrates on
         // a history it no longer updates and stores nothing: its final state was stored at its
         // last step.
+        const bool live = k <= ns;
+        int nev = 0;
         GJ8Sums q;
         gj8_sums(ps.hist, N, i, k, q);
         double sprev[3], anew[3];
@@ -377,19 +404,38 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
                 to_itrs(M, t, base);
                 for (int it = 0; it < ncorr; ++it) {
                     gravity(ri, gi);
+                    ++nev;
+                    bool same = true;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        ri[c] = hh * __builtin_fma(c_gj[8][8], gi[c], base[c]);
+                    for (int c = 0; c < 3; ++c) {
+                        const double rn = hh * __builtin_fma(c_gj[8][8], gi[c], base[c]);
+                        same = same && same_bits(rn, ri[c]);
+                        ri[c] = rn;
+                    }
+                    // The next point is the one just evaluated: every further evaluation would
+                    // return gi again, bit for bit, so the step's result is already final.
+                    if (fp_exit && wave_all(same || !live))
+                        break;
                 }
                 to_gcrs(M, gi, anew);
                 gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
             } else {
                 for (int it = 0; it < ncorr; ++it) {
                     gravity(ri, gi);
+                    ++nev;
                     to_gcrs(M, gi, anew);
                     add_sources(f.src, f.prop_tb + 6 * (k + 4), r, v, anew);
+                    const double r0[3] = { r[0], r[1], r[2] }, v0[3] = { v[0], v[1], v[2] };
                     gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
                     to_itrs(M, r, ri);
+                    // The corrected state is the one just evaluated: the next acceleration, and
+                    // every correction after it, would repeat this one bit for bit.
+                    bool same = true;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        same = same && same_bits(r[c], r0[c]) && same_bits(v[c], v0[c]);
+                    if (fp_exit && wave_all(same || !live))
+                        break;
                 }
             }
         } else {
@@ -397,7 +443,9 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
                 eval_prop<KIND, W>(f, r, v, k + 4, anew, ex);
                 gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
             }
+            nev = ncorr;
         }
+        done += (unsigned long long)nev * (unsigned long long)__popcll(__ballot(live));
         if (k > ns)
             continue;
         const int wslot = ring_slot(k);
@@ -416,8 +464,16 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
         if (k == ns)
             store_state();
     }
+    if constexpr (W > 1 && G == 1) {
+        // release the helpers (gravity_helper_loop)
+        if (threadIdx.x == 0)
+            more = 0;
+        __syncthreads();
+    }
     if (ns >= kend)
         store_state();
+    if (evals && threadIdx.x % 64 == 0)
+        atomicAdd(evals, done);
 }
 
 // Instantiation over force kinds (analytic test forces use W = 1 only).
@@ -471,7 +527,8 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
 
 void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
     int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
-    int64_t traj_first, int64_t traj_steps, int waves, int lockstep)
+    int64_t traj_first, int64_t traj_steps, int waves, int lockstep, bool fixed_point_exit,
+    unsigned long long* evals)
 {
     if (nobj <= 0 || k1 <= k0)
         return;
@@ -479,12 +536,13 @@ void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& 
     const int G_ = W_ >= 4 && lockstep == 2 ? 2 : 1;
     const int64_t ngroups = (nobj + 63) / 64;
     const dim3 grid((unsigned)((ngroups + G_ - 1) / G_));
+    const int fp_ = fixed_point_exit ? 1 : 0;
 #define L_(K, Wv)                                                                                  \
     hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, 1>), grid, dim3(64 * (Wv)), 0, st, f, ps, obj0,   \
-        nobj, k0, k1, ncorr, traj, traj_first, traj_steps)
+        nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp_, evals)
 #define L2_(K, Wv)                                                                                 \
     hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, 2>), grid, dim3(64 * 2 * (Wv)), 0, st, f, ps,      \
-        obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps)
+        obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp_, evals)
     if (G_ == 2) {
         const bool split_ = kind == kGravitySplit;
         if (W_ == 4) {

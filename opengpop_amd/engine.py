@@ -240,6 +240,11 @@ class Context:
         """Object slices per step-size class run on concurrent streams (0 = automatic)."""
         check(self.h, lib().ogp_set_slices(self.h, slices))
 
+    def set_fixed_point_exit(self, on=True):
+        """End a GJ8 step's corrector evaluations once the next state to evaluate repeats the
+        last one bit for bit (results unchanged; the default), or always run n_corrections."""
+        check(self.h, lib().ogp_set_fixed_point_exit(self.h, 1 if on else 0))
+
     def set_derivative_memory(self, nbytes=0):
         """HBM budget for the Jacobians recorded by the derivative passes (0 = 16 GiB)."""
         check(self.h, lib().ogp_set_derivative_memory(self.h, int(nbytes)))
