@@ -209,15 +209,19 @@ __global__ void checkpoint_kernel(PropState ps, double* __restrict__ ck, int to_
 
 // GJ8 steps k0 .. k1-1 exactly as gj8_chunk_kernel<kGravity> (same arithmetic, gj8.hpp, and a
 // bit-identical acceleration), storing the GCRS gravity Jacobian of every corrector iterate:
-// Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + o], e = xx, xy, xz, yy, yz, zz.
+// Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + o], e = xx, xy, xz, yy, yz, zz. With
+// fixed_point_exit a step stops evaluating, as the chunk kernel does, once the next point repeats
+// the point just evaluated for every object that stores the step; the remaining iterates are at
+// that same point, and their Jacobians are the last one, stored again (Gbuf bit-identical).
 template <int W>
 __global__ void __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(2))) // at most 256 VGPRs + AGPRs: 2 resident waves per SIMD
 gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
-    int64_t k1, int ncorr, double* __restrict__ Gbuf)
+    int64_t k1, int ncorr, double* __restrict__ Gbuf, int fixed_point_exit)
 {
     __shared__ double red[W > 1 ? kJacExchange : 1];
     __shared__ double pos[W > 1 ? 3 * 64 : 1]; // the lead's ITRS positions
+    __shared__ int more; // the lead's control word for its helpers
     // Every barrier sits in workgroup-uniform control flow: the loops run to the group's last
     // step (the same in every wave, which hold the same objects); a lane past its span, or past
     // the objects, integrates on a history it does not update and stores nothing.
@@ -232,15 +236,16 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         return;
     if constexpr (W > 1) {
         // The W waves of a workgroup hold the same 64 objects and share the harmonic columns.
-        // Waves 1 .. W-1 only sum their Dual3 parts (gravity_jac_helper); wave 0 integrates,
-        // records the Jacobians and writes the ring and the state. Two barriers per evaluation.
+        // Waves 1 .. W-1 only sum their Dual3 parts (gravity_jac_helper_loop) for as many
+        // evaluations as wave 0 runs; wave 0 integrates, records the Jacobians and writes the ring
+        // and the state. Two barriers per evaluation, one more at the end.
         const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
         if (w > 0) {
-            for (int64_t k = k0; k < kend; ++k)
-                for (int it = 0; it < ncorr; ++it)
-                    gravity_jac_helper<W>(f.grav, red, pos, w);
+            gravity_jac_helper_loop<W>(f.grav, red, pos, &more, w);
             return;
         }
+        if (threadIdx.x == 0)
+            more = 1;
     }
     const double h = f.h, hh = h * h;
     double r[3], v[3], s[3], S[3];
@@ -282,13 +287,24 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
             else
                 gravity_itrs_jac(ri[0], ri[1], ri[2], f.grav, gi, Gi);
             jac_to_gcrs(M, gi, Gi, anew, G);
-            if (act) {
-                const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
+            const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
+            if (act)
                 for (int e = 0; e < 6; ++e)
                     Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
+            bool same = true;
+            for (int c = 0; c < 3; ++c) {
+                const double rn = hh * __builtin_fma(c_gj[8][8], gi[c], base[c]);
+                same = same && __double_as_longlong(rn) == __double_as_longlong(ri[c]);
+                ri[c] = rn;
             }
-            for (int c = 0; c < 3; ++c)
-                ri[c] = hh * __builtin_fma(c_gj[8][8], gi[c], base[c]);
+            if (fixed_point_exit && __ballot(!(same || !act)) == 0) {
+                // every later iterate is at this point: the same acceleration and Jacobian
+                if (act)
+                    for (int j = it + 1; j < ncorr; ++j)
+                        for (int e = 0; e < 6; ++e)
+                            Gbuf[(((k - k0) * ncorr + j) * 6 + e) * n + i] = g6[e];
+                break;
+            }
         }
         gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         if (!act)
@@ -299,36 +315,44 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         if (k == ns)
             store_state();
     }
+    if constexpr (W > 1) {
+        // release the helpers (gravity_jac_helper_loop)
+        if (threadIdx.x == 0)
+            more = 0;
+        __syncthreads();
+    }
     if (ns >= kend)
         store_state();
 }
 
 // Reverse pass over steps k1-1 .. k0 for R cotangent rows. lam[(rho * 33 + e) * n + o] holds the
 // cotangents of (s, S, ring) after the step being reversed; seeds[(rho * 6 + e) * n + o] the
-// cotangent of the final (r, v), injected at the object's last step.
+// cotangent of the final (r, v), injected at the object's last step. One lane per (object, row):
+// block b reverses row b % R of objects 64 (b / R) .. +63, so the R blocks of an object group are
+// adjacent in the grid and read the same recorded Jacobians close together in time.
 __global__ void __launch_bounds__(64) gj8_reverse_kernel(const int64_t* __restrict__ nsteps,
     int64_t n, double h, int64_t k0, int64_t k1, int ncorr, const double* __restrict__ Gbuf,
     int R, const double* __restrict__ seeds, double* __restrict__ lam)
 {
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int rho = (int)(blockIdx.x % R);
+    const int64_t i = (int64_t)(blockIdx.x / R) * 64 + threadIdx.x;
     if (i >= n)
         return;
     const int64_t ns = nsteps[i];
     const double hh = h * h, sa88 = c_sa[8][8], gj88 = c_gj[8][8];
+    double* L = lam + (int64_t)rho * 33 * n;
     for (int64_t k = k1 - 1; k >= k0; --k) {
         if (k > ns)
             continue;
         const int sk = ring_slot(k);
-        for (int rho = 0; rho < R; ++rho) {
-            double* L = lam + (int64_t)rho * 33 * n;
-            double le[kMaxCorr][3];
-            for (int it = 0; it < ncorr; ++it)
-                for (int c = 0; c < 3; ++c)
-                    le[it][c] = 0.0;
+        {
+            // the cotangent of iterate it's acceleration gathers only in that iterate's reversal
+            // (and, for the last iterate, from the ring entry a_k it became)
+            double lk[3];
             double lr[3], lv[3], lsp[3], lT1[3], lT2[3], ls[3], lal[3], lS[3];
             for (int c = 0; c < 3; ++c) {
                 // the new ring entry a_k is the last iterate's acceleration
-                le[ncorr - 1][c] = L[(6 + sk * 3 + c) * n + i];
+                lk[c] = L[(6 + sk * 3 + c) * n + i];
                 L[(6 + sk * 3 + c) * n + i] = 0.0; // the slot now accumulates a_{k-9}
                 lr[c] = k == ns ? seeds[(rho * 6 + c) * n + i] : 0.0;
                 lv[c] = k == ns ? seeds[(rho * 6 + 3 + c) * n + i] : 0.0;
@@ -337,25 +361,27 @@ __global__ void __launch_bounds__(64) gj8_reverse_kernel(const int64_t* __restri
                 lT1[c] = lT2[c] = ls[c] = lal[c] = 0.0;
             }
             for (int it = ncorr - 1; it >= 0; --it) {
+                double le[3];
                 for (int c = 0; c < 3; ++c) {
+                    le[c] = it == ncorr - 1 ? lk[c] : 0.0;
                     const double t = hh * lr[c]; // r_{it+1} = hh ((T2 + gj88 e) + S')
                     lT2[c] += t;
-                    le[it][c] += gj88 * t;
+                    le[c] += gj88 * t;
                     lS[c] += t;
                     const double u = h * lv[c]; // v_{it+1} = h ((T1 + sa88 e) + s'_it)
                     lT1[c] += u;
-                    le[it][c] += sa88 * u;
+                    le[c] += sa88 * u;
                     lsp[c] += u;
                     ls[c] += lsp[c]; // s'_it = s + (e + a_{k-1}) / 2
-                    le[it][c] += lsp[c] / 2;
+                    le[c] += lsp[c] / 2;
                     lal[c] += lsp[c] / 2;
                 }
                 const double* gp = Gbuf + ((((k - k0) * ncorr + it) * 6) * n + i);
                 const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
                              gyz = gp[4 * n], gzz = gp[5 * n];
-                lr[0] = gxx * le[it][0] + gxy * le[it][1] + gxz * le[it][2];
-                lr[1] = gxy * le[it][0] + gyy * le[it][1] + gyz * le[it][2];
-                lr[2] = gxz * le[it][0] + gyz * le[it][1] + gzz * le[it][2];
+                lr[0] = gxx * le[0] + gxy * le[1] + gxz * le[2];
+                lr[1] = gxy * le[0] + gyy * le[1] + gyz * le[2];
+                lr[2] = gxz * le[0] + gyz * le[1] + gzz * le[2];
                 for (int c = 0; c < 3; ++c)
                     lv[c] = lsp[c] = 0.0;
             }
@@ -461,14 +487,14 @@ void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_
 }
 
 void launch_gj8_record(hipStream_t st, const ForceCtx& f, const PropState& ps, int64_t k0,
-    int64_t k1, int ncorr, double* Gbuf, int waves)
+    int64_t k1, int ncorr, double* Gbuf, int waves, bool fixed_point_exit)
 {
     if (ps.N <= 0 || k1 <= k0)
         return;
     const dim3 grid((unsigned)((ps.N + 63) / 64));
 #define L_(Wv)                                                                                     \
     hipLaunchKernelGGL(gj8_record_kernel<Wv>, grid, dim3(64 * (Wv)), 0, st, f, ps, k0, k1, ncorr, \
-        Gbuf)
+        Gbuf, fixed_point_exit ? 1 : 0)
     // 4 or 8 waves per group: the 64 KB exchange area leaves a 2-wave shape one wave per SIMD
     if (waves >= 8)
         L_(8);
@@ -480,9 +506,9 @@ void launch_gj8_record(hipStream_t st, const ForceCtx& f, const PropState& ps, i
 void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
     int64_t k1, int ncorr, const double* Gbuf, int R, const double* seeds, double* lam)
 {
-    if (n > 0 && k1 > k0)
-        hipLaunchKernelGGL(gj8_reverse_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
-            nsteps, n, h, k0, k1, ncorr, Gbuf, R, seeds, lam);
+    if (n > 0 && k1 > k0 && R > 0)
+        hipLaunchKernelGGL(gj8_reverse_kernel, dim3((unsigned)(R * ((n + 63) / 64))), dim3(64), 0,
+            st, nsteps, n, h, k0, k1, ncorr, Gbuf, R, seeds, lam);
 }
 
 void launch_adjoint_combine(hipStream_t st, const int64_t* nsteps, int64_t n, int R,

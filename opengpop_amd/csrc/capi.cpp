@@ -2499,7 +2499,8 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
     if (record_all) {
         // forward pass = record pass over the whole span
         if (nseg > 0)
-            launch_gj8_record(ctx->stream, f, pc, 5, max_steps + 1, ncorr, ctx->ad_G.as<double>(), W);
+            launch_gj8_record(ctx->stream, f, pc, 5, max_steps + 1, ncorr, ctx->ad_G.as<double>(), W,
+                ctx->fp_exit);
         ctx->check_launch();
     } else {
         // forward pass over the GJ8 steps, (s, S, ring) stored at the start of every segment
@@ -2525,7 +2526,8 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
         const int64_t k0 = 5 + q * C, k1 = std::min<int64_t>(k0 + C, max_steps + 1);
         if (!record_all) {
             launch_checkpoint(ctx->stream, pc, ctx->ad_ck.as<double>() + q * 33 * n, false);
-            launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>(), W);
+            launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>(), W,
+                ctx->fp_exit);
         }
         launch_gj8_reverse(ctx->stream, pc.nsteps, n, h, k0, k1, ncorr, ctx->ad_G.as<double>(), R,
             ctx->ad_seeds.as<double>(), ctx->ad_lam.as<double>());
@@ -2602,7 +2604,8 @@ int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, dou
         ctx->ad_G.reserve(sizeof(double) * C * ncorr * 6 * n, &ctx->dev_bytes);
         for (int64_t k0 = 5; k0 <= max_steps; k0 += C) {
             const int64_t k1 = std::min<int64_t>(k0 + C, max_steps + 1);
-            launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>(), W);
+            launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>(), W,
+                ctx->fp_exit);
             launch_gj8_tangent(ctx->stream, pc.nsteps, n, step_s, k0, k1, ncorr,
                 ctx->ad_G.as<double>(), ctx->ad_Zd.as<double>(), ctx->ad_RVd.as<double>());
             ctx->check_launch();

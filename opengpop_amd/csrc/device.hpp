@@ -751,6 +751,26 @@ __device__ __forceinline__ void gravity_jac_helper(const GravityDev& g, double* 
     __syncthreads();
 }
 
+// gravity_jac_helper for as many evaluations as the lead runs, under the control word `more`
+// of gravity_helper_loop (1 before the lead's first evaluation, 0 and one barrier after its
+// last).
+template <int W>
+__device__ __forceinline__ void gravity_jac_helper_loop(const GravityDev& g, double* red,
+    const double* pos, const int* more, int w)
+{
+    const int lane = threadIdx.x % 64;
+    for (;;) {
+        __syncthreads();
+        if (__builtin_amdgcn_readfirstlane(*more) == 0)
+            return;
+        const Dual3 X = dvar(pos[lane], 0), Y = dvar(pos[64 + lane], 1),
+                    Z = dvar(pos[128 + lane], 2);
+        const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
+        jac_parts_to_lds<W>(p, g, red, w);
+        __syncthreads();
+    }
+}
+
 // Force of the variational-equation lanes (kGravityTangent) at node matrix Mg.
 __device__ __forceinline__ void tangent_force(const double* Mg, const double* x,
     const GravityDev& g, double* out)

@@ -134,8 +134,9 @@ void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_
 // Steps k0 .. k1-1 of the plain state ps with the gravity Jacobian of every corrector iterate
 // stored (6 unique entries); the state is advanced as by the chunk kernel. `waves` (4 below 8,
 // else 8) splits the harmonic columns as in the chunk kernel; results do not depend on it.
+// fixed_point_exit: as launch_chunk; the iterates after the fixed point store its Jacobian.
 void launch_gj8_record(hipStream_t st, const ForceCtx& f, const PropState& ps, int64_t k0,
-    int64_t k1, int ncorr, double* Gbuf, int waves);
+    int64_t k1, int ncorr, double* Gbuf, int waves, bool fixed_point_exit = false);
 void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
     int64_t k1, int ncorr, const double* Gbuf, int R, const double* seeds, double* lam);
 void launch_adjoint_combine(hipStream_t st, const int64_t* nsteps, int64_t n, int R,

@@ -92,3 +92,24 @@ def test_fixed_point_exit_conjunction_and_stepwise():
     assert len(out[True][1]) > 0
     for x, y in zip(out[False], out[True]):
         assert np.array_equal(x, y)
+
+
+def test_fixed_point_exit_derivatives_bit_identical():
+    # the derivative record pass stores the fixed point's Jacobian for the skipped iterates: the
+    # adjoint Jacobian (whole-span record and checkpointed), the VJP and the STM are unchanged
+    print("[fixed-point] derivatives", flush=True)
+    r0, v0 = scenarios.leo(150, seed=47)
+    steps = np.r_[np.full(75, 300), 5 + (np.arange(75) * 7) % 290]
+    rng = np.random.default_rng(5)
+    gr, gv = rng.normal(size=(150, 3)), rng.normal(size=(150, 3))
+    out = {}
+    with ogp.Context(degree=20) as c:
+        for on in (False, True):
+            c.set_fixed_point_exit(on)
+            out[on] = (c.adjoint_jacobian(r0, v0, 60.0, steps),
+                       c.adjoint_jacobian(r0, v0, 60.0, steps, checkpoint_steps=64),
+                       c.adjoint_vjp(r0, v0, 60.0, steps, gr, gv),
+                       c.stm(r0, v0, 60.0, steps))
+    for x, y in zip(out[False], out[True]):
+        for a, b in zip(x, y):
+            assert np.array_equal(a, b)
