@@ -661,6 +661,7 @@ def main():
         line.update(value=f["value"], ms_per_step=f["ms_per_step"], roofline=f["roofline"])
         line["config"]["object_steps"] = f["object_steps"]
         line["device"] = f["device"]
+        line["all_corrections"] = f["all_corrections"]
         line["wave_tuning"] = f["tuning"]
     split_sample = adjoint_sample = pairs_sample = None
     if "split" in legs:

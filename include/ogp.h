@@ -238,12 +238,15 @@ int ogp_set_lockstep(ogp_context* ctx, int groups);
 int ogp_get_lockstep(const ogp_context* ctx, int64_t n, int* groups);
 
 /* Corrector evaluations of a GJ8 step (no counterpart in the reference, which always runs
- * n_corrections, gpu_propagator.cu:102-117). With on = 1 (the default) a step of a group of 64
- * objects stops evaluating once the state to evaluate next is, for every object of the group that
- * stores the step, bit for bit the state just evaluated: each further evaluation and correction
- * would repeat the last one exactly, so every result is bit-identical to running all
- * n_corrections evaluations (typically 2-4 of 7 are run). 0 always runs n_corrections. Lockstep
- * groups (ogp_set_lockstep 2) always run n_corrections. */
+ * n_corrections, gpu_propagator.cu:102-117, and n_corrections startup corrections,
+ * gpu_startup.cu:63-65). With on = 1 (the default) a step of a group of 64 objects stops
+ * evaluating once the state to evaluate next is, for every object of the group that stores the
+ * step, bit for bit the state just evaluated: each further evaluation and correction would repeat
+ * the last one exactly, so every result is bit-identical to running all n_corrections
+ * evaluations (typically 2-4 of 7 are run). The startup corrections stop alike once the corrected
+ * nodes repeat the nodes of the last acceleration matrix, and the derivative passes record the
+ * fixed point's Jacobian for the remaining iterates. 0 always runs n_corrections. Lockstep groups
+ * (ogp_set_lockstep 2) always run n_corrections. */
 int ogp_set_fixed_point_exit(ogp_context* ctx, int on);
 
 /* Object slices per step-size class (no counterpart in the reference): each class is cut into

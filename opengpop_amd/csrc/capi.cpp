@@ -497,7 +497,7 @@ struct ogp_context {
             const Group& g = groups[q];
             const int W = waves_for(g.count);
             launch_startup(stream, kind, force_ctx(tabs[q], g.h), rk8_variant, ps, scratch(),
-                g.first, g.count, cfg.n_corrections, traj, traj_steps, W);
+                g.first, g.count, cfg.n_corrections, traj, traj_steps, W, fp_exit);
             check_launch();
         }
         HIPC(hipEventRecord(e2.e, stream));
@@ -605,7 +605,8 @@ struct ogp_context {
                     for (hipStream_t st : mine)
                         join(st, big);
                     launch_startup(mine[0], kind, fctx[q], rk8_variant, ps, scratch(), g.first,
-                        g.count, cfg.n_corrections, traj, traj_steps, waves_for(g.count));
+                        g.count, cfg.n_corrections, traj, traj_steps, waves_for(g.count),
+                        fp_exit);
                     check_launch();
                     for (hipStream_t st : mine)
                         join(st, { mine[0] });
@@ -1592,7 +1593,7 @@ void propagator_step(ogp_propagator& p, double* host_r, double* host_v, double* 
         launch_aos_to_soa(st, p.v_aos, ps.v, n);
         ctx->reserve_scratch(n);
         launch_startup(st, ctx->kind, f, ctx->rk8_variant, ps, ctx->scratch(), 0, n,
-            ctx->cfg.n_corrections, traj, 4, W);
+            ctx->cfg.n_corrections, traj, 4, W, ctx->fp_exit);
         ctx->check_launch();
     }
     if (k <= 4) {
@@ -1767,7 +1768,8 @@ int ogp_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
         const int W = ctx->waves_for(n);
         ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, nullptr);
         launch_startup(ctx->stream, ctx->kind, ctx->force_ctx(nt, step_s), ctx->rk8_variant,
-            ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0, W);
+            ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0, W,
+            ctx->fp_exit);
         ctx->check_launch();
         ctx->sync();
         // scratch [slot][c][obj] -> caller [obj][slot][c]
@@ -1817,7 +1819,8 @@ int ogp_startup_device(ogp_context* ctx, const double* d_r0, const double* d_v0,
         const int W = ctx->waves_for(n);
         ctx->bc_obj = ctx->stage_bc(ctx->d_bc, n, nullptr);
         launch_startup(ctx->stream, ctx->kind, ctx->force_ctx(nt, step_s), ctx->rk8_variant,
-            ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0, W);
+            ctx->prop_state(n), ctx->scratch(), 0, n, ctx->cfg.n_corrections, nullptr, 0, W,
+            ctx->fp_exit);
         // scratch [slot][c][obj] -> caller [obj][slot][c]
         double* outs[5] = { d_pos, d_vel, d_acc, d_s, d_S };
         for (int q = 0; q < 5; ++q)
@@ -2413,7 +2416,7 @@ PropState tangent_startup(ogp_context* ctx, const double* r0, const double* v0, 
     NodeTable* nt = ctx->node_table(h, max_steps, 0.0);
     const ForceCtx f = ctx->force_ctx(nt, h);
     launch_startup(ctx->stream, kGravityTangent, f, ctx->rk8_variant, ps7, ctx->scratch(), 0, N7,
-        ctx->cfg.n_corrections, nullptr, 0, 1);
+        ctx->cfg.n_corrections, nullptr, 0, 1, ctx->fp_exit);
     ctx->check_launch();
     *nt_out = nt;
     *f_out = f;

@@ -73,9 +73,12 @@ void launch_gravity_aos(hipStream_t st, bool split, const GravityDev& g, const d
 
 // Startup and GJ8 kernels (propagate.hip). `waves` = W waves per group of 64 objects
 // (1, 2, 4 or 8; analytic test forces always use 1). Results do not depend on W.
+// fixed_point_exit: the startup corrections of a 64-object group stop once the corrected nodes
+// repeat the nodes of the last acceleration matrix bit for bit for every object that runs (the
+// results are those of all ncorr corrections).
 void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant,
     const PropState& ps, const StartupScratch& sc, int64_t obj0, int64_t nobj, int ncorr,
-    double* traj, int64_t traj_steps, int waves);
+    double* traj, int64_t traj_steps, int waves, bool fixed_point_exit = false);
 
 // GJ8 steps k0 .. k1-1 (k0 >= 5) of objects [obj0, obj0 + nobj). With traj != null, steps
 // traj_first .. traj_first + traj_steps - 1 are stored at rows 0.. of traj[row][9][N].

@@ -24,6 +24,15 @@ namespace ogp {
 
 namespace {
 
+// Equal bit patterns (unlike ==, tells +0 from -0).
+__device__ __forceinline__ bool same_bits(double a, double b)
+{
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+// pred holds in every lane of the wave (the same answer in every lane).
+__device__ __forceinline__ bool wave_all(bool pred) { return __ballot(!pred) == 0; }
+
 // One object's rows of a startup work area ([slot][component][object]). Stores happen only for
 // lanes that run (`on`); the others read and discard.
 struct Slots {
@@ -135,11 +144,15 @@ __device__ void acc_matrix(const ForceCtx& f, const Slots& nr, const Slots& nv, 
 }
 
 // StartupCorrector::Correct (gpu_startup.cu:247-294) with CalculateS0 (:170-195) and
-// CalculateAllSKernel (calculate_all_s.cu:19-47).
+// CalculateAllSKernel (calculate_all_s.cu:19-47). With fixed_point_exit, when the corrected
+// nodes are bit for bit the nodes the acceleration matrix was evaluated at, for every lane that
+// runs, the matrix is not evaluated again and true is returned: every later correction would
+// repeat this one exactly (the caller stops).
 template <int KIND, int W>
-__device__ void startup_correct(const ForceCtx& f, double h, const Slots& nr, const Slots& nv,
-    const Slots& acc, const Slots& sm, const Slots& Sm, Exchange& ex)
+__device__ bool startup_correct(const ForceCtx& f, double h, const Slots& nr, const Slots& nv,
+    const Slots& acc, const Slots& sm, const Slots& Sm, Exchange& ex, bool fixed_point_exit)
 {
+    bool same = true;
     const double invh = 1.0 / h, invh2 = 1.0 / (h * h), hh = h * h;
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
@@ -164,25 +177,21 @@ __device__ void startup_correct(const ForceCtx& f, double h, const Slots& nr, co
                 sv = fma(acc.get(k, c), c_sa[j][k], sv);
                 sp = fma(acc.get(k, c), c_gj[j][k], sp);
             }
-            nv.set(j, c, (sm.get(j, c) + sv) * h);
-            nr.set(j, c, (Sm.get(j, c) + sp) * hh);
+            const double vn = (sm.get(j, c) + sv) * h, rn = (Sm.get(j, c) + sp) * hh;
+            same = same && same_bits(vn, nv.get(j, c)) && same_bits(rn, nr.get(j, c));
+            nv.set(j, c, vn);
+            nr.set(j, c, rn);
         }
     }
+    if (fixed_point_exit && wave_all(same || !nr.on))
+        return true;
     acc_matrix<KIND, W>(f, nr, nv, acc, false, ex);
+    return false;
 }
 
 template <int W> struct ExchangeArea {
     static constexpr int size = W > 1 ? kParts * 4 * 64 : 1;
 };
-
-// Equal bit patterns (unlike ==, tells +0 from -0).
-__device__ __forceinline__ bool same_bits(double a, double b)
-{
-    return __double_as_longlong(a) == __double_as_longlong(b);
-}
-
-// pred holds in every lane of the wave (the same answer in every lane).
-__device__ __forceinline__ bool wave_all(bool pred) { return __ballot(!pred) == 0; }
 
 // Lane l's object in object group `group` of the launch. Lanes past the launch's objects read
 // object obj0 (a valid index) and count as having no steps.
@@ -210,10 +219,12 @@ template <int KIND, int W>
 __global__ void __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(KIND == kGravity || KIND == kGravitySplit ? 4 : 2)))
 startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
-    StartupScratch sc, int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps)
+    StartupScratch sc, int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps,
+    int fixed_point_exit)
 {
     __shared__ double red[ExchangeArea<W>::size];
     __shared__ double pos[W > 1 ? kPointLen * 64 : 1]; // the lead's geometry (gravity_gcrs_lead)
+    __shared__ int more; // the lead's control word for its helpers (gravity_helper_loop)
     Exchange ex { red, pos };
     const Lane ln = lane_object(obj0, nobj, blockIdx.x);
     const int64_t i = ln.i;
@@ -227,13 +238,12 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
         const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
         if (w > 0) {
             // one part sum per force evaluation of the lead: 8 RK8 steps of 10 stages, the
-            // acceleration matrix (9 nodes) and ncorr corrections of 8 nodes each
-            const int nevals = 8 * 10 + 9 + 8 * ncorr;
-#pragma unroll 1
-            for (int e = 0; e < nevals; ++e)
-                gravity_helper<KIND == kGravitySplit, W>(f.grav, red, pos, w);
+            // acceleration matrix (9 nodes) and up to ncorr corrections of 8 nodes each
+            gravity_helper_loop<KIND == kGravitySplit, W>(f.grav, red, pos, &more, w);
             return;
         }
+        if (threadIdx.x == 0)
+            more = 1;
     }
     const Slots nr { sc.node_r, N, i, act }, nv { sc.node_v, N, i, act }, acc { sc.acc, N, i, act };
     const Slots sm { sc.sm, N, i, act }, Sm { sc.Sm, N, i, act }, Kx { sc.Kx, N, i, act },
@@ -253,7 +263,14 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     acc_matrix<KIND, W>(f, nr, nv, acc, true, ex);
 #pragma unroll 1
     for (int it = 0; it < ncorr; ++it)
-        startup_correct<KIND, W>(f, h, nr, nv, acc, sm, Sm, ex);
+        if (startup_correct<KIND, W>(f, h, nr, nv, acc, sm, Sm, ex, fixed_point_exit != 0))
+            break;
+    if constexpr (W > 1 && (KIND == kGravity || KIND == kGravitySplit)) {
+        // release the helpers (gravity_helper_loop), in wave-uniform control flow
+        if (threadIdx.x == 0)
+            more = 0;
+        __syncthreads();
+    }
     if (!act)
         return;
     const int last = (int)(ns < 4 ? ns : 4);
@@ -512,7 +529,7 @@ int effective_waves(int kind, int waves)
 
 void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant,
     const PropState& ps, const StartupScratch& sc, int64_t obj0, int64_t nobj, int ncorr,
-    double* traj, int64_t traj_steps, int waves)
+    double* traj, int64_t traj_steps, int waves, bool fixed_point_exit)
 {
     if (nobj <= 0)
         return;
@@ -520,7 +537,7 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
     const int W_ = effective_waves(kind, waves);
 #define L_(K, Wv)                                                                                  \
     hipLaunchKernelGGL((startup_kernel<K, Wv>), grid, dim3(64 * (Wv)), 0, st, f, rk8_variant, ps,   \
-        sc, obj0, nobj, ncorr, traj, traj_steps)
+        sc, obj0, nobj, ncorr, traj, traj_steps, fixed_point_exit ? 1 : 0)
     OGP_DISPATCH(kind, W_, L_);
 #undef L_
 }

@@ -113,3 +113,21 @@ def test_fixed_point_exit_derivatives_bit_identical():
     for x, y in zip(out[False], out[True]):
         for a, b in zip(x, y):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("force", ["gravity", "sources"])
+def test_fixed_point_exit_startup_bit_identical(force):
+    # GpuStartup::Startup's work areas (nodes, s, S and the acceleration matrix): the startup
+    # corrections stop once the corrected nodes repeat the evaluated ones, with the same results
+    print(f"[fixed-point] startup {force}", flush=True)
+    d = scenarios.mixed(150, seed=49)
+    out = {}
+    with ogp.Context(degree=20) as c:
+        if force == "sources":
+            c.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON | ogp.SOURCE_DRAG, 0.02)
+        for on in (False, True):
+            c.set_fixed_point_exit(on)
+            res = [c.startup(d["r"], d["v"], h) for h in (30.0, 300.0)]
+            out[on] = [res[j][k] for j in range(2) for k in sorted(res[j])]
+    for x, y in zip(out[False], out[True]):
+        assert np.array_equal(x, y)
