@@ -190,6 +190,13 @@ struct ogp_context {
     // device count of the object-evaluations the chunk kernels ran in the last call
     bool fp_exit = true;
     DevBuf d_evals;
+    // orbit-size keys of the device-resident inputs (plan_groups)
+    DevBuf d_key;
+    // The last-step window within which plan_groups orders objects by orbit size: 16 steps, or
+    // the largest divisor of the chunk length below it (100,000 mixed objects: 619 ms per
+    // propagation against 635 ms ordered by step count alone, 622-626 ms with windows of 8, 32
+    // and 64 steps; 50,000 LEO over 7 days 7% faster with any window)
+    int64_t sort_window() const { return std::gcd<int64_t>(chunk, 16); }
     // HBM for the gravity Jacobians recorded by the derivative passes (ogp_set_derivative_memory)
     double record_bytes = 16.0 * (1 << 30);
 
@@ -451,8 +458,9 @@ struct ogp_context {
 
     // Startup and GJ8 steps for all groups of the internal state (class-sorted order).
     // traj (optional) receives steps 1..traj_steps of the single group. ns (optional, host,
-    // class-sorted order, descending within a group) trims every chunk launch to the prefix of
-    // objects that still have steps in it.
+    // class-sorted order, plan_groups: the objects that run at a chunk start form a prefix of
+    // their class) trims every chunk launch to the prefix of objects that still have steps in
+    // it.
     void run_groups(const std::vector<Group>& groups, int64_t N, double t0, double* traj,
         int64_t traj_steps, const int64_t* ns = nullptr)
     {
@@ -1032,17 +1040,39 @@ int ogp_time_nodes(ogp_context* ctx, double step_s, int before, const double* of
 namespace {
 
 // Groups objects by step size (one time-node table and one launch sequence per class) and
-// sorts each class by descending step count so that the lanes of a wave end together.
+// orders each class by descending step count so that the lanes of a wave end together. With
+// key (the semi-major axis, orbit_size), objects whose last steps fall in the same `sub`-step
+// window of their chunks are ordered by key instead: a group of 64 objects runs each step's
+// corrector evaluations until its slowest object reaches the fixed point, and the count an object
+// needs rises as its orbit shrinks (a stronger gravity gradient), so groups of similar orbits run
+// fewer evaluations. The windows nest in the chunks (sub divides chunk), so the objects that
+// still run at a chunk start remain a prefix of the class.
 std::vector<ogp_context::Group> plan_groups(const double* step_s, const int64_t* steps, int64_t n,
-    std::vector<int64_t>& perm)
+    std::vector<int64_t>& perm, const float* key = nullptr, int64_t sub = 0)
 {
     perm.resize(n);
     std::iota(perm.begin(), perm.end(), 0);
-    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
-        if (step_s[a] != step_s[b])
-            return step_s[a] < step_s[b];
-        return steps[a] > steps[b];
-    });
+    if (key && sub > 0) {
+        // last-step window (GJ8 steps start at 5), -1 for objects without GJ8 steps
+        std::vector<int64_t> win(n);
+        for (int64_t i = 0; i < n; ++i)
+            win[i] = steps[i] >= 5 ? (steps[i] - 5) / sub : -1;
+        std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+            if (step_s[a] != step_s[b])
+                return step_s[a] < step_s[b];
+            if (win[a] != win[b])
+                return win[a] > win[b];
+            if (win[a] >= 0 && key[a] != key[b])
+                return key[a] < key[b];
+            return steps[a] > steps[b];
+        });
+    } else {
+        std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+            if (step_s[a] != step_s[b])
+                return step_s[a] < step_s[b];
+            return steps[a] > steps[b];
+        });
+    }
     std::vector<ogp_context::Group> groups;
     for (int64_t p = 0; p < n; ++p) {
         const int64_t i = perm[p];
@@ -1052,6 +1082,16 @@ std::vector<ogp_context::Group> plan_groups(const double* step_s, const int64_t*
         groups.back().max_steps = std::max(groups.back().max_steps, steps[i]);
     }
     return groups;
+}
+
+// Semi-major axis from the vis-viva equation (0 where it is not finite), the grouping key of
+// plan_groups.
+float orbit_size(const double* r, const double* v, double gm)
+{
+    const double rr = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double a = 1.0 / (2.0 / rr - v2 / gm);
+    return std::isfinite(a) ? (float)a : 0.0f;
 }
 
 void validate_prop(const double* step_s, const int64_t* steps, int64_t n)
@@ -1104,7 +1144,14 @@ void propagate_host(ogp_context* ctx, const double* r0, const double* v0, int64_
         check_finite(v0, 3 * n, "velocities");
     }
     std::vector<int64_t> perm;
-    auto groups = plan_groups(step_s, steps.data(), n, perm);
+    std::vector<float> key;
+    if (ctx->cfg.force == OGP_FORCE_GRAVITY) {
+        key.resize(n);
+        for (int64_t i = 0; i < n; ++i)
+            key[i] = orbit_size(r0 + 3 * i, v0 + 3 * i, ctx->grav.gm);
+    }
+    auto groups = plan_groups(step_s, steps.data(), n, perm, key.empty() ? nullptr : key.data(),
+        ctx->sort_window());
     std::vector<int64_t> ns(n);
     for (int64_t p = 0; p < n; ++p)
         ns[p] = steps[perm[p]];
@@ -1173,6 +1220,15 @@ void propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v0, 
     ctx->check_launch();
     uint32_t bad = 0;
     HIPC(hipMemcpyAsync(&bad, ctx->d_count.p, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<float> key;
+    if (ctx->cfg.force == OGP_FORCE_GRAVITY) {
+        key.resize(n);
+        ctx->d_key.reserve(sizeof(float) * n, &ctx->dev_bytes);
+        launch_orbit_key_soa(ctx->stream, d_r0, d_v0, n, ctx->grav.gm, ctx->d_key.as<float>());
+        ctx->check_launch();
+        HIPC(hipMemcpyAsync(key.data(), ctx->d_key.p, sizeof(float) * n, hipMemcpyDeviceToHost,
+            ctx->stream));
+    }
     ctx->sync();
     std::vector<int32_t> flags;
     if (bad > 0 || status) {
@@ -1191,7 +1247,8 @@ void propagate_device(ogp_context* ctx, const double* d_r0, const double* d_v0, 
                 steps[i] = 0;
     }
     std::vector<int64_t> perm;
-    auto groups = plan_groups(step_s, steps.data(), n, perm);
+    auto groups = plan_groups(step_s, steps.data(), n, perm, key.empty() ? nullptr : key.data(),
+        ctx->sort_window());
     std::vector<int64_t> ns(n);
     for (int64_t p = 0; p < n; ++p)
         ns[p] = steps[perm[p]];

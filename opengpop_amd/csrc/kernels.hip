@@ -535,6 +535,31 @@ void launch_finite_soa(hipStream_t st, const double* r, const double* v, int64_t
             r, v, n, flags, count, code, fresh ? 1 : 0);
 }
 
+// Semi-major axis a = 1 / (2 / |r| - |v|^2 / GM) of SoA states (vis-viva), 0 where it is not
+// finite: the grouping key of the propagation (orbit_size in capi.cpp).
+__global__ void orbit_key_soa_kernel(const double* r, const double* v, int64_t n, double gm,
+    float* key)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double r2 = 0.0, v2 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+        r2 += r[c * n + i] * r[c * n + i];
+        v2 += v[c * n + i] * v[c * n + i];
+    }
+    const double a = 1.0 / (2.0 / sqrt(r2) - v2 / gm);
+    key[i] = isfinite(a) ? (float)a : 0.0f;
+}
+
+void launch_orbit_key_soa(hipStream_t st, const double* r, const double* v, int64_t n, double gm,
+    float* key)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(orbit_key_soa_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+            st, r, v, n, gm, key);
+}
+
 void launch_soa_to_aos(hipStream_t st, const double* soa, double* aos, int64_t n)
 {
     if (n > 0)

@@ -114,6 +114,11 @@ void launch_slots_to_aos(hipStream_t st, const double* src, double* dst, int64_t
 void launch_finite_soa(hipStream_t st, const double* r, const double* v, int64_t n, int32_t* flags,
     uint32_t* count, int32_t code, bool fresh);
 
+// key[i] = semi-major axis (vis-viva, as float; 0 where not finite) of SoA states r, v (stride
+// n) for gravitational parameter gm.
+void launch_orbit_key_soa(hipStream_t st, const double* r, const double* v, int64_t n, double gm,
+    float* key);
+
 // Permutations between the caller's order and the internal (class-sorted) order.
 // mode 0: AoS src -> SoA dst gather (dst[p] = src[perm[p]]); 1: SoA -> AoS scatter;
 // 2: SoA -> SoA gather; 3: SoA -> SoA scatter.
