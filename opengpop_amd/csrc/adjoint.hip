@@ -6,9 +6,12 @@
 //    kernel runs on the variational-equation layout (kGravityTangent, device.hpp), giving the
 //    derivatives of the post-startup state Z = (s, S, a_{-4..4}) w.r.t. the six initial
 //    components;
-//  * the GJ8 steps are differentiated backward. When the gravity Jacobians of every corrector
-//    iterate of the span fit in HBM, the forward pass itself records them (gj8_record_kernel,
-//    bit-identical trajectory); otherwise the forward pass stores the state (s, S and the
+//  * the GJ8 steps are differentiated backward. Each step's gravity Jacobian is taken at the
+//    point of its last corrector evaluation (the corrector's fixed point; the earlier iterates
+//    lie within the predictor's error of it, where the gradient differs by ~1e-11 of itself) and
+//    used for every iterate of the step. When the Jacobians of the span fit in HBM, the forward
+//    pass itself records them (gj8_record_kernel, bit-identical trajectory); otherwise the
+//    forward pass stores the state (s, S and the
 //    acceleration ring, 33 doubles per object) every C steps and, per segment, last to first, a
 //    recording pass re-runs the steps from the checkpoint. The reverse pass carries the
 //    cotangents of (s, S, ring) back through each segment. Cotangents of Z times dZ/dX give the
@@ -18,7 +21,7 @@
 // acceleration at iterate r_it (it = 0 .. K-1), r_{it+1} = hh((T2 + gj88 e_it) + S'),
 // v_{it+1} = h((T1 + sa88 e_it) + s'_it), s'_it = s + (e_it + a_{k-1})/2, S' = S + s + a_{k-1}/2,
 // r_0 = hh(Pr + S'); the predicted velocity does not reach the gravity, so Pv carries no
-// cotangent. The cotangent of e_it is mapped to r_it by G(r_it), symmetric (a Hessian).
+// cotangent. The cotangent of e_it is mapped to r_it by the step's G, symmetric (a Hessian).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -114,10 +117,10 @@ __global__ void __launch_bounds__(64) gj8_tangent_kernel(const int64_t* __restri
         gj8_sums(ring, n, o, k, q);
         double sprev[3], anew[3];
         gj8_predict(q, h, s, S, r, v, sprev);
+        const double* gp = Gbuf + (((k - k0) * 6) * n + o);
+        const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
+                     gyz = gp[4 * n], gzz = gp[5 * n];
         for (int it = 0; it < ncorr; ++it) {
-            const double* gp = Gbuf + ((((k - k0) * ncorr + it) * 6) * n + o);
-            const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
-                         gyz = gp[4 * n], gzz = gp[5 * n];
             anew[0] = gxx * r[0] + gxy * r[1] + gxz * r[2];
             anew[1] = gxy * r[0] + gyy * r[1] + gyz * r[2];
             anew[2] = gxz * r[0] + gyz * r[1] + gzz * r[2];
@@ -207,20 +210,27 @@ __global__ void checkpoint_kernel(PropState ps, double* __restrict__ ck, int to_
     }
 }
 
-// GJ8 steps k0 .. k1-1 exactly as gj8_chunk_kernel<kGravity> (same arithmetic, gj8.hpp, and a
-// bit-identical acceleration), storing the GCRS gravity Jacobian of every corrector iterate:
-// Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + o], e = xx, xy, xz, yy, yz, zz. With
-// fixed_point_exit a step stops evaluating, as the chunk kernel does, once the next point repeats
-// the point just evaluated for every object that stores the step; the remaining iterates are at
-// that same point, and their Jacobians are the last one, stored again (Gbuf bit-identical).
+// GJ8 steps k0 .. k1-1 exactly as gj8_chunk_kernel<kGravity> (the same plain evaluations and
+// arithmetic, gj8.hpp, so the trajectory is bit-identical; with fixed_point_exit each step stops
+// at the corrector's fixed point as the chunk kernel does), then per step one Jacobian
+// evaluation (Dual3) at the point of the step's last evaluation, stored as the step's GCRS
+// gravity Jacobian: Gbuf[((k - k0) * 6 + e) * n + o], e = xx, xy, xz, yy, yz, zz. The
+// derivative sweeps use it for every corrector iterate of the step: the iterates converge to
+// that point (from the predictor's point, a few 1e-5 m away for LEO at 60 s), where the gravity
+// gradient changes by about 1e-11 of itself, so the derivatives are those of the scheme with each
+// step's Jacobian taken at its fixed point. The point, and so G, is the same with or without the
+// fixed-point exit (the last evaluation of all ncorr is at the fixed point).
 template <int W>
 __global__ void __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(2))) // at most 256 VGPRs + AGPRs: 2 resident waves per SIMD
 gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
     int64_t k1, int ncorr, double* __restrict__ Gbuf, int fixed_point_exit)
 {
+    // one exchange area for both kinds of evaluation (the Dual3 parts need the larger); pos holds
+    // the plain geometry (kPointLen per lane) or the ITRS position (3 per lane)
+    static_assert(kJacExchange >= kParts * 4 * 64, "the plain part sums fit the exchange");
     __shared__ double red[W > 1 ? kJacExchange : 1];
-    __shared__ double pos[W > 1 ? 3 * 64 : 1]; // the lead's ITRS positions
+    __shared__ double pos[W > 1 ? kPointLen * 64 : 1];
     __shared__ int more; // the lead's control word for its helpers
     // Every barrier sits in workgroup-uniform control flow: the loops run to the group's last
     // step (the same in every wave, which hold the same objects); a lane past its span, or past
@@ -236,17 +246,21 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         return;
     if constexpr (W > 1) {
         // The W waves of a workgroup hold the same 64 objects and share the harmonic columns.
-        // Waves 1 .. W-1 only sum their Dual3 parts (gravity_jac_helper_loop) for as many
-        // evaluations as wave 0 runs; wave 0 integrates, records the Jacobians and writes the ring
-        // and the state. Two barriers per evaluation, one more at the end.
+        // Waves 1 .. W-1 only sum their parts (gravity_record_helper_loop) for every evaluation
+        // wave 0 runs; wave 0 integrates, records the Jacobians and writes the ring and the state.
+        // Two barriers per evaluation, one more at the end.
         const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
         if (w > 0) {
-            gravity_jac_helper_loop<W>(f.grav, red, pos, &more, w);
+            gravity_record_helper_loop<W>(f.grav, red, pos, &more, w);
             return;
         }
-        if (threadIdx.x == 0)
-            more = 1;
     }
+    auto announce = [&](int what) {
+        if constexpr (W > 1) {
+            if (threadIdx.x == 0)
+                more = what;
+        }
+    };
     const double h = f.h, hh = h * h;
     double r[3], v[3], s[3], S[3];
     for (int c = 0; c < 3; ++c) {
@@ -275,35 +289,41 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         // for bit: the predictor's point rotated into the ITRS, then hh (M^T (T2 + S) + g88 g_ITRS)
         // after each evaluation (contexts with sources have no derivatives)
         const uniform_dptr M = (uniform_dptr)(f.prop_M + 9 * (k + 4));
-        double ri[3], base[3], t[3];
+        double ri[3], rl[3], base[3], t[3], gi[3];
         to_itrs(M, r, ri);
         for (int c = 0; c < 3; ++c)
             t[c] = q.T2[c] + S[c];
         to_itrs(M, t, base);
         for (int it = 0; it < ncorr; ++it) {
-            double gi[3], Gi[3][3], G[3][3];
+            announce(1);
             if constexpr (W > 1)
-                gravity_itrs_jac_lead<W>(ri, f.grav, red, pos, gi, Gi);
+                gravity_itrs_lead<false, W>(ri, f.grav, red, pos, gi);
             else
-                gravity_itrs_jac(ri[0], ri[1], ri[2], f.grav, gi, Gi);
-            jac_to_gcrs(M, gi, Gi, anew, G);
-            const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
-            if (act)
-                for (int e = 0; e < 6; ++e)
-                    Gbuf[(((k - k0) * ncorr + it) * 6 + e) * n + i] = g6[e];
+                gravity_itrs<false>(ri[0], ri[1], ri[2], f.grav, gi[0], gi[1], gi[2]);
             bool same = true;
             for (int c = 0; c < 3; ++c) {
                 const double rn = hh * __builtin_fma(c_gj[8][8], gi[c], base[c]);
                 same = same && __double_as_longlong(rn) == __double_as_longlong(ri[c]);
+                rl[c] = ri[c];
                 ri[c] = rn;
             }
-            if (fixed_point_exit && __ballot(!(same || !act)) == 0) {
-                // every later iterate is at this point: the same acceleration and Jacobian
-                if (act)
-                    for (int j = it + 1; j < ncorr; ++j)
-                        for (int e = 0; e < 6; ++e)
-                            Gbuf[(((k - k0) * ncorr + j) * 6 + e) * n + i] = g6[e];
-                break;
+            if (fixed_point_exit && __ballot(!(same || !act)) == 0)
+                break; // every later evaluation would repeat this one
+        }
+        to_gcrs(M, gi, anew);
+        {
+            // the step's Jacobian at the point of its last evaluation
+            double gj[3], Gi[3][3], aj[3], G[3][3];
+            announce(2);
+            if constexpr (W > 1)
+                gravity_itrs_jac_lead<W>(rl, f.grav, red, pos, gj, Gi);
+            else
+                gravity_itrs_jac(rl[0], rl[1], rl[2], f.grav, gj, Gi);
+            jac_to_gcrs(M, gj, Gi, aj, G);
+            if (act) {
+                const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
+                for (int e = 0; e < 6; ++e)
+                    Gbuf[((k - k0) * 6 + e) * n + i] = g6[e];
             }
         }
         gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
@@ -316,9 +336,8 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
             store_state();
     }
     if constexpr (W > 1) {
-        // release the helpers (gravity_jac_helper_loop)
-        if (threadIdx.x == 0)
-            more = 0;
+        // release the helpers (gravity_record_helper_loop)
+        announce(0);
         __syncthreads();
     }
     if (ns >= kend)
@@ -360,6 +379,9 @@ __global__ void __launch_bounds__(64) gj8_reverse_kernel(const int64_t* __restri
                 lS[c] = L[(3 + c) * n + i];
                 lT1[c] = lT2[c] = ls[c] = lal[c] = 0.0;
             }
+            const double* gp = Gbuf + (((k - k0) * 6) * n + i);
+            const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
+                         gyz = gp[4 * n], gzz = gp[5 * n];
             for (int it = ncorr - 1; it >= 0; --it) {
                 double le[3];
                 for (int c = 0; c < 3; ++c) {
@@ -376,9 +398,6 @@ __global__ void __launch_bounds__(64) gj8_reverse_kernel(const int64_t* __restri
                     le[c] += lsp[c] / 2;
                     lal[c] += lsp[c] / 2;
                 }
-                const double* gp = Gbuf + ((((k - k0) * ncorr + it) * 6) * n + i);
-                const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
-                             gyz = gp[4 * n], gzz = gp[5 * n];
                 lr[0] = gxx * le[0] + gxy * le[1] + gxz * le[2];
                 lr[1] = gxy * le[0] + gyy * le[1] + gyz * le[2];
                 lr[2] = gxz * le[0] + gyz * le[1] + gzz * le[2];

@@ -2610,14 +2610,14 @@ void adjoint_run(ogp_context* ctx, const double* r0, const double* v0, int64_t n
     const int64_t max_steps = *std::max_element(steps, steps + n);
     const int64_t gj_steps = std::max<int64_t>(max_steps - 4, 0);
     const bool record_all = C == 0
-        && (double)gj_steps * ncorr * 6 * (double)n * sizeof(double) <= ctx->record_bytes;
+        && (double)gj_steps * 6 * (double)n * sizeof(double) <= ctx->record_bytes;
     if (record_all)
         C = std::max<int64_t>(gj_steps, 1);
     else if (C == 0)
         C = 64;
     const int64_t nseg = max_steps >= 5 ? (max_steps - 4 + C - 1) / C : 0;
     const int W = ctx->waves_for(n);
-    ctx->ad_G.reserve(sizeof(double) * C * ncorr * 6 * n, &ctx->dev_bytes);
+    ctx->ad_G.reserve(sizeof(double) * C * 6 * n, &ctx->dev_bytes);
     if (record_all) {
         // forward pass = record pass over the whole span
         if (nseg > 0)
@@ -2709,9 +2709,9 @@ int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, dou
     return guarded(ctx, [&] {
         check_derivative_inputs(ctx, r0, v0, n, step_s, steps);
         arg_check(r_out && v_out && stm, "null output array");
-        // forward mode: each segment of GJ8 steps is recorded (the trajectory and the Jacobian
-        // of every corrector iterate, the propagation's arithmetic) and then the six tangent
-        // columns are carried through it; one segment when the whole span's Jacobians fit
+        // forward mode: each segment of GJ8 steps is recorded (the trajectory, the propagation's
+        // arithmetic, and each step's Jacobian at its corrector fixed point) and then the six
+        // tangent columns are carried through it; one segment when the whole span's Jacobians fit
         Event e0, e1;
         HIPC(hipEventRecord(e0.e, ctx->stream));
         ForceCtx f;
@@ -2719,11 +2719,11 @@ int ogp_stm(ogp_context* ctx, const double* r0, const double* v0, int64_t n, dou
         const int ncorr = ctx->cfg.n_corrections;
         const int64_t max_steps = *std::max_element(steps, steps + n);
         const int64_t gj_steps = std::max<int64_t>(max_steps - 4, 0);
-        const double per_step = (double)ncorr * 6 * (double)n * sizeof(double);
+        const double per_step = 6.0 * (double)n * sizeof(double);
         const int64_t C = std::max<int64_t>(1,
             std::min<int64_t>(gj_steps, (int64_t)(ctx->record_bytes / per_step)));
         const int W = ctx->waves_for(n);
-        ctx->ad_G.reserve(sizeof(double) * C * ncorr * 6 * n, &ctx->dev_bytes);
+        ctx->ad_G.reserve(sizeof(double) * C * 6 * n, &ctx->dev_bytes);
         for (int64_t k0 = 5; k0 <= max_steps; k0 += C) {
             const int64_t k1 = std::min<int64_t>(k0 + C, max_steps + 1);
             launch_gj8_record(ctx->stream, f, pc, k0, k1, ncorr, ctx->ad_G.as<double>(), W,

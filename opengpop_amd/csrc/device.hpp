@@ -703,10 +703,11 @@ __device__ __forceinline__ void gravity_gcrs_jac(const double* Mg, const double*
     jac_to_gcrs(M, gi, Gi, a, G);
 }
 
-// Lead/helper form of gravity_gcrs_jac<W> for W > 1 (the derivative record kernel), as
+// Lead/helper form of the Jacobian gravity for W > 1 (the derivative record kernel), as
 // gravity_gcrs_lead: wave 0 leaves the ITRS position in pos before the first barrier, every wave
-// sums its Dual3 parts there, and wave 0 alone folds, assembles and rotates. Bit-identical to
-// gravity_gcrs_jac<W>; the barrier hazards are those of gravity_gcrs_lead.
+// sums its Dual3 parts there (gravity_record_helper_loop for waves 1 .. W-1), and wave 0 alone
+// folds and assembles. Bit-identical to gravity_itrs_jac_split<W>; the barrier hazards are those
+// of gravity_gcrs_lead.
 //
 // gravity_itrs_jac_lead is the lead wave's part at the ITRS point ri (the position to pos, the
 // Dual3 parts, the fold and assembly): gi and its Jacobian Gi in the ITRS.
@@ -727,46 +728,38 @@ __device__ __forceinline__ void gravity_itrs_jac_lead(const double* ri, const Gr
     jac_fold_assemble(X, Y, Z, p, g, red, gi, Gi);
 }
 
+// The helpers' share of the derivative record pass, whose lead runs plain evaluations (the
+// trajectory) and one Jacobian evaluation per step: after each first barrier the control word
+// `more` says what the lead evaluates: 1 = the plain gravity at the geometry in pos
+// (gravity_itrs_lead, part sums to red), 2 = the Dual3 gravity at the ITRS position in pos
+// (gravity_itrs_jac_lead, Dual3 part sums to red), 0 = the end (after the lead's last barrier).
+// The lead writes `more` before the first barrier of each evaluation; the helpers read it
+// between that barrier and the second.
 template <int W>
-__device__ __forceinline__ void gravity_gcrs_jac_lead(const double* Mg, const double* r,
-    const GravityDev& g, double* red, double* pos, double* a, double (*G)[3])
-{
-    const uniform_dptr M = (uniform_dptr)Mg;
-    double ri[3], gi[3], Gi[3][3];
-    to_itrs(M, r, ri);
-    gravity_itrs_jac_lead<W>(ri, g, red, pos, gi, Gi);
-    jac_to_gcrs(M, gi, Gi, a, G);
-}
-
-// The part of one gravity_gcrs_jac_lead evaluation that wave w (1 <= w < W) runs.
-template <int W>
-__device__ __forceinline__ void gravity_jac_helper(const GravityDev& g, double* red,
-    const double* pos, int w)
-{
-    const int lane = threadIdx.x % 64;
-    __syncthreads();
-    const Dual3 X = dvar(pos[lane], 0), Y = dvar(pos[64 + lane], 1), Z = dvar(pos[128 + lane], 2);
-    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
-    jac_parts_to_lds<W>(p, g, red, w);
-    __syncthreads();
-}
-
-// gravity_jac_helper for as many evaluations as the lead runs, under the control word `more`
-// of gravity_helper_loop (1 before the lead's first evaluation, 0 and one barrier after its
-// last).
-template <int W>
-__device__ __forceinline__ void gravity_jac_helper_loop(const GravityDev& g, double* red,
+__device__ __forceinline__ void gravity_record_helper_loop(const GravityDev& g, double* red,
     const double* pos, const int* more, int w)
 {
     const int lane = threadIdx.x % 64;
     for (;;) {
         __syncthreads();
-        if (__builtin_amdgcn_readfirstlane(*more) == 0)
+        const int what = __builtin_amdgcn_readfirstlane(*more);
+        if (what == 0)
             return;
-        const Dual3 X = dvar(pos[lane], 0), Y = dvar(pos[64 + lane], 1),
-                    Z = dvar(pos[128 + lane], 2);
-        const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
-        jac_parts_to_lds<W>(p, g, red, w);
+        if (what == 1) {
+            const GravPoint p = get_point(pos, lane);
+            grav_parts<false>(p, g, wave_part<W>(w), wave_part<W>(w) + kParts / W,
+                [&](int q, const GravSums& s) {
+                    red[(q * 4 + 0) * 64 + lane] = s.G;
+                    red[(q * 4 + 1) * 64 + lane] = s.H;
+                    red[(q * 4 + 2) * 64 + lane] = s.J;
+                    red[(q * 4 + 3) * 64 + lane] = s.K;
+                });
+        } else {
+            const Dual3 X = dvar(pos[lane], 0), Y = dvar(pos[64 + lane], 1),
+                        Z = dvar(pos[128 + lane], 2);
+            const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
+            jac_parts_to_lds<W>(p, g, red, w);
+        }
         __syncthreads();
     }
 }

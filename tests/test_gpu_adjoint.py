@@ -106,7 +106,7 @@ def test_derivative_memory_budget_segments():
     steps = np.array([0, 1, 3, 4, 5, 6, 12, 13, 70, 133, 200, 69])
     with ogp.Context(degree=20) as c:
         r1, v1, P1 = c.stm(r0, v0, 60.0, steps)
-        c.set_derivative_memory(7 * 7 * 6 * 12 * 8)  # 7 steps of 7 iterates x 6 entries x 12
+        c.set_derivative_memory(7 * 6 * 12 * 8)  # 7 steps x 6 entries x 12 objects
         r2, v2, P2 = c.stm(r0, v0, 60.0, steps)
         _, _, Ja = c.adjoint_jacobian(r0, v0, 60.0, steps)
         _, _, J64 = c.adjoint_jacobian(r0, v0, 60.0, steps, checkpoint_steps=64)
