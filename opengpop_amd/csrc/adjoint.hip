@@ -3,9 +3,10 @@
 //
 // d(r, v)_final / d(r, v)_initial is assembled from two pieces:
 //  * the startup (RK8 + startup corrector, gpu_startup.cu) is differentiated forward: the startup
-//    kernel runs on the variational-equation layout (kGravityTangent, device.hpp), giving the
-//    derivatives of the post-startup state Z = (s, S, a_{-4..4}) w.r.t. the six initial
-//    components;
+//    of the objects runs with their gravity on dual numbers and records the Jacobian of every
+//    force evaluation (kGravityRecord, device.hpp); the startup kernel then runs again on six
+//    tangent lanes per object with those Jacobians (kRecordedTangent), giving the derivatives of
+//    the post-startup state Z = (s, S, a_{-4..4}) w.r.t. the six initial components;
 //  * the GJ8 steps are differentiated backward. Each step's gravity Jacobian is taken at the
 //    point of its last corrector evaluation (the corrector's fixed point; the earlier iterates
 //    lie within the predictor's error of it, where the gradient differs by ~1e-11 of itself) and
@@ -36,9 +37,6 @@ namespace {
 
 constexpr int kMaxCorr = 16;
 
-// Lane of the 7-lane layout for object o (tangent t = 0: the state itself).
-__device__ __forceinline__ int64_t tlane(int64_t o, int t) { return (o / 9) * 64 + (o % 9) * 7 + t; }
-
 } // namespace
 
 __global__ void gravity_jac_kernel(GravityDev g, const double* __restrict__ r, int64_t n,
@@ -56,28 +54,22 @@ __global__ void gravity_jac_kernel(GravityDev g, const double* __restrict__ r, i
     }
 }
 
-// Variational-equation layout: state lanes take (r0, v0), tangent lane t (1..6) the unit vector
-// of initial component t-1; unused lanes get no steps.
-__global__ void tangent_init_kernel(const double* __restrict__ r0, const double* __restrict__ v0,
-    const int64_t* __restrict__ steps, int64_t n, PropState ps, int64_t* __restrict__ ns_out)
+// Tangent lanes of the startup's derivative: lane col * n + o holds the derivative of object o's
+// state with respect to its initial component col (x, y, z, vx, vy, vz): it starts at that unit
+// vector and runs the startup with the recorded Jacobians of object o (kRecordedTangent).
+__global__ void tangent_init_kernel(const int64_t* __restrict__ steps, int64_t n, PropState pt,
+    int64_t* __restrict__ ns_out)
 {
     const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= ps.N)
+    if (L >= 6 * n)
         return;
-    const int l = (int)(L % 64);
-    const int64_t o = (L / 64) * 9 + l / 7;
-    const bool used = l < 63 && o < n;
-    const int t = l % 7;
+    const int col = (int)(L / n);
+    const int64_t o = L % n;
     for (int c = 0; c < 3; ++c) {
-        double rc = 0.0, vc = 0.0;
-        if (used) {
-            rc = t == 0 ? r0[3 * o + c] : (t - 1 == c ? 1.0 : 0.0);
-            vc = t == 0 ? v0[3 * o + c] : (t - 4 == c ? 1.0 : 0.0);
-        }
-        ps.r[c * ps.N + L] = rc;
-        ps.v[c * ps.N + L] = vc;
+        pt.r[c * pt.N + L] = col == c ? 1.0 : 0.0;
+        pt.v[c * pt.N + L] = col == 3 + c ? 1.0 : 0.0;
     }
-    ns_out[L] = used ? steps[o] : 0;
+    ns_out[L] = steps[o];
 }
 
 // Forward tangent pass over GJ8 steps k0 .. k1-1: the variational equations of the discrete
@@ -152,36 +144,25 @@ __global__ void stm_final_kernel(const int64_t* __restrict__ nsteps, int64_t n,
             stm[36 * o + 6 * e + col] = ns == 0 ? (e == col ? 1.0 : 0.0) : RVd[(col * 6 + e) * n + o];
 }
 
-// After the variational startup: the state lanes become the compact state (n objects) of the
-// forward pass; the tangent lanes give dZ/dX, Zd[(col * 33 + e) * n + o] with e = 0..2 s,
-// 3..5 S, 6 + 3 slot + c the ring; and d(r, v)/dX after the startup, RVd[(col * 6 + e) * n + o]
-// (the final derivative of spans of at most four steps).
-__global__ void tangent_split_kernel(PropState ps7, int64_t n, PropState pc,
-    int64_t* __restrict__ pc_ns, double* __restrict__ Zd, double* __restrict__ RVd)
+// After the tangent startup: the derivatives of the post-startup state of object o with respect
+// to initial component col, Zd[(col * 33 + e) * n + o] with e = 0..2 s, 3..5 S, 6 + 3 slot + c
+// the ring, and of (r, v), RVd[(col * 6 + e) * n + o] (the final derivative of spans of at most
+// four steps), from tangent lane col * n + o.
+__global__ void tangent_columns_kernel(PropState pt, int64_t n, double* __restrict__ Zd,
+    double* __restrict__ RVd)
 {
-    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= n)
+    const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= 6 * n)
         return;
-    const int64_t N7 = ps7.N, L0 = tlane(o, 0);
+    const int col = (int)(L / n);
+    const int64_t o = L % n, N = pt.N;
     for (int c = 0; c < 3; ++c) {
-        pc.r[c * n + o] = ps7.r[c * N7 + L0];
-        pc.v[c * n + o] = ps7.v[c * N7 + L0];
-        pc.s[c * n + o] = ps7.s[c * N7 + L0];
-        pc.S[c * n + o] = ps7.S[c * N7 + L0];
+        Zd[(col * 33 + c) * n + o] = pt.s[c * N + L];
+        Zd[(col * 33 + 3 + c) * n + o] = pt.S[c * N + L];
         for (int sl = 0; sl < 9; ++sl)
-            pc.hist[(sl * 3 + c) * n + o] = ps7.hist[(sl * 3 + c) * N7 + L0];
-    }
-    pc_ns[o] = ps7.nsteps[L0];
-    for (int col = 0; col < 6; ++col) {
-        const int64_t L = L0 + 1 + col;
-        for (int c = 0; c < 3; ++c) {
-            Zd[(col * 33 + c) * n + o] = ps7.s[c * N7 + L];
-            Zd[(col * 33 + 3 + c) * n + o] = ps7.S[c * N7 + L];
-            for (int sl = 0; sl < 9; ++sl)
-                Zd[(col * 33 + 6 + sl * 3 + c) * n + o] = ps7.hist[(sl * 3 + c) * N7 + L];
-            RVd[(col * 6 + c) * n + o] = ps7.r[c * N7 + L];
-            RVd[(col * 6 + 3 + c) * n + o] = ps7.v[c * N7 + L];
-        }
+            Zd[(col * 33 + 6 + sl * 3 + c) * n + o] = pt.hist[(sl * 3 + c) * N + L];
+        RVd[(col * 6 + c) * n + o] = pt.r[c * N + L];
+        RVd[(col * 6 + 3 + c) * n + o] = pt.v[c * N + L];
     }
 }
 
@@ -467,11 +448,12 @@ void launch_gravity_jac(hipStream_t st, const GravityDev& g, const double* r, in
             n, a, G);
 }
 
-void launch_tangent_init(hipStream_t st, const double* r0, const double* v0, const int64_t* steps,
-    int64_t n, const PropState& ps7, int64_t* ns7)
+void launch_tangent_init(hipStream_t st, const int64_t* steps, int64_t n, const PropState& pt,
+    int64_t* ns_out)
 {
-    hipLaunchKernelGGL(tangent_init_kernel, dim3((unsigned)((ps7.N + 255) / 256)), dim3(256), 0, st,
-        r0, v0, steps, n, ps7, ns7);
+    if (n > 0)
+        hipLaunchKernelGGL(tangent_init_kernel, dim3((unsigned)((6 * n + 255) / 256)), dim3(256), 0,
+            st, steps, n, pt, ns_out);
 }
 
 void launch_gj8_tangent(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
@@ -490,12 +472,12 @@ void launch_stm_final(hipStream_t st, const int64_t* nsteps, int64_t n, const do
             nsteps, n, RVd, stm);
 }
 
-void launch_tangent_split(hipStream_t st, const PropState& ps7, int64_t n, const PropState& pc,
-    int64_t* pc_ns, double* Zd, double* RVd)
+void launch_tangent_columns(hipStream_t st, const PropState& pt, int64_t n, double* Zd,
+    double* RVd)
 {
     if (n > 0)
-        hipLaunchKernelGGL(tangent_split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-            ps7, n, pc, pc_ns, Zd, RVd);
+        hipLaunchKernelGGL(tangent_columns_kernel, dim3((unsigned)((6 * n + 255) / 256)), dim3(256),
+            0, st, pt, n, Zd, RVd);
 }
 
 void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_checkpoint)

@@ -151,7 +151,7 @@ struct ogp_context {
     ogp_conj_timing cj_timing {};
 
     // derivatives
-    DevBuf ad_in, ad_steps, ad_r, ad_v, ad_s, ad_S, ad_hist, ad_ns, ad_Zd, ad_RVd, ad_ck, ad_G,
+    DevBuf ad_in, ad_r, ad_v, ad_s, ad_S, ad_hist, ad_ns, ad_Zd, ad_RVd, ad_ck, ad_G, ad_grec,
         ad_lam, ad_seeds, ad_out;
     ogp_adjoint_timing ad_timing {};
 
@@ -2512,50 +2512,22 @@ void check_derivative_inputs(ogp_context* ctx, const double* r0, const double* v
     check_finite(v0, 3 * n, "velocities");
 }
 
-// Uploads r0, v0 (AoS) and the step counts, and runs the startup on the variational-equation
-// layout (64 ceil(n / 9) lanes). Returns the layout's state; its lane count is ps7.N.
-PropState tangent_startup(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
-    double h, const int64_t* steps, NodeTable** nt_out, ForceCtx* f_out)
-{
-    const int64_t N7 = 64 * ((n + 8) / 9);
-    const int64_t max_steps = *std::max_element(steps, steps + n);
-    ctx->reserve_state(N7);
-    ctx->ad_in.reserve(sizeof(double) * 6 * n, &ctx->dev_bytes);
-    ctx->ad_steps.reserve(sizeof(int64_t) * n, &ctx->dev_bytes);
-    double* d_r0 = ctx->ad_in.as<double>();
-    double* d_v0 = d_r0 + 3 * n;
-    HIPC(hipMemcpyAsync(d_r0, r0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
-    HIPC(hipMemcpyAsync(d_v0, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
-    HIPC(hipMemcpyAsync(ctx->ad_steps.p, steps, sizeof(int64_t) * n, hipMemcpyHostToDevice,
-        ctx->stream));
-    const PropState ps7 = ctx->prop_state(N7);
-    launch_tangent_init(ctx->stream, d_r0, d_v0, ctx->ad_steps.as<int64_t>(), n, ps7,
-        ctx->st_ns.as<int64_t>());
-    ctx->check_launch();
-    NodeTable* nt = ctx->node_table(h, max_steps, 0.0);
-    const ForceCtx f = ctx->force_ctx(nt, h);
-    launch_startup(ctx->stream, kGravityTangent, f, ctx->rk8_variant, ps7, ctx->scratch(), 0, N7,
-        ctx->cfg.n_corrections, nullptr, 0, 1, ctx->fp_exit);
-    ctx->check_launch();
-    *nt_out = nt;
-    *f_out = f;
-    return ps7;
-}
-
 // Recorded gravity Jacobians of a whole span up to ctx->record_bytes are kept in HBM and the
 // record pass is the forward pass; beyond it (or with an explicit checkpoint interval) the
 // forward pass keeps checkpoints and every segment is recorded again in the reverse sweep. The
 // forward-mode STM records and consumes segments of at most that size.
 
-// Startup on the variational layout, then its split into the compact state pc (n objects,
-// ctx->ad_* buffers) and the derivatives of the post-startup state: Zd (s, S, ring) and RVd
-// (r, v), see launch_tangent_split.
+// The startup of the objects (the compact state pc, n objects, ctx->ad_* buffers) with the
+// Jacobian of every force evaluation recorded, then the same startup on six tangent lanes per
+// object (the st_* buffers) driven by those Jacobians, and their derivatives of the post-startup
+// state: Zd (s, S, ring) and RVd (r, v), see launch_tangent_columns.
 PropState derivative_start(ogp_context* ctx, const double* r0, const double* v0, int64_t n,
     double h, const int64_t* steps, ForceCtx* f)
 {
-    NodeTable* nt = nullptr;
-    const PropState ps7 = tangent_startup(ctx, r0, v0, n, h, steps, &nt, f);
+    const int64_t max_steps = *std::max_element(steps, steps + n);
+    const int ncorr = ctx->cfg.n_corrections;
     const size_t v3 = sizeof(double) * 3 * n;
+    ctx->ad_in.reserve(2 * v3, &ctx->dev_bytes);
     ctx->ad_r.reserve(v3, &ctx->dev_bytes);
     ctx->ad_v.reserve(v3, &ctx->dev_bytes);
     ctx->ad_s.reserve(v3, &ctx->dev_bytes);
@@ -2564,6 +2536,8 @@ PropState derivative_start(ogp_context* ctx, const double* r0, const double* v0,
     ctx->ad_ns.reserve(sizeof(int64_t) * n, &ctx->dev_bytes);
     ctx->ad_Zd.reserve(sizeof(double) * 6 * 33 * n, &ctx->dev_bytes);
     ctx->ad_RVd.reserve(sizeof(double) * 36 * n, &ctx->dev_bytes);
+    const int64_t slots = 80 + 9 * (ncorr + 1); // the startup's force evaluations
+    ctx->ad_grec.reserve(sizeof(double) * 6 * slots * n, &ctx->dev_bytes);
     PropState pc;
     pc.r = ctx->ad_r.as<double>();
     pc.v = ctx->ad_v.as<double>();
@@ -2572,8 +2546,29 @@ PropState derivative_start(ogp_context* ctx, const double* r0, const double* v0,
     pc.hist = ctx->ad_hist.as<double>();
     pc.nsteps = ctx->ad_ns.as<int64_t>();
     pc.N = n;
-    launch_tangent_split(ctx->stream, ps7, n, pc, ctx->ad_ns.as<int64_t>(),
-        ctx->ad_Zd.as<double>(), ctx->ad_RVd.as<double>());
+    double* d_in = ctx->ad_in.as<double>();
+    HIPC(hipMemcpyAsync(d_in, r0, v3, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(d_in + 3 * n, v0, v3, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->ad_ns.p, steps, sizeof(int64_t) * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    launch_aos_to_soa(ctx->stream, d_in, pc.r, n);
+    launch_aos_to_soa(ctx->stream, d_in + 3 * n, pc.v, n);
+    NodeTable* nt = ctx->node_table(h, max_steps, 0.0);
+    *f = ctx->force_ctx(nt, h);
+    ForceCtx fr = *f;
+    fr.grec = ctx->ad_grec.as<double>();
+    fr.grec_n = n;
+    // the tangent lanes and the startup work areas for them (the recording startup needs n)
+    ctx->reserve_state(6 * n);
+    launch_startup(ctx->stream, kGravityRecord, fr, ctx->rk8_variant, pc, ctx->scratch(), 0, n,
+        ncorr, nullptr, 0, ctx->waves_for(n), ctx->fp_exit);
+    ctx->check_launch();
+    const PropState pt = ctx->prop_state(6 * n);
+    launch_tangent_init(ctx->stream, pc.nsteps, n, pt, ctx->st_ns.as<int64_t>());
+    // every correction (the derivative of all ncorr, whose values the fixed-point exit repeats)
+    launch_startup(ctx->stream, kRecordedTangent, fr, ctx->rk8_variant, pt, ctx->scratch(), 0,
+        6 * n, ncorr, nullptr, 0, 1, false);
+    launch_tangent_columns(ctx->stream, pt, n, ctx->ad_Zd.as<double>(), ctx->ad_RVd.as<double>());
     ctx->check_launch();
     return pc;
 }

@@ -688,21 +688,6 @@ __device__ __forceinline__ void jac_to_gcrs(const uniform_dptr M, const double* 
             G[i][j] = M[i] * T[0][j] + M[3 + i] * T[1][j] + M[6 + i] * T[2][j];
 }
 
-// Acceleration and Jacobian in the GCRS: a = M g(M^T r), G = M Gi M^T (Gi the ITRS Jacobian).
-// The acceleration is bit-identical to gravity_gcrs<false>.
-template <int W = 1>
-__device__ __forceinline__ void gravity_gcrs_jac(const double* Mg, const double* r,
-    const GravityDev& g, double* a, double (*G)[3], double* red = nullptr)
-{
-    const uniform_dptr M = (uniform_dptr)Mg;
-    double ri[3], gi[3], Gi[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        ri[i] = __builtin_fma(M[i * 3 + 2], r[2], __builtin_fma(M[i * 3 + 1], r[1], M[i * 3] * r[0]));
-    gravity_itrs_jac_split<W>(ri[0], ri[1], ri[2], g, red, gi, Gi);
-    jac_to_gcrs(M, gi, Gi, a, G);
-}
-
 // Lead/helper form of the Jacobian gravity for W > 1 (the derivative record kernel), as
 // gravity_gcrs_lead: wave 0 leaves the ITRS position in pos before the first barrier, every wave
 // sums its Dual3 parts there (gravity_record_helper_loop for waves 1 .. W-1), and wave 0 alone
@@ -764,34 +749,33 @@ __device__ __forceinline__ void gravity_record_helper_loop(const GravityDev& g, 
     }
 }
 
-// Force of the variational-equation lanes (kGravityTangent) at node matrix Mg.
-__device__ __forceinline__ void tangent_force(const double* Mg, const double* x,
-    const GravityDev& g, double* out)
-{
-    const int lane = threadIdx.x % 64;
-    const int src = lane == 63 ? 63 : lane - lane % 7;
-    double rp[3], a[3], G[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        rp[c] = __shfl(x[c], src, 64);
-    gravity_gcrs_jac(Mg, rp, g, a, G);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        out[i] = lane == src ? a[i] : G[i][0] * x[0] + G[i][1] * x[1] + G[i][2] * x[2];
-}
-
 // ------------------------------------------------------------------------------------------
 // Analytic test forces (unit_tests/include/gpu_test_helpers.cuh) and their time mapping
 // (unit_tests/include/mock_force_model.cuh:69-73 and :99)
 // ------------------------------------------------------------------------------------------
-// kGravityTangent: the variational equations. Lanes 7g .. 7g+6 of a wave hold one object: lane
-// 7g its state, lanes 7g+1 .. 7g+6 six tangent states (derivatives of the state with respect to
-// the initial x, y, z, vx, vy, vz). The state lane receives the acceleration g(r); a tangent lane
-// receives G(r) dr, with G = dg/dr at the state lane's position. The integrator is linear in
-// states and accelerations, so the same startup and GJ8 code integrates the tangents exactly as
-// the discrete scheme's derivative (lane 63 is unused).
+// The startup's derivative (the variational equations of the RK8 startup and its corrector):
+//  * kGravityRecord runs the startup of the objects themselves with the gravity evaluated on
+//    dual numbers (its value part bit-identical to kGravity) and records, for every force
+//    evaluation, the GCRS Jacobian G = dg/dr (6 unique entries) at the evaluation's slot
+//    (grec_slot_gen / grec_slot_prop);
+//  * kRecordedTangent runs the same startup code on tangent states (the derivatives of the
+//    state with respect to one initial component each), whose acceleration at an evaluation is
+//    G dr with the G recorded for that evaluation of their object. The integrator is linear in
+//    states and accelerations, so this is the discrete startup's exact derivative.
 enum ForceKind { kGravity = 0, kNone = 1, kConst = 2, kPoly7 = 3, kPoly8 = 4, kPoly9 = 5,
-    kExp = 6, kDamped = 7, kGravitySplit = 100, kGravityTangent = 101 };
+    kExp = 6, kDamped = 7, kGravitySplit = 100, kGravityRecord = 102, kRecordedTangent = 103 };
+
+// Slots of the recorded startup Jacobians: the 80 RK8 stage evaluations, then acceleration
+// matrix m (0 = the epoch matrix, 1 .. ncorr = the corrections') at node j.
+constexpr int kGenSlots = 80;
+__device__ __forceinline__ int64_t grec_slot_gen(int s, int k)
+{
+    return ((s > 0 ? s - 1 : s) + 4) * 10 + k;
+}
+__device__ __forceinline__ int64_t grec_slot_prop(int m, int64_t node)
+{
+    return kGenSlots + 9 * m + node;
+}
 
 template <int KIND>
 __device__ __forceinline__ void analytic_force(const double* p, const double* v, double t, double* a)
@@ -926,24 +910,75 @@ struct ForceCtx {
     const double* prop_tb;
     SourcesDev src;
     double h;
+    // recorded startup Jacobians (kGravityRecord writes, kRecordedTangent reads): entry e of slot
+    // q of object o at grec[(q * 6 + e) * grec_n + o]; object_ctx points grec at the object
+    double* grec = nullptr;
+    int64_t grec_n = 1;
 };
 
 // The force context of object i (index into the state): drag with that object's coefficient
-// when per-object coefficients are given.
+// when per-object coefficients are given; the recorded Jacobians of object i mod grec_n (the
+// tangent lanes of object o are o, o + n, ..., o + 5n).
 __device__ __forceinline__ ForceCtx object_ctx(const ForceCtx& f, int64_t i)
 {
     ForceCtx o = f;
     if (f.src.bc_obj)
         o.src.drag_bc = f.src.bc_obj[i];
+    if (f.grec)
+        o.grec = f.grec + i % f.grec_n;
     return o;
 }
 
 // Per-workgroup exchange areas of the column-split evaluation (unused when W == 1): the part
-// sums, and the lead's geometry of the evaluation point (lead/helper form).
+// sums, and the lead's geometry of the evaluation point (lead/helper form). Per lane: mat counts
+// the startup's acceleration matrices begun (the recorded Jacobians' slots), on says whether the
+// lane records (its object runs).
 struct Exchange {
     double* red;
     double* pos = nullptr;
+    int mat = 0;
+    bool on = true;
 };
+
+// Entry e of recorded slot q of the object (ForceCtx::grec after object_ctx).
+__device__ __forceinline__ double& grec_at(const ForceCtx& f, int64_t q, int e)
+{
+    return f.grec[(q * 6 + e) * f.grec_n];
+}
+
+// kGravityRecord: the gravity at the GCRS position x for node matrix Mg on dual numbers (W > 1:
+// the lead's part of a lead/helper evaluation, whose helpers run gravity_record_helper_loop),
+// the acceleration into a (bit-identical to gravity_gcrs<false, W>) and G into slot q.
+template <int W>
+__device__ __forceinline__ void record_gravity(const ForceCtx& f, const double* Mg,
+    const double* x, int64_t q, double* a, Exchange& ex)
+{
+    const uniform_dptr M = (uniform_dptr)Mg;
+    double ri[3], gi[3], Gi[3][3], G[3][3];
+    to_itrs(M, x, ri);
+    if constexpr (W > 1)
+        gravity_itrs_jac_lead<W>(ri, f.grav, ex.red, ex.pos, gi, Gi);
+    else
+        gravity_itrs_jac(ri[0], ri[1], ri[2], f.grav, gi, Gi);
+    jac_to_gcrs(M, gi, Gi, a, G);
+    if (ex.on) {
+        const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
+#pragma unroll
+        for (int e = 0; e < 6; ++e)
+            grec_at(f, q, e) = g6[e];
+    }
+}
+
+// kRecordedTangent: a = G dx with the G of slot q (symmetric, 6 stored entries).
+__device__ __forceinline__ void recorded_tangent(const ForceCtx& f, int64_t q, const double* x,
+    double* a)
+{
+    const double gxx = grec_at(f, q, 0), gxy = grec_at(f, q, 1), gxz = grec_at(f, q, 2),
+                 gyy = grec_at(f, q, 3), gyz = grec_at(f, q, 4), gzz = grec_at(f, q, 5);
+    a[0] = gxx * x[0] + gxy * x[1] + gxz * x[2];
+    a[1] = gxy * x[0] + gyy * x[1] + gyz * x[2];
+    a[2] = gxz * x[0] + gyz * x[1] + gzz * x[2];
+}
 
 // ForceModel::ComputeGenerationAcceleration (force_model.cu:59-89): RK8 stage k of startup
 // step s (s = +-1..4).
@@ -955,9 +990,11 @@ __device__ __forceinline__ void force_gen(const ForceCtx& f, const double* x, co
         const int idx = ((s > 0 ? s - 1 : s) + 4) * 10 + k;
         gravity_gcrs<KIND == kGravitySplit, W>(f.gen_M + 9 * idx, x, f.grav, a, ex.red);
         add_sources(f.src, f.gen_tb + 6 * idx, x, v, a);
-    } else if (KIND == kGravityTangent) {
+    } else if (KIND == kGravityRecord) {
         const int idx = ((s > 0 ? s - 1 : s) + 4) * 10 + k;
-        tangent_force(f.gen_M + 9 * idx, x, f.grav, a);
+        record_gravity<W>(f, f.gen_M + 9 * idx, x, grec_slot_gen(s, k), a, ex);
+    } else if (KIND == kRecordedTangent) {
+        recorded_tangent(f, grec_slot_gen(s, k), x, a);
     } else {
         const int dir = s > 0 ? -1 : 1;
         const double t = (double)(s + dir) * f.h + (double)(-dir) * c_rk8c[k] * f.h;
@@ -973,8 +1010,10 @@ __device__ __forceinline__ void force_prop(const ForceCtx& f, const double* x, c
     if (KIND == kGravity || KIND == kGravitySplit) {
         gravity_gcrs<KIND == kGravitySplit, W>(f.prop_M + 9 * node, x, f.grav, a, ex.red);
         add_sources(f.src, f.prop_tb + 6 * node, x, v, a);
-    } else if (KIND == kGravityTangent) {
-        tangent_force(f.prop_M + 9 * node, x, f.grav, a);
+    } else if (KIND == kGravityRecord) {
+        record_gravity<W>(f, f.prop_M + 9 * node, x, grec_slot_prop(ex.mat - 1, node), a, ex);
+    } else if (KIND == kRecordedTangent) {
+        recorded_tangent(f, grec_slot_prop(ex.mat - 1, node), x, a);
     } else {
         const double t = (double)(node - 4) * f.h;
         analytic_force<KIND>(x, v, t, a);

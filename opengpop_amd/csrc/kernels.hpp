@@ -129,15 +129,15 @@ void launch_permute(hipStream_t st, int mode, const double* src, const int64_t* 
 void launch_aos_to_soa(hipStream_t st, const double* aos, double* soa, int64_t n);
 void launch_soa_to_aos(hipStream_t st, const double* soa, double* aos, int64_t n);
 
-// Derivatives (adjoint.hip). The variational-equation layout puts object o in lanes
-// 64 (o / 9) + 7 (o % 9) + t, t = 0 the state and t = 1..6 its tangents; it needs
-// 64 ceil(n / 9) lanes.
+// Derivatives (adjoint.hip). The startup's derivative runs on 6 tangent lanes per object (lane
+// col * n + o: the derivative of object o's state with respect to initial component col), with
+// the Jacobians a kGravityRecord startup of the objects recorded (kRecordedTangent).
 void launch_gravity_jac(hipStream_t st, const GravityDev& g, const double* r, int64_t n,
     double* a, double* G);
-void launch_tangent_init(hipStream_t st, const double* r0, const double* v0, const int64_t* steps,
-    int64_t n, const PropState& ps7, int64_t* ns7);
-void launch_tangent_split(hipStream_t st, const PropState& ps7, int64_t n, const PropState& pc,
-    int64_t* pc_ns, double* Zd, double* RVd);
+void launch_tangent_init(hipStream_t st, const int64_t* steps, int64_t n, const PropState& pt,
+    int64_t* ns_out);
+void launch_tangent_columns(hipStream_t st, const PropState& pt, int64_t n, double* Zd,
+    double* RVd);
 void launch_checkpoint(hipStream_t st, const PropState& ps, double* ck, bool to_checkpoint);
 // Steps k0 .. k1-1 of the plain state ps with the gravity Jacobian of every corrector iterate
 // stored (6 unique entries); the state is advanced as by the chunk kernel. `waves` (4 below 8,
@@ -150,7 +150,7 @@ void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double
 void launch_adjoint_combine(hipStream_t st, const int64_t* nsteps, int64_t n, int R,
     const double* lam, const double* Zd, const double* RVd, const double* seeds, double* out);
 // Forward tangent pass over steps k0 .. k1-1 with the Jacobians recorded for those steps:
-// Zd / RVd (tangent_split layout) are advanced in place.
+// Zd / RVd (tangent_columns layout) are advanced in place.
 void launch_gj8_tangent(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,
     int64_t k1, int ncorr, const double* Gbuf, double* Zd, double* RVd);
 void launch_stm_final(hipStream_t st, const int64_t* nsteps, int64_t n, const double* RVd,

@@ -126,6 +126,7 @@ template <int KIND, int W>
 __device__ void acc_matrix(const ForceCtx& f, const Slots& nr, const Slots& nv, const Slots& acc,
     bool epoch, Exchange& ex)
 {
+    ++ex.mat; // matrix ex.mat - 1 (recorded Jacobians' slots)
 #pragma unroll 1
     for (int j = 0; j < 9; ++j) {
         if (j == 4 && !epoch)
@@ -222,7 +223,9 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     StartupScratch sc, int64_t obj0, int64_t nobj, int ncorr, double* traj, int64_t traj_steps,
     int fixed_point_exit)
 {
-    __shared__ double red[ExchangeArea<W>::size];
+    constexpr bool kLead = W > 1 && (KIND == kGravity || KIND == kGravitySplit);
+    constexpr bool kRecLead = W > 1 && KIND == kGravityRecord; // Dual3 parts, 64 KB exchange
+    __shared__ double red[kRecLead ? kJacExchange : ExchangeArea<W>::size];
     __shared__ double pos[W > 1 ? kPointLen * 64 : 1]; // the lead's geometry (gravity_gcrs_lead)
     __shared__ int more; // the lead's control word for its helpers (gravity_helper_loop)
     Exchange ex { red, pos };
@@ -234,16 +237,20 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     if (__builtin_amdgcn_readfirstlane((int)std::min<int64_t>(wave_max(ns), 1)) <= 0)
         return; // no lane of the group runs (the same in every wave of the workgroup)
     const bool act = ns > 0;
-    if constexpr (W > 1 && (KIND == kGravity || KIND == kGravitySplit)) {
+    ex.on = act;
+    if constexpr (kLead || kRecLead) {
         const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
         if (w > 0) {
             // one part sum per force evaluation of the lead: 8 RK8 steps of 10 stages, the
             // acceleration matrix (9 nodes) and up to ncorr corrections of 8 nodes each
-            gravity_helper_loop<KIND == kGravitySplit, W>(f.grav, red, pos, &more, w);
+            if constexpr (kRecLead)
+                gravity_record_helper_loop<W>(f.grav, red, pos, &more, w);
+            else
+                gravity_helper_loop<KIND == kGravitySplit, W>(f.grav, red, pos, &more, w);
             return;
         }
         if (threadIdx.x == 0)
-            more = 1;
+            more = kRecLead ? 2 : 1; // every recording evaluation is a Dual3 evaluation
     }
     const Slots nr { sc.node_r, N, i, act }, nv { sc.node_v, N, i, act }, acc { sc.acc, N, i, act };
     const Slots sm { sc.sm, N, i, act }, Sm { sc.Sm, N, i, act }, Kx { sc.Kx, N, i, act },
@@ -265,11 +272,23 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     for (int it = 0; it < ncorr; ++it)
         if (startup_correct<KIND, W>(f, h, nr, nv, acc, sm, Sm, ex, fixed_point_exit != 0))
             break;
-    if constexpr (W > 1 && (KIND == kGravity || KIND == kGravitySplit)) {
+    if constexpr (kLead || kRecLead) {
         // release the helpers (gravity_helper_loop), in wave-uniform control flow
         if (threadIdx.x == 0)
             more = 0;
         __syncthreads();
+    }
+    if constexpr (KIND == kGravityRecord) {
+        // matrices skipped at the corrections' fixed point are at the last matrix's nodes: the
+        // same Jacobians (the tangent startup runs every correction)
+        if (act)
+            for (int m = ex.mat; m <= ncorr; ++m)
+                for (int j = 0; j < 9; ++j)
+                    if (j != 4)
+#pragma unroll
+                        for (int e = 0; e < 6; ++e)
+                            grec_at(f, grec_slot_prop(m, j), e) =
+                                grec_at(f, grec_slot_prop(ex.mat - 1, j), e);
     }
     if (!act)
         return;
@@ -504,8 +523,6 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
             case 8: if (split_) LAUNCH(kGravitySplit, 8); else LAUNCH(kGravity, 8); break;         \
             default: if (split_) LAUNCH(kGravitySplit, 1); else LAUNCH(kGravity, 1); break;        \
             }                                                                                      \
-        } else if (kind == kGravityTangent) {                                                      \
-            LAUNCH(kGravityTangent, 1);                                                            \
         } else {                                                                                   \
             switch (kind) {                                                                        \
             case kNone: LAUNCH(kNone, 1); break;                                                   \
@@ -538,7 +555,17 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
 #define L_(K, Wv)                                                                                  \
     hipLaunchKernelGGL((startup_kernel<K, Wv>), grid, dim3(64 * (Wv)), 0, st, f, rk8_variant, ps,   \
         sc, obj0, nobj, ncorr, traj, traj_steps, fixed_point_exit ? 1 : 0)
-    OGP_DISPATCH(kind, W_, L_);
+    if (kind == kGravityRecord) {
+        // 4 or 8 waves: the 64 KB Dual3 exchange leaves a 2-wave shape one wave per SIMD
+        if (waves >= 8)
+            L_(kGravityRecord, 8);
+        else
+            L_(kGravityRecord, 4);
+    } else if (kind == kRecordedTangent) {
+        L_(kRecordedTangent, 1);
+    } else {
+        OGP_DISPATCH(kind, W_, L_);
+    }
 #undef L_
 }
 
