@@ -3,7 +3,7 @@
 //
 // d(r, v)_final / d(r, v)_initial is assembled from two pieces:
 //  * the startup (RK8 + startup corrector, gpu_startup.cu) is differentiated forward: the startup
-//    of the objects runs with their gravity on dual numbers and records the Jacobian of every
+//    of the objects runs with the gravity gradient evaluation and records the Jacobian of every
 //    force evaluation (kGravityRecord, device.hpp); the startup kernel then runs again on six
 //    tangent lanes per object with those Jacobians (kRecordedTangent), giving the derivatives of
 //    the post-startup state Z = (s, S, a_{-4..4}) w.r.t. the six initial components;
@@ -39,7 +39,9 @@ constexpr int kMaxCorr = 16;
 
 } // namespace
 
-__global__ void gravity_jac_kernel(GravityDev g, const double* __restrict__ r, int64_t n,
+// One object per lane, all parts (gravity_itrs_jac); 256 VGPRs, where 128 spill the fold.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+gravity_jac_kernel(GravityDev g, const double* __restrict__ r, int64_t n,
     double* __restrict__ a, double* __restrict__ G)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,6 +54,31 @@ __global__ void gravity_jac_kernel(GravityDev g, const double* __restrict__ r, i
         for (int j = 0; j < 3; ++j)
             G[9 * i + 3 * c + j] = Gi[c][j];
     }
+}
+
+// The recorded steps' gravity Jacobians (after gj8_record_kernel): slot (k - k0, o) of Gbuf
+// holds the ITRS point of object o's last evaluation of step k in its first three entries; it is
+// replaced by the GCRS Jacobian G = M Gi M^T (6 unique entries), M the node's ITRS->GCRS matrix.
+// One lane per (object, step): block (x, y) = (object group x, step k0 + y), so M is
+// wave-uniform. Slots of steps past an object's span hold nothing and are skipped (the sweeps do
+// not read them).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+gravity_gradient_kernel(GravityDev g, const double* __restrict__ prop_M,
+    const int64_t* __restrict__ nsteps, int64_t n, int64_t k0, double* __restrict__ Gbuf)
+{
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t k = k0 + blockIdx.y;
+    if (i >= n || k > nsteps[i])
+        return;
+    double* slot = Gbuf + (int64_t)blockIdx.y * 6 * n + i;
+    const double ri[3] = { slot[0], slot[n], slot[2 * n] };
+    const uniform_dptr M = (uniform_dptr)(prop_M + 9 * (k + 4));
+    double gi[3], Gi[3][3], a[3], G[3][3];
+    gravity_itrs_jac(ri[0], ri[1], ri[2], g, gi, Gi);
+    jac_to_gcrs(M, gi, Gi, a, G);
+    const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
+    for (int e = 0; e < 6; ++e)
+        slot[e * n] = g6[e];
 }
 
 // Tangent lanes of the startup's derivative: lane col * n + o holds the derivative of object o's
@@ -91,6 +118,7 @@ __global__ void __launch_bounds__(64) gj8_tangent_kernel(const int64_t* __restri
     const int64_t ns = nsteps[o];
     if (ns < k0)
         return;
+    const int64_t klast = std::min<int64_t>(k1 - 1, ns);
     double* Z = Zd + (int64_t)col * 33 * n;
     double* RV = RVd + (int64_t)col * 6 * n;
     double* ring = Z + 6 * n;
@@ -102,26 +130,45 @@ __global__ void __launch_bounds__(64) gj8_tangent_kernel(const int64_t* __restri
         r[c] = RV[c * n + o];
         v[c] = RV[(3 + c) * n + o];
     }
-    for (int64_t k = k0; k < k1; ++k) {
-        if (k > ns)
-            break;
+    // the ring in registers for the whole segment: A[j] = a_{k-9+j} when step k begins
+    double A[9][3];
+    for (int j = 0; j < 9; ++j)
+        for (int c = 0; c < 3; ++c)
+            A[j][c] = ring[(ring_slot(k0 - 9 + j) * 3 + c) * n + o];
+    // the step's Jacobian, loaded one step ahead
+    auto load_g = [&](int64_t k, double* g) {
+        const double* gp = Gbuf + (((k - k0) * 6) * n + o);
+        for (int e = 0; e < 6; ++e)
+            g[e] = gp[e * n];
+    };
+    double g[6], gn[6];
+    load_g(k0, g);
+    for (int64_t k = k0; k <= klast; ++k) {
+        if (k < klast)
+            load_g(k + 1, gn);
         GJ8Sums q;
-        gj8_sums(ring, n, o, k, q);
+        gj8_sums_of([&](int j, int c) { return A[j][c]; }, q);
         double sprev[3], anew[3];
         gj8_predict(q, h, s, S, r, v, sprev);
-        const double* gp = Gbuf + (((k - k0) * 6) * n + o);
-        const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
-                     gyz = gp[4 * n], gzz = gp[5 * n];
+        const double gxx = g[0], gxy = g[1], gxz = g[2], gyy = g[3], gyz = g[4], gzz = g[5];
         for (int it = 0; it < ncorr; ++it) {
             anew[0] = gxx * r[0] + gxy * r[1] + gxz * r[2];
             anew[1] = gxy * r[0] + gyy * r[1] + gyz * r[2];
             anew[2] = gxz * r[0] + gyz * r[1] + gzz * r[2];
             gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         }
-        const int wslot = ring_slot(k);
+        for (int j = 0; j < 8; ++j)
+            for (int c = 0; c < 3; ++c)
+                A[j][c] = A[j + 1][c];
         for (int c = 0; c < 3; ++c)
-            ring[(wslot * 3 + c) * n + o] = anew[c];
+            A[8][c] = anew[c];
+        for (int e = 0; e < 6; ++e)
+            g[e] = gn[e];
     }
+    // A[j] = a_{klast-8+j}, at its ring slot
+    for (int j = 0; j < 9; ++j)
+        for (int c = 0; c < 3; ++c)
+            ring[(ring_slot(klast - 8 + j) * 3 + c) * n + o] = A[j][c];
     for (int c = 0; c < 3; ++c) {
         Z[c * n + o] = s[c];
         Z[(3 + c) * n + o] = S[c];
@@ -193,24 +240,25 @@ __global__ void checkpoint_kernel(PropState ps, double* __restrict__ ck, int to_
 
 // GJ8 steps k0 .. k1-1 exactly as gj8_chunk_kernel<kGravity> (the same plain evaluations and
 // arithmetic, gj8.hpp, so the trajectory is bit-identical; with fixed_point_exit each step stops
-// at the corrector's fixed point as the chunk kernel does), then per step one Jacobian
-// evaluation (Dual3) at the point of the step's last evaluation, stored as the step's GCRS
-// gravity Jacobian: Gbuf[((k - k0) * 6 + e) * n + o], e = xx, xy, xz, yy, yz, zz. The
-// derivative sweeps use it for every corrector iterate of the step: the iterates converge to
-// that point (from the predictor's point, a few 1e-5 m away for LEO at 60 s), where the gravity
-// gradient changes by about 1e-11 of itself, so the derivatives are those of the scheme with each
-// step's Jacobian taken at its fixed point. The point, and so G, is the same with or without the
-// fixed-point exit (the last evaluation of all ncorr is at the fixed point).
+// at the corrector's fixed point as the chunk kernel does), storing per step the ITRS point of
+// the step's last evaluation in the first three entries of its Jacobian slot,
+// Gbuf[((k - k0) * 6 + e) * n + o]; gravity_gradient_kernel then replaces it with the step's
+// GCRS gravity Jacobian there (e = xx, xy, xz, yy, yz, zz). The derivative sweeps use it for
+// every corrector iterate of the step: the iterates converge to that point (from the
+// predictor's point, a few 1e-5 m away for LEO at 60 s), where the gravity gradient changes by
+// about 1e-11 of itself, so the derivatives are those of the scheme with each step's Jacobian
+// taken at its fixed point. The point, and so G, is the same with or without the fixed-point
+// exit (the last evaluation of all ncorr is at the fixed point). The Jacobians are evaluated
+// apart from the steps because they do not feed the trajectory: in the steps they would lengthen
+// each step's serial chain, while all of a segment's Jacobians evaluate in one launch that
+// fills the chip.
 template <int W>
 __global__ void __launch_bounds__(64 * W)
-__attribute__((amdgpu_waves_per_eu(2))) // at most 256 VGPRs + AGPRs: 2 resident waves per SIMD
+__attribute__((amdgpu_waves_per_eu(4))) // 128 VGPRs: 4 resident waves per SIMD
 gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
     int64_t k1, int ncorr, double* __restrict__ Gbuf, int fixed_point_exit)
 {
-    // one exchange area for both kinds of evaluation (the Dual3 parts need the larger); pos holds
-    // the plain geometry (kPointLen per lane) or the ITRS position (3 per lane)
-    static_assert(kJacExchange >= kParts * 4 * 64, "the plain part sums fit the exchange");
-    __shared__ double red[W > 1 ? kJacExchange : 1];
+    __shared__ double red[W > 1 ? kParts * 4 * 64 : 1];
     __shared__ double pos[W > 1 ? kPointLen * 64 : 1];
     __shared__ int more; // the lead's control word for its helpers
     // Every barrier sits in workgroup-uniform control flow: the loops run to the group's last
@@ -227,12 +275,12 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         return;
     if constexpr (W > 1) {
         // The W waves of a workgroup hold the same 64 objects and share the harmonic columns.
-        // Waves 1 .. W-1 only sum their parts (gravity_record_helper_loop) for every evaluation
-        // wave 0 runs; wave 0 integrates, records the Jacobians and writes the ring and the state.
-        // Two barriers per evaluation, one more at the end.
+        // Waves 1 .. W-1 only sum their parts (gravity_helper_loop) for every evaluation wave 0
+        // runs; wave 0 integrates, stores the evaluation points and writes the ring and the
+        // state. Two barriers per evaluation, one more at the end.
         const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
         if (w > 0) {
-            gravity_record_helper_loop<W>(f.grav, red, pos, &more, w);
+            gravity_helper_loop<false, W>(f.grav, red, pos, &more, w);
             return;
         }
     }
@@ -292,20 +340,10 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
                 break; // every later evaluation would repeat this one
         }
         to_gcrs(M, gi, anew);
-        {
-            // the step's Jacobian at the point of its last evaluation
-            double gj[3], Gi[3][3], aj[3], G[3][3];
-            announce(2);
-            if constexpr (W > 1)
-                gravity_itrs_jac_lead<W>(rl, f.grav, red, pos, gj, Gi);
-            else
-                gravity_itrs_jac(rl[0], rl[1], rl[2], f.grav, gj, Gi);
-            jac_to_gcrs(M, gj, Gi, aj, G);
-            if (act) {
-                const double g6[6] = { G[0][0], G[0][1], G[0][2], G[1][1], G[1][2], G[2][2] };
-                for (int e = 0; e < 6; ++e)
-                    Gbuf[((k - k0) * 6 + e) * n + i] = g6[e];
-            }
+        if (act) {
+            // the point of the step's last evaluation, for gravity_gradient_kernel
+            for (int c = 0; c < 3; ++c)
+                Gbuf[((k - k0) * 6 + c) * n + i] = rl[c];
         }
         gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
         if (!act)
@@ -317,7 +355,7 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
             store_state();
     }
     if constexpr (W > 1) {
-        // release the helpers (gravity_record_helper_loop)
+        // release the helpers (gravity_helper_loop)
         announce(0);
         __syncthreads();
     }
@@ -339,76 +377,98 @@ __global__ void __launch_bounds__(64) gj8_reverse_kernel(const int64_t* __restri
     if (i >= n)
         return;
     const int64_t ns = nsteps[i];
+    const int64_t kfirst = std::min<int64_t>(k1 - 1, ns); // steps past the span change nothing
+    if (kfirst < k0)
+        return;
     const double hh = h * h, sa88 = c_sa[8][8], gj88 = c_gj[8][8];
     double* L = lam + (int64_t)rho * 33 * n;
-    for (int64_t k = k1 - 1; k >= k0; --k) {
-        if (k > ns)
-            continue;
-        const int sk = ring_slot(k);
-        {
-            // the cotangent of iterate it's acceleration gathers only in that iterate's reversal
-            // (and, for the last iterate, from the ring entry a_k it became)
-            double lk[3];
-            double lr[3], lv[3], lsp[3], lT1[3], lT2[3], ls[3], lal[3], lS[3];
+    // The cotangents in registers for the whole segment: lsp (s), lS (S) and the ring A, where
+    // A[j] holds the cotangent of a_{k-8+j} when step k begins (A[8]: the entry a_k that step k
+    // wrote); L holds them between launches, the ring at ring_slot of its acceleration's index.
+    double lsp[3], lS[3], A[9][3];
+    for (int c = 0; c < 3; ++c) {
+        lsp[c] = L[c * n + i];
+        lS[c] = L[(3 + c) * n + i];
+    }
+    for (int j = 0; j < 9; ++j)
+        for (int c = 0; c < 3; ++c)
+            A[j][c] = L[(6 + ring_slot(kfirst - 8 + j) * 3 + c) * n + i];
+    // the step's Jacobian, loaded one step ahead
+    auto load_g = [&](int64_t k, double* g) {
+        const double* gp = Gbuf + (((k - k0) * 6) * n + i);
+        for (int e = 0; e < 6; ++e)
+            g[e] = gp[e * n];
+    };
+    double g[6], gn[6];
+    load_g(kfirst, g);
+    for (int64_t k = kfirst; k >= k0; --k) {
+        if (k > k0)
+            load_g(k - 1, gn);
+        const double gxx = g[0], gxy = g[1], gxz = g[2], gyy = g[3], gyz = g[4], gzz = g[5];
+        // the cotangent of iterate it's acceleration gathers only in that iterate's reversal
+        // (and, for the last iterate, from the ring entry a_k it became)
+        double lk[3];
+        double lr[3], lv[3], lT1[3], lT2[3], ls[3], lal[3];
+        for (int c = 0; c < 3; ++c) {
+            lk[c] = A[8][c]; // the new ring entry a_k is the last iterate's acceleration
+            lr[c] = k == ns ? seeds[(rho * 6 + c) * n + i] : 0.0;
+            lv[c] = k == ns ? seeds[(rho * 6 + 3 + c) * n + i] : 0.0;
+            lT1[c] = lT2[c] = ls[c] = lal[c] = 0.0;
+        }
+        for (int it = ncorr - 1; it >= 0; --it) {
+            double le[3];
             for (int c = 0; c < 3; ++c) {
-                // the new ring entry a_k is the last iterate's acceleration
-                lk[c] = L[(6 + sk * 3 + c) * n + i];
-                L[(6 + sk * 3 + c) * n + i] = 0.0; // the slot now accumulates a_{k-9}
-                lr[c] = k == ns ? seeds[(rho * 6 + c) * n + i] : 0.0;
-                lv[c] = k == ns ? seeds[(rho * 6 + 3 + c) * n + i] : 0.0;
-                lsp[c] = L[c * n + i]; // the new s is s'_{K-1}
-                lS[c] = L[(3 + c) * n + i];
-                lT1[c] = lT2[c] = ls[c] = lal[c] = 0.0;
+                le[c] = it == ncorr - 1 ? lk[c] : 0.0;
+                const double t = hh * lr[c]; // r_{it+1} = hh ((T2 + gj88 e) + S')
+                lT2[c] += t;
+                le[c] += gj88 * t;
+                lS[c] += t;
+                const double u = h * lv[c]; // v_{it+1} = h ((T1 + sa88 e) + s'_it)
+                lT1[c] += u;
+                le[c] += sa88 * u;
+                lsp[c] += u;
+                ls[c] += lsp[c]; // s'_it = s + (e + a_{k-1}) / 2
+                le[c] += lsp[c] / 2;
+                lal[c] += lsp[c] / 2;
             }
-            const double* gp = Gbuf + (((k - k0) * 6) * n + i);
-            const double gxx = gp[0], gxy = gp[n], gxz = gp[2 * n], gyy = gp[3 * n],
-                         gyz = gp[4 * n], gzz = gp[5 * n];
-            for (int it = ncorr - 1; it >= 0; --it) {
-                double le[3];
-                for (int c = 0; c < 3; ++c) {
-                    le[c] = it == ncorr - 1 ? lk[c] : 0.0;
-                    const double t = hh * lr[c]; // r_{it+1} = hh ((T2 + gj88 e) + S')
-                    lT2[c] += t;
-                    le[c] += gj88 * t;
-                    lS[c] += t;
-                    const double u = h * lv[c]; // v_{it+1} = h ((T1 + sa88 e) + s'_it)
-                    lT1[c] += u;
-                    le[c] += sa88 * u;
-                    lsp[c] += u;
-                    ls[c] += lsp[c]; // s'_it = s + (e + a_{k-1}) / 2
-                    le[c] += lsp[c] / 2;
-                    lal[c] += lsp[c] / 2;
-                }
-                lr[0] = gxx * le[0] + gxy * le[1] + gxz * le[2];
-                lr[1] = gxy * le[0] + gyy * le[1] + gyz * le[2];
-                lr[2] = gxz * le[0] + gyz * le[1] + gzz * le[2];
-                for (int c = 0; c < 3; ++c)
-                    lv[c] = lsp[c] = 0.0;
-            }
-            double lPr[3];
+            lr[0] = gxx * le[0] + gxy * le[1] + gxz * le[2];
+            lr[1] = gxy * le[0] + gyy * le[1] + gyz * le[2];
+            lr[2] = gxz * le[0] + gyz * le[1] + gzz * le[2];
+            for (int c = 0; c < 3; ++c)
+                lv[c] = lsp[c] = 0.0;
+        }
+        double lPr[3];
+        for (int c = 0; c < 3; ++c) {
+            lPr[c] = hh * lr[c]; // r_0 = hh (Pr + S')
+            lS[c] += hh * lr[c];
+            ls[c] += lS[c]; // S' = S + s + a_{k-1} / 2
+            lal[c] += lS[c] / 2;
+        }
+        // Pr, T1, T2 and a_{k-1} over a_{k-9} .. a_{k-1}: entry j of the new ring is a_{k-9+j},
+        // the old entry j - 1 (a_{k-9} starts from zero in the slot a_k leaves)
+        for (int j = 8; j >= 0; --j) {
             for (int c = 0; c < 3; ++c) {
-                lPr[c] = hh * lr[c]; // r_0 = hh (Pr + S')
-                lS[c] += hh * lr[c];
-                ls[c] += lS[c]; // S' = S + s + a_{k-1} / 2
-                lal[c] += lS[c] / 2;
-            }
-            for (int j = 0; j < 9; ++j) { // Pr, T1, T2 and a_{k-1} over a_{k-9} .. a_{k-1}
-                const int sl = ring_slot(k - 9 + j);
-                for (int c = 0; c < 3; ++c) {
-                    double add = c_gj[9][j] * lPr[c];
-                    if (j >= 1)
-                        add += c_sa[8][j - 1] * lT1[c] + c_gj[8][j - 1] * lT2[c];
-                    if (j == 8)
-                        add += lal[c];
-                    L[(6 + sl * 3 + c) * n + i] += add;
-                }
-            }
-            for (int c = 0; c < 3; ++c) {
-                L[c * n + i] = ls[c];
-                L[(3 + c) * n + i] = lS[c];
+                double add = c_gj[9][j] * lPr[c];
+                if (j >= 1)
+                    add += c_sa[8][j - 1] * lT1[c] + c_gj[8][j - 1] * lT2[c];
+                if (j == 8)
+                    add += lal[c];
+                A[j][c] = (j >= 1 ? A[j - 1][c] : 0.0) + add;
             }
         }
+        for (int c = 0; c < 3; ++c)
+            lsp[c] = ls[c];
+        for (int e = 0; e < 6; ++e)
+            g[e] = gn[e];
     }
+    // the ring now begins step k0 - 1: A[j] is the cotangent of a_{k0-9+j}
+    for (int c = 0; c < 3; ++c) {
+        L[c * n + i] = lsp[c];
+        L[(3 + c) * n + i] = lS[c];
+    }
+    for (int j = 0; j < 9; ++j)
+        for (int c = 0; c < 3; ++c)
+            L[(6 + ring_slot(k0 - 9 + j) * 3 + c) * n + i] = A[j][c];
 }
 
 // Rows of the Jacobian (or the vector-Jacobian product when R = 1): out[(o * R + rho) * 6 + col].
@@ -496,12 +556,18 @@ void launch_gj8_record(hipStream_t st, const ForceCtx& f, const PropState& ps, i
 #define L_(Wv)                                                                                     \
     hipLaunchKernelGGL(gj8_record_kernel<Wv>, grid, dim3(64 * (Wv)), 0, st, f, ps, k0, k1, ncorr, \
         Gbuf, fixed_point_exit ? 1 : 0)
-    // 4 or 8 waves per group: the 64 KB exchange area leaves a 2-wave shape one wave per SIMD
     if (waves >= 8)
         L_(8);
     else
         L_(4);
 #undef L_
+    // the segment's Jacobians, behind the steps on the same stream; at most 65,535 steps per
+    // launch (the grid's y limit)
+    for (int64_t y0 = 0; y0 < k1 - k0; y0 += 65535) {
+        const unsigned ny = (unsigned)std::min<int64_t>(65535, k1 - k0 - y0);
+        hipLaunchKernelGGL(gravity_gradient_kernel, dim3((unsigned)((ps.N + 63) / 64), ny), dim3(64),
+            0, st, f.grav, f.prop_M, ps.nsteps, ps.N, k0 + y0, Gbuf + y0 * 6 * ps.N);
+    }
 }
 
 void launch_gj8_reverse(hipStream_t st, const int64_t* nsteps, int64_t n, double h, int64_t k0,

@@ -117,7 +117,7 @@ struct ogp_context {
 
     GravityModel grav;
     FrameData frames;
-    DevBuf d_rec, d_recf, d_diag, d_parts, d_part_diag, d_eop, d_leap_mjd, d_leap_sec, d_nut[4];
+    DevBuf d_rec, d_recf, d_jext, d_diag, d_parts, d_part_diag, d_eop, d_leap_mjd, d_leap_sec, d_nut[4];
     GravityDev gdev {};
     FrameDev fdev {};
     int kind = kGravity;  // kernel variant of the force
@@ -810,13 +810,17 @@ int ogp_create(const ogp_config* cfg, ogp_context** out)
             std::vector<float> recf = c.grav.records_f;
             rec.resize(rec.size() + kRecordPad * kRecordLen, 0.0);
             recf.resize(recf.size() + kRecordPad * kRecordLen, 0.0f);
+            std::vector<double> jext = c.grav.jac_ext;
+            jext.resize(jext.size() + kRecordPad * kJacExtLen, 0.0);
             std::vector<double> diag = c.grav.diag_ratio;
             diag.resize(diag.size() + 2, 0.0);
             upload(&c, c.d_rec, rec);
             upload(&c, c.d_recf, recf);
+            upload(&c, c.d_jext, jext);
             upload(&c, c.d_diag, diag);
             c.gdev.rec = c.d_rec.as<double>();
             c.gdev.recf = c.d_recf.as<float>();
+            c.gdev.jext = c.d_jext.as<double>();
             c.gdev.diag = c.d_diag.as<double>();
             std::vector<int> parts(kParts + 1);
             column_parts(c.grav.L, parts.data());

@@ -97,6 +97,8 @@ GravityModel load_gravity(const std::string& path, int degree)
     };
     g.records.clear();
     g.records.reserve(kRecordLen * tri(N, 0));
+    g.jac_ext.clear();
+    g.jac_ext.reserve(kJacExtLen * tri(N, 0));
     std::vector<double> scale(N + 1); // c_n of the current column
     for (int m = 0; m <= g.L; ++m) {
         for (int n = m; n <= g.L; ++n) {
@@ -128,6 +130,14 @@ GravityModel load_gravity(const std::string& path, int degree)
             const double rec[kRecordLen] = { xi, m * (cn * c), m * (cn * s), cn * kc, cn * ks,
                 cn * zc, cn * zs };
             g.records.insert(g.records.end(), rec, rec + kRecordLen);
+            double tc = 0, ts = 0;
+            if (n >= 2 && m >= 2) {
+                const double z2 = zeta(n, m - 2) * zeta(n, m - 1);
+                tc = z2 * g.C[tri(n, m - 2)];
+                ts = z2 * g.S[tri(n, m - 2)];
+            }
+            const double ext[kJacExtLen] = { (double)n, cn * tc, cn * ts, 0.0 };
+            g.jac_ext.insert(g.jac_ext.end(), ext, ext + kJacExtLen);
         }
     }
     g.records_f.assign(g.records.begin(), g.records.end());

@@ -40,6 +40,14 @@ struct FrameData {
 // Coefficients of degrees n < 2 are zero: the central term is added separately
 // (gravity_source_device.cu:146-152, :219).
 constexpr int kRecordLen = 7;
+// The gravity gradient's extension of each record (same order as the records), four values:
+//   n       : the degree, weight of the radial derivative ((R/r)^n differentiates to -n/r)
+//   tc, ts  : c_n zeta_{n,m-2} zeta_{n,m-1} Cbar_{n,m-2}, the same with Sbar (the second
+//             derivative in sin(phi) of the term (n, m - 2), carried by column m); zero for m < 2
+//   0       : padding (a record is one 32-byte scalar load)
+// The other second-derivative and radial sums reuse the seven record values (see the gradient
+// evaluation in device.hpp).
+constexpr int kJacExtLen = 4;
 struct GravityModel {
     int L = 0;         // degree/order used
     int file_max = 0;  // max_degree of the file
@@ -47,6 +55,7 @@ struct GravityModel {
     std::vector<double> C, S;       // triangular, index n(n+1)/2 + m
     std::vector<double> records;    // kRecordLen doubles per record
     std::vector<float> records_f;   // same in FP32
+    std::vector<double> jac_ext;    // kJacExtLen doubles per record
     std::vector<double> diag_ratio; // Pbar_mm / Pbar_{m-1,m-1} (m >= 2), [1] = sqrt(3)
     size_t num_records() const { return records.size() / kRecordLen; }
 };

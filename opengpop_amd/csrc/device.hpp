@@ -6,7 +6,6 @@
 
 #include "coefficients.hpp"
 #include "datasets.hpp"
-#include "dual.hpp"
 
 namespace ogp {
 
@@ -29,6 +28,7 @@ constexpr int kParts = 8;
 struct GravityDev {
     const double* rec;  // kRecordLen doubles per record, see datasets.hpp (GravityModel::records)
     const float* recf;  // the same records in FP32
+    const double* jext; // kJacExtLen doubles per record: the gravity gradient's extension
     const double* diag; // Pbar_mm / Pbar_{m-1,m-1}, m = 0..L, and two zero entries after
     const int* parts;   // kParts + 1 column bounds: part q = columns [parts[q], parts[q + 1])
     const double* part_diag; // per part q: D_m = prod_{j=1..m} diag[j] at m = parts[q] (D_0 = 1)
@@ -38,7 +38,7 @@ struct GravityDev {
 };
 
 // Partial harmonic sums of one column range (gravity_source_device.cu:122-210 restated).
-// V is double, or Dual3 for the Jacobian (dual.hpp).
+// V is double (FP32 parts of the split precision run on float values inside the columns).
 template <typename V> struct GravSumsT {
     V G, H, J, K;
 };
@@ -47,6 +47,19 @@ using GravSums = GravSumsT<double>;
 // Fused multiply-add in the precision of the operands (no promotion of FP32 to FP64).
 __device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float tofloat(double a) { return (float)a; }
+
+// 1 / sqrt(a) for a positive normal a: the hardware estimate (v_rsq_f64, about 24 correct bits)
+// and two Newton steps y += y (1/2 - (a/2) y^2). Measured on 2^20 values over 22 decades: below
+// one ulp, correctly rounded in 86% of cases. Seven dependent operations, where sqrt followed
+// by a division is about seventeen.
+__device__ __forceinline__ double drsqrt(double a)
+{
+    const double y0 = __builtin_amdgcn_rsq(a);
+    const double h = 0.5 * a;
+    const double y1 = __builtin_fma(y0, __builtin_fma(-h * y0, y0, 0.5), y0);
+    return __builtin_fma(y1, __builtin_fma(-h * y1, y1, 0.5), y1);
+}
 
 #ifndef OGP_RECORD_BATCH
 #define OGP_RECORD_BATCH 4 // gravity records per scalar-load wait
@@ -54,6 +67,15 @@ __device__ __forceinline__ double fmat(double a, double b, double c) { return __
 
 // Zero records after the last one of each table: a column's last batch is fetched whole.
 constexpr int kRecordPad = OGP_RECORD_BATCH - 1;
+
+// The gravity gradient's column sums beside the plain ones (JAC below), with V_n = n U_n:
+//   NA1, NA2, NB1, NB2, NE1, NE2 : sum V_n r[1..6] (the radial derivatives of the J/K, Gamma and
+//                                  H sums: (R/r)^n differentiates to -n (R/r)^n / r)
+//   T1, T2                       : sum U_n tc, U_n ts (second derivative in sin(phi) of the H
+//                                  terms of column m - 2)
+struct JacColumn {
+    double NA1, NA2, NB1, NB2, NE1, NE2, T1, T2;
+};
 
 // One column m of records [k0, k1): k0 is the diagonal n = m (accumulated only), the rest run
 // the recursion U_n = (xi'_n * eps * rho) U_{n-1} - rho^2 U_{n-2} on the scaled, rescaled
@@ -69,15 +91,23 @@ constexpr int kRecordPad = OGP_RECORD_BATCH - 1;
 // the short columns of low degrees would otherwise wait once per record of their last batch.
 // ZONAL: column m = 0, whose records feed the Gamma sum B1 only (its J, K and H terms carry the
 // factor m, and it has no sine terms): one FMA per record instead of six.
-template <typename T, typename V, bool ZONAL = false>
+// JAC: the gradient's sums jc as well (FP64), from the extension records ext; the plain sums
+// are formed by the same operations as without JAC, so they are bit-identical. Nine more
+// instructions per record (one multiplication and eight FMAs; two for ZONAL).
+template <typename T, typename V, bool ZONAL = false, bool JAC = false,
+    int BATCH = OGP_RECORD_BATCH>
 __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, V er, V rr2,
-    V& P, V& Pp, V& A1, V& A2, V& B1, V& B2, V& E1, V& E2, bool first_is_diag)
+    V& P, V& Pp, V& A1, V& A2, V& B1, V& B2, V& E1, V& E2, bool first_is_diag,
+    const double* ext = nullptr, JacColumn* jc = nullptr)
 {
+    static_assert(BATCH - 1 <= kRecordPad, "a whole batch must stay inside the padded table");
     constexpr int R = kRecordLen;
-    constexpr int BATCH = OGP_RECORD_BATCH;
+    constexpr int X = JAC ? kJacExtLen : 1;
     typedef const __attribute__((address_space(4))) T* tptr;
+    typedef const __attribute__((address_space(4))) double* xptr;
     tptr q = (tptr)table + R * (int64_t)k0;
-    auto accumulate = [&](const T* r) {
+    xptr qx = JAC ? (xptr)ext + X * (int64_t)k0 : nullptr;
+    auto accumulate = [&](const T* r, const double* x) {
         B1 = fmat(P, r[3], B1);
         if constexpr (!ZONAL) {
             A1 = fmat(P, r[1], A1);
@@ -86,53 +116,81 @@ __device__ __forceinline__ void column_terms(const void* table, int k0, int k1, 
             E1 = fmat(P, r[5], E1);
             E2 = fmat(P, r[6], E2);
         }
+        if constexpr (JAC) {
+            const V Vn = x[0] * P;
+            jc->NB1 = fmat(Vn, r[3], jc->NB1);
+            if constexpr (!ZONAL) {
+                jc->NA1 = fmat(Vn, r[1], jc->NA1);
+                jc->NA2 = fmat(Vn, r[2], jc->NA2);
+                jc->NB2 = fmat(Vn, r[4], jc->NB2);
+                jc->NE1 = fmat(Vn, r[5], jc->NE1);
+                jc->NE2 = fmat(Vn, r[6], jc->NE2);
+                jc->T1 = fmat(P, x[1], jc->T1);
+                jc->T2 = fmat(P, x[2], jc->T2);
+            }
+        }
     };
-    auto step = [&](const T* r) {
+    auto step = [&](const T* r, const double* x) {
         const V Pn = fmat(r[0] * er, P, -(rr2 * Pp));
         Pp = P;
         P = Pn;
-        accumulate(r);
+        accumulate(r, x);
     };
-    auto fetch = [&](T (*rec)[R]) {
+    auto fetch = [&](T (*rec)[R], double (*xr)[X]) {
 #pragma unroll
-        for (int b = 0; b < BATCH; ++b)
+        for (int b = 0; b < BATCH; ++b) {
 #pragma unroll
             for (int e = 0; e < R; ++e)
                 rec[b][e] = q[R * b + e];
+            if constexpr (JAC) {
+#pragma unroll
+                for (int e = 0; e < X; ++e)
+                    xr[b][e] = qx[X * b + e];
+            }
+        }
+    };
+    auto advance = [&]() {
+        q += R * BATCH;
+        if constexpr (JAC)
+            qx += X * BATCH;
     };
     int k = k0;
     if (first_is_diag) {
         T rec[BATCH][R];
-        fetch(rec);
-        accumulate(rec[0]);
+        double xr[BATCH][X];
+        fetch(rec, xr);
+        accumulate(rec[0], xr[0]);
         if (k1 - k0 >= BATCH) {
 #pragma unroll
             for (int b = 1; b < BATCH; ++b)
-                step(rec[b]);
+                step(rec[b], xr[b]);
         } else {
 #pragma unroll
             for (int b = 1; b < BATCH - 1; ++b)
                 if (b < k1 - k0)
-                    step(rec[b]);
+                    step(rec[b], xr[b]);
         }
         k += BATCH;
-        q += R * BATCH;
+        advance();
     }
-    for (; k + BATCH <= k1; k += BATCH, q += R * BATCH) {
+    for (; k + BATCH <= k1; k += BATCH) {
         T rec[BATCH][R];
-        fetch(rec);
+        double xr[BATCH][X];
+        fetch(rec, xr);
 #pragma unroll
         for (int b = 0; b < BATCH; ++b)
-            step(rec[b]);
+            step(rec[b], xr[b]);
+        advance();
     }
     if (k < k1) {
         T rec[BATCH][R];
-        fetch(rec);
-        step(rec[0]);
+        double xr[BATCH][X];
+        fetch(rec, xr);
+        step(rec[0], xr[0]);
 #pragma unroll
         for (int b = 1; b < BATCH - 1; ++b)
             if (b < k1 - k)
-                step(rec[b]);
+                step(rec[b], xr[b]);
     }
 }
 
@@ -169,7 +227,7 @@ __device__ __forceinline__ GravPointT<V> grav_point(V x, V y, V z, double radius
     p.cphi = 0.0;
     p.U1 = 1.0;
     p.W1 = 0.0;
-    if (val(pxy2) > 0.0) {
+    if (pxy2 > 0.0) {
         const V ip = drsqrt(pxy2);
         p.cphi = (pxy2 * ip) * p.ir;
         p.U1 = x * ip;
@@ -233,16 +291,31 @@ __device__ __forceinline__ void part_start(const GravPointT<V>& p, int m0, doubl
     diagP = p.rho * cr * Dm0;
 }
 
+// The gravity gradient's part sums (JAC evaluations): the plain sums s (bit-identical to the
+// plain evaluation's), the radial sums NJ, NK, NH (n-weighted J, K, H) and NF (n (n + 1)
+// weighted potential terms), and the second derivatives of the potential's angular sum in the
+// direction cosines (x/r, y/r, z/r) = (s, t, sin phi) taken as independent variables (the
+// formulation of Pines 1973): Tss (= -Ttt), Tst, Tse, Tte, Tee, each still to be divided by
+// cos(phi) once (they pair the functions of column m with the longitude terms of column m - 2).
+struct GravJacSums {
+    GravSums s;
+    double NJ, NK, NH, NF, Tss, Tst, Tse, Tte, Tee;
+};
+constexpr int kJacSums = 13; // doubles of a GravJacSums
+
 // Sums of parts [q0, q1) (q0 < q1, wave-uniform); sink(q, sums) receives the sums of part q, in
 // part order. Each part starts from part_start's closed-form values.
-template <bool SPLIT, typename V, class Sink>
+// JAC (FP64 only): the gravity gradient's sums as well; sink then receives GravJacSums.
+template <bool SPLIT, bool JAC = false, typename V, class Sink>
 __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const GravityDev& g, int q0,
     int q1, Sink&& sink)
 {
+    static_assert(!(JAC && SPLIT), "the gradient is evaluated in FP64");
     const V er = p.er, rr2 = p.rr2, cphi = p.cphi, rho = p.rho, U1 = p.U1, W1 = p.W1;
     const float erf_ = tofloat(er), rr2f = tofloat(rr2);
     V Um, Wm;   // cos(m lambda), sin(m lambda)
     V Um1, Wm1; // (m - 1)
+    V Um2, Wm2; // (m - 2), JAC only
     V diagP;    // scaled diagonal cos(phi)^(m-1) (R/r)^m Pbar_mm
     const uniform_dptr diag = (uniform_dptr)g.diag;
     const uniform_iptr parts = (uniform_iptr)g.parts;
@@ -251,10 +324,19 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
     for (int q = q0; q < q1; ++q) {
         const int mb = parts[q];
         part_start(p, mb, part_diag[q], Um, Wm, Um1, Wm1, diagP);
+        if constexpr (JAC) {
+            // cos and sin of (mb - 2) lambda; used from column 2 on, which a part that starts
+            // at column 0 or 1 reaches through the advance below
+            Um2 = fmat(Um1, U1, Wm1 * W1);
+            Wm2 = fmat(Wm1, U1, -(Um1 * W1));
+        }
         int m = mb;
         int kcol = m * (L + 1) - m * (m - 1) / 2; // first record of column m
         // Gc: the Gamma sum of the columns m >= 1 before its factor cos(phi), applied once
         V G = 0.0, Gc = 0.0, H = 0.0, J = 0.0, K = 0.0;
+        // the gradient's part sums (JAC); NFc: the columns m >= 1 of NF before the cos(phi)
+        V NJ = 0.0, NK = 0.0, NH = 0.0, NF = 0.0, NFc = 0.0;
+        V Tss = 0.0, Tst = 0.0, Tse = 0.0, Tte = 0.0, Tee = 0.0;
         // the next column's diagonal ratio, loaded one column ahead so that its wait is the
         // wait of the current column's first record batch
         double ratio = diag[mb + 1];
@@ -268,7 +350,20 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
             }
             const int ncol = L - m + 1;
             V A1, A2, B1, B2, E1, E2;
-            if constexpr (!SPLIT) {
+            JacColumn jc;
+            if constexpr (JAC) {
+                V P = diagP, Pp = 0.0;
+                A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
+                jc = JacColumn { 0, 0, 0, 0, 0, 0, 0, 0 };
+                // batches of 2: the plain and extension records of a batch, 22 doubles, stay
+                // within the scalar registers
+                if (m == 0)
+                    column_terms<double, V, true, true, 2>(g.rec, kcol, kcol + ncol, er, rr2, P,
+                        Pp, A1, A2, B1, B2, E1, E2, true, g.jext, &jc);
+                else
+                    column_terms<double, V, false, true, 2>(g.rec, kcol, kcol + ncol, er, rr2, P,
+                        Pp, A1, A2, B1, B2, E1, E2, true, g.jext, &jc);
+            } else if constexpr (!SPLIT) {
                 V P = diagP, Pp = 0.0;
                 A1 = A2 = B1 = B2 = E1 = E2 = 0.0;
                 if (m == 0)
@@ -306,14 +401,39 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
             }
             if (m == 0) {
                 G = B1;
+                if constexpr (JAC)
+                    NF = jc.NB1; // (1 + n) c_n C: r[3] - r[1] with r[1] = 0 in column 0
             } else {
                 // A1 and A2 carry the factor m (GravityModel::records)
                 Gc = fmat(Um, B1, fmat(Wm, B2, Gc));
                 J = fmat(Um1, A1, fmat(Wm1, A2, J));
                 K = fmat(Um1, A2, fmat(-Wm1, A1, K));
                 H = fmat(Um1, E1, fmat(Wm1, E2, H));
+                if constexpr (JAC) {
+                    // n (n + 1) c_n C = n ((1 + n + m) - m) c_n C = n (r[3] - r[1])
+                    NFc = fmat(Um, jc.NB1 - jc.NA1, fmat(Wm, jc.NB2 - jc.NA2, NFc));
+                    NJ = fmat(Um1, jc.NA1, fmat(Wm1, jc.NA2, NJ));
+                    NK = fmat(Um1, jc.NA2, fmat(-Wm1, jc.NA1, NK));
+                    NH = fmat(Um1, jc.NE1, fmat(Wm1, jc.NE2, NH));
+                    if (m >= 2) {
+                        // d2/ds2 and d2/ds dt of the J, K terms: m (m - 1) c_n C = (m - 1) r[1];
+                        // d/d sin(phi) of the J, K terms of column m - 1, carried here:
+                        // (m - 1) zeta_{n,m-1} c_n C_{n,m-1} = (m - 1) r[5]
+                        const double f = (double)(m - 1);
+                        const V a1 = f * A1, a2 = f * A2, e1 = f * E1, e2 = f * E2;
+                        Tss = fmat(Um2, a1, fmat(Wm2, a2, Tss));
+                        Tst = fmat(Um2, a2, fmat(-Wm2, a1, Tst));
+                        Tse = fmat(Um2, e1, fmat(Wm2, e2, Tse));
+                        Tte = fmat(Um2, e2, fmat(-Wm2, e1, Tte));
+                        Tee = fmat(Um2, jc.T1, fmat(Wm2, jc.T2, Tee));
+                    }
+                }
             }
             // advance the longitude terms: (U + iW)_{m+1} = (U + iW)_m (U1 + i W1)
+            if constexpr (JAC) {
+                Um2 = Um1;
+                Wm2 = Wm1;
+            }
             Um1 = Um;
             Wm1 = Wm;
             const V un = fmat(Um, U1, -Wm * W1);
@@ -321,7 +441,11 @@ __device__ __forceinline__ void grav_parts(const GravPointT<V>& p, const Gravity
             Um = un;
             kcol += ncol;
         }
-        sink(q, GravSumsT<V> { fmat(cphi, Gc, G), H, J, K });
+        if constexpr (JAC)
+            sink(q, GravJacSums { GravSumsT<V> { fmat(cphi, Gc, G), H, J, K }, NJ, NK, NH,
+                                     fmat(cphi, NFc, NF), Tss, Tst, Tse, Tte, Tee });
+        else
+            sink(q, GravSumsT<V> { fmat(cphi, Gc, G), H, J, K });
     }
 }
 
@@ -401,95 +525,159 @@ __device__ __forceinline__ void gravity_itrs(
     grav_assemble(x, y, z, p, f.t, g.gm, ax, ay, az);
 }
 
-// Acceleration and its Jacobian G[i][j] = d a_i / d r_j in the ITRS (FP64, one wave): the same
-// evaluation on dual numbers, so the acceleration is bit-identical to gravity_itrs<false>.
+// ------------------------------------------------------------------------------------------
+// Gravity gradient: the acceleration and its Jacobian G[i][j] = d a_i / d r_j in the ITRS
+// ------------------------------------------------------------------------------------------
+// With u = (s, t, e) = r / |r| and the potential U = (gm / r) (1 + sum_n (R/r)^n Phi_n(u)), where
+// each Phi_n is a polynomial in s, t and e taken as independent variables (Pines 1973), the
+// acceleration is a = -(gm / r^2) (Lam u - Q) with Q = (J, K, H) the first derivatives of the
+// angular sum and Lam = 1 + Gamma + e H (grav_assemble). Differentiating once more, with Hs the
+// Hessian of the angular sum in u (Tss, Tst, Tse; Tst, -Tss, Tte; Tse, Tte, Tee), NQ = (NJ, NK,
+// NH) and NF the n- and n (n + 1)-weighted sums (the radial derivative of (R/r)^n):
+//   G = -(gm / r^3) (Lam I - Hs + u W^T + W u^T - (3 Lam + sig) u u^T),
+//   W = 2 Q + NQ + Hs u,  sig = NF + u.NQ + u.W.
+// The derivatives of the derived Legendre functions are those of the next columns
+// (d/de Pbar_n^m = zeta_{n,m} Pbar_n^{m+1}), so every second-derivative sum is formed while the
+// column recursion visits column m, from the J/K and H records ((m - 1) times their column sums)
+// and the extension records (GravityModel::jac_ext). The functions of column m carry
+// cos(phi)^(m-1) and the second derivatives pair them with the longitude terms of column m - 2,
+// whose factor is cos(phi)^(m-2): the T sums are divided by cos(phi) once, after the fold.
+// Exactly on the polar axis (x = y = 0, cos(phi) = 0) the columns m >= 2 vanish and their
+// second-derivative terms are omitted.
+//
+// The plain sums G, H, J, K are formed and folded by the operations of the plain evaluation, so
+// the acceleration is bit-identical to gravity_itrs<false> (and gravity_gcrs<false, W>).
+
+__device__ __forceinline__ void jac_sums_to_array(const GravJacSums& s, double* v)
+{
+    v[0] = s.s.G;
+    v[1] = s.s.H;
+    v[2] = s.s.J;
+    v[3] = s.s.K;
+    v[4] = s.NJ;
+    v[5] = s.NK;
+    v[6] = s.NH;
+    v[7] = s.NF;
+    v[8] = s.Tss;
+    v[9] = s.Tst;
+    v[10] = s.Tse;
+    v[11] = s.Tte;
+    v[12] = s.Tee;
+}
+
+// The part sums added in part order with the fold tree of PartFoldT: the plain sums fold exactly
+// as in the plain evaluation.
+struct JacFold {
+    double t[kJacSums], quad[kJacSums], pair[kJacSums];
+    __device__ __forceinline__ void operator()(int q, const double* v)
+    {
+        static_assert(kParts == 8, "the fold tree is written for 8 parts");
+#pragma unroll
+        for (int c = 0; c < kJacSums; ++c) {
+            if (q % 2 == 0) {
+                pair[c] = v[c];
+                continue;
+            }
+            pair[c] += v[c];
+            if (q % 4 == 1) {
+                quad[c] = pair[c];
+                continue;
+            }
+            quad[c] += pair[c];
+            if (q == 3)
+                t[c] = quad[c];
+            else
+                t[c] += quad[c];
+        }
+    }
+};
+
+// The folded sums v (kJacSums, JacFold order) at the point (x, y, z) with geometry p: the
+// acceleration a (grav_assemble's) and G.
+__device__ __forceinline__ void grav_jac_assemble(double x, double y, double z, const GravPoint& p,
+    const double* v, double gm, double* a, double (*G)[3])
+{
+    const GravSums s { v[0], v[1], v[2], v[3] };
+    grav_assemble(x, y, z, p, s, gm, a[0], a[1], a[2]);
+    const double ic = p.cphi > 0.0 ? 1.0 / p.cphi : 0.0;
+    const double Tss = v[8] * ic, Tst = v[9] * ic, Tse = v[10] * ic, Tte = v[11] * ic,
+                 Tee = v[12] * ic;
+    const double u[3] = { x * p.ir, y * p.ir, p.eps };
+    const double Q[3] = { s.J, s.K, s.H };
+    const double NQ[3] = { v[4], v[5], v[6] };
+    const double Hs[3][3] = { { Tss, Tst, Tse }, { Tst, -Tss, Tte }, { Tse, Tte, Tee } };
+    const double Lam = (s.G + 1.0) + z * s.H * p.ir;
+    double Wv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double Hu = fmat(Hs[i][2], u[2], fmat(Hs[i][1], u[1], Hs[i][0] * u[0]));
+        Wv[i] = fmat(2.0, Q[i], NQ[i] + Hu);
+    }
+    const double uNQ = fmat(u[2], NQ[2], fmat(u[1], NQ[1], u[0] * NQ[0]));
+    const double uW = fmat(u[2], Wv[2], fmat(u[1], Wv[1], u[0] * Wv[0]));
+    const double kuu = fmat(3.0, Lam, v[7] + uNQ + uW);
+    const double c3 = -gm * p.ir * p.ir * p.ir;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+            const double e = (i == j ? Lam : 0.0) - Hs[i][j]
+                + fmat(u[i], Wv[j], fmat(u[j], Wv[i], -(kuu * u[i]) * u[j]));
+            G[i][j] = c3 * e;
+            G[j][i] = G[i][j];
+        }
+}
+
+// One wave: every part, folded in registers.
 __device__ __forceinline__ void gravity_itrs_jac(
     double x, double y, double z, const GravityDev& g, double* a, double (*G)[3])
 {
-    const Dual3 X = dvar(x, 0), Y = dvar(y, 1), Z = dvar(z, 2);
-    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
-    PartFoldT<Dual3> f;
-    grav_parts<false>(p, g, 0, kParts, f);
-    Dual3 A[3];
-    grav_assemble(X, Y, Z, p, f.t, g.gm, A[0], A[1], A[2]);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        a[i] = A[i].v;
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            G[i][j] = A[i].d[j];
-    }
+    const GravPoint p = grav_point(x, y, z, g.radius);
+    JacFold f;
+    grav_parts<false, true>(p, g, 0, kParts, [&](int q, const GravJacSums& s) {
+        double v[kJacSums];
+        jac_sums_to_array(s, v);
+        f(q, v);
+    });
+    grav_jac_assemble(x, y, z, p, f.t, g.gm, a, G);
 }
 
-// Column-split Jacobian evaluation for a workgroup of W waves holding the same 64 objects (the
-// dual-number counterpart of gravity_itrs_split below): each part's Dual3 sums (16 doubles per
-// lane) go to LDS, red holding kParts * 16 * 64 doubles, and every wave folds the kParts slots in
-// part order, so value and Jacobian are bit-identical to gravity_itrs_jac for every W.
-constexpr int kJacExchange = kParts * 16 * 64;
+// Column-split form for a workgroup of W waves holding the same 64 objects: each part's
+// kJacSums sums go to LDS (red[(q * kJacSums + c) * 64 + lane], red holding kJacExchange
+// doubles, 53 KB), and the folding wave adds the kParts slots in part order.
+constexpr int kJacExchange = kParts * kJacSums * 64;
 
-// Wave w's Dual3 part sums (parts [wave_part(w), wave_part(w) + kParts / W)) into their LDS
-// slots.
+// Wave w's part sums (parts [wave_part(w), wave_part(w) + kParts / W)) into their LDS slots.
 template <int W>
-__device__ __forceinline__ void jac_parts_to_lds(const GravPointT<Dual3>& p, const GravityDev& g,
+__device__ __forceinline__ void jac_parts_to_lds(const GravPoint& p, const GravityDev& g,
     double* red, int w)
 {
     const int lane = threadIdx.x % 64;
-    auto put = [&](int q, int t, const Dual3& d) {
-        double* o = red + (q * 16 + 4 * t) * 64 + lane;
-        o[0] = d.v;
-        o[64] = d.d[0];
-        o[128] = d.d[1];
-        o[192] = d.d[2];
-    };
-    grav_parts<false>(p, g, wave_part<W>(w), wave_part<W>(w) + kParts / W,
-        [&](int q, const GravSumsT<Dual3>& s) {
-            put(q, 0, s.G);
-            put(q, 1, s.H);
-            put(q, 2, s.J);
-            put(q, 3, s.K);
+    grav_parts<false, true>(p, g, wave_part<W>(w), wave_part<W>(w) + kParts / W,
+        [&](int q, const GravJacSums& s) {
+            double v[kJacSums];
+            jac_sums_to_array(s, v);
+#pragma unroll
+            for (int c = 0; c < kJacSums; ++c)
+                red[(q * kJacSums + c) * 64 + lane] = v[c];
         });
 }
 
-// The kParts exchanged Dual3 part sums, folded in part order, then the ITRS value and Jacobian.
-__device__ __forceinline__ void jac_fold_assemble(const Dual3& X, const Dual3& Y, const Dual3& Z,
-    const GravPointT<Dual3>& p, const GravityDev& g, const double* red, double* a, double (*G)[3])
+// The kParts exchanged part sums, folded in part order, then the ITRS value and Jacobian.
+__device__ __forceinline__ void jac_fold_assemble(double x, double y, double z,
+    const GravPoint& p, const GravityDev& g, const double* red, double* a, double (*G)[3])
 {
     const int lane = threadIdx.x % 64;
-    auto get = [&](int q, int t) {
-        const double* o = red + (q * 16 + 4 * t) * 64 + lane;
-        return Dual3(o[0], o[64], o[128], o[192]);
-    };
-    PartFoldT<Dual3> f;
+    JacFold f;
 #pragma unroll 1
-    for (int q = 0; q < kParts; ++q)
-        f(q, GravSumsT<Dual3> { get(q, 0), get(q, 1), get(q, 2), get(q, 3) });
-    Dual3 A[3];
-    grav_assemble(X, Y, Z, p, f.t, g.gm, A[0], A[1], A[2]);
+    for (int q = 0; q < kParts; ++q) {
+        double v[kJacSums];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        a[i] = A[i].v;
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            G[i][j] = A[i].d[j];
+        for (int c = 0; c < kJacSums; ++c)
+            v[c] = red[(q * kJacSums + c) * 64 + lane];
+        f(q, v);
     }
-}
-
-template <int W>
-__device__ __forceinline__ void gravity_itrs_jac_split(double x, double y, double z,
-    const GravityDev& g, double* red, double* a, double (*G)[3])
-{
-    if (W == 1) {
-        gravity_itrs_jac(x, y, z, g, a, G);
-        return;
-    }
-    static_assert(kParts % W == 0, "waves must divide the part count");
-    const Dual3 X = dvar(x, 0), Y = dvar(y, 1), Z = dvar(z, 2);
-    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
-    __syncthreads();
-    jac_parts_to_lds<W>(p, g, red, w);
-    __syncthreads();
-    jac_fold_assemble(X, Y, Z, p, g, red, a, G);
+    grav_jac_assemble(x, y, z, p, f.t, g.gm, a, G);
 }
 
 // Column-split evaluation for a workgroup of W waves that all hold the same 64 objects: wave w
@@ -688,38 +876,34 @@ __device__ __forceinline__ void jac_to_gcrs(const uniform_dptr M, const double* 
             G[i][j] = M[i] * T[0][j] + M[3 + i] * T[1][j] + M[6 + i] * T[2][j];
 }
 
-// Lead/helper form of the Jacobian gravity for W > 1 (the derivative record kernel), as
-// gravity_gcrs_lead: wave 0 leaves the ITRS position in pos before the first barrier, every wave
-// sums its Dual3 parts there (gravity_record_helper_loop for waves 1 .. W-1), and wave 0 alone
-// folds and assembles. Bit-identical to gravity_itrs_jac_split<W>; the barrier hazards are those
-// of gravity_gcrs_lead.
+// Lead/helper form of the gravity gradient for W > 1 (the startup's recording), as
+// gravity_gcrs_lead: wave 0 computes the evaluation point's geometry and leaves it in pos before
+// the first barrier, every wave sums its parts there (gravity_record_helper_loop for waves
+// 1 .. W-1), and wave 0 alone folds and assembles. The
+// acceleration is bit-identical to gravity_itrs_lead<false, W>; the barrier hazards are those of
+// gravity_gcrs_lead.
 //
-// gravity_itrs_jac_lead is the lead wave's part at the ITRS point ri (the position to pos, the
-// Dual3 parts, the fold and assembly): gi and its Jacobian Gi in the ITRS.
+// gravity_itrs_jac_lead is the lead wave's part at the ITRS point ri (the geometry to pos, its
+// parts, the fold and assembly): gi and its Jacobian Gi in the ITRS.
 template <int W>
 __device__ __forceinline__ void gravity_itrs_jac_lead(const double* ri, const GravityDev& g,
     double* red, double* pos, double* gi, double (*Gi)[3])
 {
     static_assert(W > 1 && kParts % W == 0, "waves must divide the part count");
     const int lane = threadIdx.x % 64;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-        pos[i * 64 + lane] = ri[i];
-    const Dual3 X = dvar(ri[0], 0), Y = dvar(ri[1], 1), Z = dvar(ri[2], 2);
-    const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
+    const GravPoint p = grav_point(ri[0], ri[1], ri[2], g.radius);
+    put_point(pos, lane, p);
     __syncthreads();
     jac_parts_to_lds<W>(p, g, red, 0);
     __syncthreads();
-    jac_fold_assemble(X, Y, Z, p, g, red, gi, Gi);
+    jac_fold_assemble(ri[0], ri[1], ri[2], p, g, red, gi, Gi);
 }
 
-// The helpers' share of the derivative record pass, whose lead runs plain evaluations (the
-// trajectory) and one Jacobian evaluation per step: after each first barrier the control word
-// `more` says what the lead evaluates: 1 = the plain gravity at the geometry in pos
-// (gravity_itrs_lead, part sums to red), 2 = the Dual3 gravity at the ITRS position in pos
-// (gravity_itrs_jac_lead, Dual3 part sums to red), 0 = the end (after the lead's last barrier).
-// The lead writes `more` before the first barrier of each evaluation; the helpers read it
-// between that barrier and the second.
+// The helpers' share of the startup's recording, whose lead evaluates the gravity gradient at
+// every force evaluation (gravity_itrs_jac_lead): after each first barrier the control word
+// `more` is nonzero while the lead evaluates (the gradient's part sums at the geometry in pos to
+// red) and 0 at the end (after the lead's last barrier). The lead writes `more` before the first
+// barrier of each evaluation; the helpers read it between that barrier and the second.
 template <int W>
 __device__ __forceinline__ void gravity_record_helper_loop(const GravityDev& g, double* red,
     const double* pos, const int* more, int w)
@@ -727,24 +911,9 @@ __device__ __forceinline__ void gravity_record_helper_loop(const GravityDev& g, 
     const int lane = threadIdx.x % 64;
     for (;;) {
         __syncthreads();
-        const int what = __builtin_amdgcn_readfirstlane(*more);
-        if (what == 0)
+        if (__builtin_amdgcn_readfirstlane(*more) == 0)
             return;
-        if (what == 1) {
-            const GravPoint p = get_point(pos, lane);
-            grav_parts<false>(p, g, wave_part<W>(w), wave_part<W>(w) + kParts / W,
-                [&](int q, const GravSums& s) {
-                    red[(q * 4 + 0) * 64 + lane] = s.G;
-                    red[(q * 4 + 1) * 64 + lane] = s.H;
-                    red[(q * 4 + 2) * 64 + lane] = s.J;
-                    red[(q * 4 + 3) * 64 + lane] = s.K;
-                });
-        } else {
-            const Dual3 X = dvar(pos[lane], 0), Y = dvar(pos[64 + lane], 1),
-                        Z = dvar(pos[128 + lane], 2);
-            const GravPointT<Dual3> p = grav_point(X, Y, Z, g.radius);
-            jac_parts_to_lds<W>(p, g, red, w);
-        }
+        jac_parts_to_lds<W>(get_point(pos, lane), g, red, w);
         __syncthreads();
     }
 }
@@ -754,8 +923,8 @@ __device__ __forceinline__ void gravity_record_helper_loop(const GravityDev& g, 
 // (unit_tests/include/mock_force_model.cuh:69-73 and :99)
 // ------------------------------------------------------------------------------------------
 // The startup's derivative (the variational equations of the RK8 startup and its corrector):
-//  * kGravityRecord runs the startup of the objects themselves with the gravity evaluated on
-//    dual numbers (its value part bit-identical to kGravity) and records, for every force
+//  * kGravityRecord runs the startup of the objects themselves with the gravity gradient
+//    evaluation (its acceleration bit-identical to kGravity) and records, for every force
 //    evaluation, the GCRS Jacobian G = dg/dr (6 unique entries) at the evaluation's slot
 //    (grec_slot_gen / grec_slot_prop);
 //  * kRecordedTangent runs the same startup code on tangent states (the derivatives of the
@@ -946,7 +1115,7 @@ __device__ __forceinline__ double& grec_at(const ForceCtx& f, int64_t q, int e)
     return f.grec[(q * 6 + e) * f.grec_n];
 }
 
-// kGravityRecord: the gravity at the GCRS position x for node matrix Mg on dual numbers (W > 1:
+// kGravityRecord: the gravity and its gradient at the GCRS position x for node matrix Mg (W > 1:
 // the lead's part of a lead/helper evaluation, whose helpers run gravity_record_helper_loop),
 // the acceleration into a (bit-identical to gravity_gcrs<false, W>) and G into slot q.
 template <int W>

@@ -20,18 +20,17 @@ struct GJ8Sums {
     double alast[3];     // a_{k-1}
 };
 
-__device__ __forceinline__ void gj8_sums(const double* hist, int64_t N, int64_t i, int64_t k,
-    GJ8Sums& q)
+// get(j, c): component c of a_{k-9+j}.
+template <class Get> __device__ __forceinline__ void gj8_sums_of(Get&& get, GJ8Sums& q)
 {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
         q.Pv[c] = q.Pr[c] = q.T1[c] = q.T2[c] = 0.0;
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
-        const int slot = ring_slot(k - 9 + j);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double a = hist[(slot * 3 + c) * N + i];
+            const double a = get(j, c);
             q.Pv[c] = fma(c_sa[9][j], a, q.Pv[c]);
             q.Pr[c] = fma(c_gj[9][j], a, q.Pr[c]);
             if (j >= 1) {
@@ -42,6 +41,13 @@ __device__ __forceinline__ void gj8_sums(const double* hist, int64_t N, int64_t 
                 q.alast[c] = a;
         }
     }
+}
+
+// The ring in memory: a_j at slot ring_slot(j), hist[(slot * 3 + c) * N + i].
+__device__ __forceinline__ void gj8_sums(const double* hist, int64_t N, int64_t i, int64_t k,
+    GJ8Sums& q)
+{
+    gj8_sums_of([&](int j, int c) { return hist[(ring_slot(k - 9 + j) * 3 + c) * N + i]; }, q);
 }
 
 // Predictor: S_{k} = S_{k-1} + s_{k-1} + a_{k-1}/2, predicted v and r; sprev keeps s_{k-1}.

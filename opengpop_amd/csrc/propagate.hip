@@ -214,8 +214,8 @@ __device__ __forceinline__ Lane lane_object(int64_t obj0, int64_t nobj, int64_t 
 // leaves its state untouched.
 // Register budget: 128 VGPRs (4 resident waves per SIMD) for gravity, whose lead wave then
 // spills part of its RK8 work area (100,000 objects at degree 70: 18.7 against 25.0 ms with
-// 256 VGPRs and 2 waves per SIMD; 20,000: 5.4 against 8.6 ms); 256 for the tangent form, whose
-// dual numbers would spill heavily in 128, and for the analytic test forces.
+// 256 VGPRs and 2 waves per SIMD; 20,000: 5.4 against 8.6 ms); 256 for the recording startup,
+// whose gradient evaluation holds 13 sums per part, and for the analytic test forces.
 template <int KIND, int W>
 __global__ void __launch_bounds__(64 * W)
 __attribute__((amdgpu_waves_per_eu(KIND == kGravity || KIND == kGravitySplit ? 4 : 2)))
@@ -224,7 +224,7 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
     int fixed_point_exit)
 {
     constexpr bool kLead = W > 1 && (KIND == kGravity || KIND == kGravitySplit);
-    constexpr bool kRecLead = W > 1 && KIND == kGravityRecord; // Dual3 parts, 64 KB exchange
+    constexpr bool kRecLead = W > 1 && KIND == kGravityRecord; // gradient parts, 53 KB exchange
     __shared__ double red[kRecLead ? kJacExchange : ExchangeArea<W>::size];
     __shared__ double pos[W > 1 ? kPointLen * 64 : 1]; // the lead's geometry (gravity_gcrs_lead)
     __shared__ int more; // the lead's control word for its helpers (gravity_helper_loop)
@@ -250,7 +250,7 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
             return;
         }
         if (threadIdx.x == 0)
-            more = kRecLead ? 2 : 1; // every recording evaluation is a Dual3 evaluation
+            more = kRecLead ? 2 : 1; // every recording evaluation evaluates the gradient
     }
     const Slots nr { sc.node_r, N, i, act }, nv { sc.node_v, N, i, act }, acc { sc.acc, N, i, act };
     const Slots sm { sc.sm, N, i, act }, Sm { sc.Sm, N, i, act }, Kx { sc.Kx, N, i, act },
@@ -556,7 +556,7 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
     hipLaunchKernelGGL((startup_kernel<K, Wv>), grid, dim3(64 * (Wv)), 0, st, f, rk8_variant, ps,   \
         sc, obj0, nobj, ncorr, traj, traj_steps, fixed_point_exit ? 1 : 0)
     if (kind == kGravityRecord) {
-        // 4 or 8 waves: the 64 KB Dual3 exchange leaves a 2-wave shape one wave per SIMD
+        // 4 or 8 waves: the 53 KB gradient exchange leaves a 2-wave shape one wave per SIMD
         if (waves >= 8)
             L_(kGravityRecord, 8);
         else

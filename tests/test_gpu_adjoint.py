@@ -51,6 +51,37 @@ def test_gravity_jacobian_matches_oracle_differences(oracle70):
         assert err.max() < 1e-8
 
 
+def _near_pole_points(n, seed):
+    # |(x, y)| / r from 1e-9 to 1e-1: the columns m >= 2 of the gradient are divided by cos(phi)
+    rng = np.random.default_rng(seed)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    c = 10.0 ** rng.uniform(-9, -1, n)
+    u = np.stack([c * np.cos(ang), c * np.sin(ang),
+                  np.where(rng.random(n) < 0.5, -1.0, 1.0) * np.sqrt(1 - c * c)], 1)
+    return u * rng.uniform(6.6e6, 4.3e7, (n, 1))
+
+
+@pytest.mark.parametrize("degree", [20, 70])
+def test_gravity_jacobian_matches_richardson_differences(degree):
+    # the gradient against differences of the same library's acceleration, Richardson-extrapolated
+    # (error O(h^4)) with h = 3e-5 r: truncation ~1e-18 and rounding ~5e-12 of |G|, while the
+    # terms of each degree up to 70 add more than ~1e-7 of |G| at LEO radii, so a wrong radial
+    # weight or second-derivative coefficient of any degree cannot pass
+    r = np.vstack([_rand_points(384, 6), _near_pole_points(128, 7)])
+    h = 3e-5 * np.linalg.norm(r, axis=1)[:, None]
+    with ogp.Context(degree=degree) as c:
+        _, G = c.gravity_jacobian_itrs(r)
+        fd = np.empty_like(G)
+        for j in range(3):
+            e = np.zeros(3)
+            e[j] = 1.0
+            d1 = (c.gravity_itrs(r + h * e) - c.gravity_itrs(r - h * e)) / (2 * h)
+            d2 = (c.gravity_itrs(r + 2 * h * e) - c.gravity_itrs(r - 2 * h * e)) / (4 * h)
+            fd[:, :, j] = (4 * d1 - d2) / 3
+    err = np.linalg.norm(G - fd, axis=(1, 2)) / np.linalg.norm(G, axis=(1, 2))
+    assert err.max() < 3e-11, err.max()
+
+
 def test_gravity_jacobian_point_mass_closed_form():
     r = _rand_points(128, 5)
     with ogp.Context(degree=1) as c:
