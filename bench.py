@@ -528,9 +528,8 @@ def leg_adjoint(ogp, scenarios):
     rel = np.linalg.norm(J - Phi, axis=(1, 2)) / np.linalg.norm(Phi, axis=(1, 2))
     gj = float(np.maximum(steps - 4, 0).sum())
     out = dict(workload="10,000 LEO, 60 s, 1,440 steps (24 h), JGM3 20x20, up to 7 corrector "
-                        "evaluations per step (the propagation stops at the corrector's fixed "
-                        "point, the record pass runs all 7); Jacobians recorded in the forward "
-                        "pass",
+                        "evaluations per step (the propagation and the record pass stop at the "
+                        "corrector's fixed point); Jacobians recorded in the forward pass",
                forward_ms=fwd_ms,
                jacobian_ms=ta["forward_ms"] + ta["adjoint_ms"],
                jacobian_forward_ms=ta["forward_ms"], jacobian_adjoint_ms=ta["adjoint_ms"],
@@ -549,8 +548,8 @@ def leg_adjoint(ogp, scenarios):
                    / PEAK_FP64_TFLOPS,
                    basis="algorithmic FLOPs of the GJ8 steps (executed evaluations) over the "
                          "whole propagation (startup included)"),
-               # the Jacobian's forward pass is the record pass over the whole span (its
-               # variational startup included in the time)
+               # the Jacobian's forward pass is the record pass over the whole span and the
+               # gradient evaluations at its points (the recording startup included in the time)
                record_hw_fp64=hw_rate("gj8_record_adjoint10k_24h_deg20_counters.md",
                                       "hw_fp64_flop_per_record_object_step", gj,
                                       ta["forward_ms"] * 1e-3, PEAK_FP64_TFLOPS))
