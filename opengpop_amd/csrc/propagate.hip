@@ -146,13 +146,18 @@ __device__ void acc_matrix(const ForceCtx& f, const Slots& nr, const Slots& nv, 
 
 // StartupCorrector::Correct (gpu_startup.cu:247-294) with CalculateS0 (:170-195) and
 // CalculateAllSKernel (calculate_all_s.cu:19-47). With fixed_point_exit, when the corrected
-// nodes are bit for bit the nodes the acceleration matrix was evaluated at, for every lane that
-// runs, the matrix is not evaluated again and true is returned: every later correction would
-// repeat this one exactly (the caller stops).
+// nodes are bit for bit the nodes the acceleration matrix was evaluated at (the positions, and
+// the velocities for forces that read them), for every lane that runs, the matrix is not
+// evaluated again and true is returned: a correction depends only on the matrix and the fixed
+// node 4, so every later correction would repeat this one exactly (the caller stops).
 template <int KIND, int W>
 __device__ bool startup_correct(const ForceCtx& f, double h, const Slots& nr, const Slots& nv,
     const Slots& acc, const Slots& sm, const Slots& Sm, Exchange& ex, bool fixed_point_exit)
 {
+    // the nodes the next acceleration matrix reads: positions, and velocities for forces that
+    // depend on them (drag, the damped test force)
+    constexpr bool kGrav = KIND == kGravity || KIND == kGravitySplit || KIND == kGravityRecord;
+    const bool vel = !kGrav || (f.src.mask & kSourceDrag) != 0;
     bool same = true;
     const double invh = 1.0 / h, invh2 = 1.0 / (h * h), hh = h * h;
 #pragma unroll 1
@@ -179,7 +184,7 @@ __device__ bool startup_correct(const ForceCtx& f, double h, const Slots& nr, co
                 sp = fma(acc.get(k, c), c_gj[j][k], sp);
             }
             const double vn = (sm.get(j, c) + sv) * h, rn = (Sm.get(j, c) + sp) * hh;
-            same = same && same_bits(vn, nv.get(j, c)) && same_bits(rn, nr.get(j, c));
+            same = same && (!vel || same_bits(vn, nv.get(j, c))) && same_bits(rn, nr.get(j, c));
             nv.set(j, c, vn);
             nr.set(j, c, rn);
         }
@@ -465,11 +470,15 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
                     gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
                     to_itrs(M, r, ri);
                     // The corrected state is the one just evaluated: the next acceleration, and
-                    // every correction after it, would repeat this one bit for bit.
+                    // every correction after it, would repeat this one bit for bit. A correction
+                    // depends on the step's sums and the latest acceleration only, so with forces
+                    // of the position alone (gravity, Sun, Moon) the position decides; drag reads
+                    // the velocity as well.
+                    const bool vel = (f.src.mask & kSourceDrag) != 0;
                     bool same = true;
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
-                        same = same && same_bits(r[c], r0[c]) && same_bits(v[c], v0[c]);
+                        same = same && same_bits(r[c], r0[c]) && (!vel || same_bits(v[c], v0[c]));
                     if (fp_exit && wave_all(same || !live))
                         break;
                 }

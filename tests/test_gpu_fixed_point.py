@@ -49,16 +49,18 @@ def test_fixed_point_exit_bit_identical_every_shape(precision, degree):
         c.set_lockstep(0)
 
 
-def test_fixed_point_exit_with_sources_bit_identical():
-    # Sun, Moon and per-object drag: each evaluation is corrected, and a step ends when the
-    # corrected position and velocity repeat the evaluated ones
-    print("[fixed-point] sources", flush=True)
+@pytest.mark.parametrize("drag", [True, False])
+def test_fixed_point_exit_with_sources_bit_identical(drag):
+    # Sun, Moon and (per-object) drag: each evaluation is corrected, and a step ends when the
+    # corrected position repeats the evaluated one, and the velocity too when drag reads it
+    print(f"[fixed-point] sources, drag {drag}", flush=True)
     d = scenarios.mixed(200, seed=43)
     steps = 5 + (np.arange(200) * 29) % 120
     bc = np.linspace(0.002, 0.03, 200)
     with ogp.Context(degree=20) as c:
-        c.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON | ogp.SOURCE_DRAG, 0.01)
-        c.set_drag_coefficients(bc)
+        c.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON | (ogp.SOURCE_DRAG if drag else 0), 0.01)
+        if drag:
+            c.set_drag_coefficients(bc)
         for w in (1, 8):
             c.set_tuning(w, 32)
             r0, v0, t0 = _run(c, d, steps, False)
@@ -115,7 +117,7 @@ def test_fixed_point_exit_derivatives_bit_identical():
             assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("force", ["gravity", "sources"])
+@pytest.mark.parametrize("force", ["gravity", "sun-moon", "sources"])
 def test_fixed_point_exit_startup_bit_identical(force):
     # GpuStartup::Startup's work areas (nodes, s, S and the acceleration matrix): the startup
     # corrections stop once the corrected nodes repeat the evaluated ones, with the same results
@@ -125,6 +127,8 @@ def test_fixed_point_exit_startup_bit_identical(force):
     with ogp.Context(degree=20) as c:
         if force == "sources":
             c.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON | ogp.SOURCE_DRAG, 0.02)
+        elif force == "sun-moon":
+            c.set_sources(ogp.SOURCE_SUN | ogp.SOURCE_MOON, 0.0)
         for on in (False, True):
             c.set_fixed_point_exit(on)
             res = [c.startup(d["r"], d["v"], h) for h in (30.0, 300.0)]
