@@ -461,9 +461,23 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
                 to_gcrs(M, gi, anew);
                 gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
             } else {
+                // With drag the velocity settles an iteration after the position: with the
+                // fixed-point exit, an evaluation whose ITRS point repeats the last one (in every
+                // lane that stores the step) keeps that gravity, bit for bit the same, and
+                // recomputes only the sources.
+                double rg[3] = { 0.0, 0.0, 0.0 };
                 for (int it = 0; it < ncorr; ++it) {
-                    gravity(ri, gi);
-                    ++nev;
+                    bool same_point = fp_exit && it > 0;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        same_point = same_point && same_bits(ri[c], rg[c]);
+                    if (!wave_all(same_point || !live)) {
+                        gravity(ri, gi);
+                        ++nev;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            rg[c] = ri[c];
+                    }
                     to_gcrs(M, gi, anew);
                     add_sources(f.src, f.prop_tb + 6 * (k + 4), r, v, anew);
                     const double r0[3] = { r[0], r[1], r[2] }, v0[3] = { v[0], v[1], v[2] };
