@@ -32,6 +32,8 @@ complete propagation of that population. Side legs (same line):
   copy      a 1 GiB device copy, the calibration dispatch of the HBM byte counters (not default)
 """
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -53,6 +55,19 @@ PEAK_HBM_GBS = 8000.0
 
 def terms(L):
     return (L + 1) * (L + 2) // 2 - 3
+
+
+@contextlib.contextmanager
+def quiet_gc():
+    """A timed region without Python garbage-collection pauses: collected before, paused
+    during (a full collection with torch loaded takes tens of ms; the timed work is the C
+    library's)."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def flops_eval(L):
@@ -137,14 +152,15 @@ def leg_fp64(ogp, scenarios, torch, args):
         run()
         log(f"fp64 warmup {i + 1}/{args.warmup}: {ctx.timing()['total_ms']:.1f} ms")
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     tm = []
-    for i in range(args.steps):
-        run()
-        tm.append(ctx.timing())
-        log(f"fp64 step {i + 1}/{args.steps}: {tm[-1]['total_ms']:.1f} ms (device)")
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with quiet_gc():
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            run()
+            tm.append(ctx.timing())
+            log(f"fp64 step {i + 1}/{args.steps}: {tm[-1]['total_ms']:.1f} ms (device)")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     obj_steps = float(steps.sum())
     ms_per_step = wall * 1e3 / args.steps
     gj = tm[-1]["gj_object_steps"]
@@ -289,14 +305,27 @@ def leg_pairs(ogp, scenarios):
         ca.find_closest(threshold=10e3)
         ca = ogp.ClosestApproach(ctx, c4["step"])
         ca.initialize(c4["r"], c4["v"], c4["sets"])
-        t0 = time.perf_counter()
-        ca.propagate(c4["steps"])
-        t1 = time.perf_counter()
-        prop_evals = ctx.timing()["evaluations"]
-        res = ca.find_closest(threshold=10e3)
-        t2 = time.perf_counter()
+        with quiet_gc():
+            t0 = time.perf_counter()
+            ca.propagate(c4["steps"])
+            t1 = time.perf_counter()
+            prop_evals = ctx.timing()["evaluations"]
+            res = ca.find_closest(threshold=10e3)
+            t2 = time.perf_counter()
         t = ca.timing()
+        # the same search screening every tile
+        ctx.set_radial_filter(0)
+        res_all = ca.find_closest(threshold=10e3)
+        t_all = ca.timing()
+        ctx.set_radial_filter(1)
+    unfiltered = dict(screen_ms=t_all["screen_ms"],
+                      same_results=all(np.array_equal(getattr(res, f), getattr(res_all, f))
+                                       for f in ("a", "b", "step", "distance", "time", "pos_a",
+                                                 "pos_b")))
     pair_steps = t["pair_steps"]
+    # the pair-steps the screen kernels evaluated: the radial filter skips the tiles of
+    # radially separated objects, whose pairs cannot come within the threshold
+    screened = t["screened_pair_steps"]
     prop_gj = float(len(c4["r"])) * (c4["steps"] - 4)
     gpu = dict(a=np.asarray(res.a), b=np.asarray(res.b), step=np.asarray(res.step),
                distance=np.asarray(res.distance), time=np.asarray(res.time))
@@ -307,15 +336,21 @@ def leg_pairs(ogp, scenarios):
         pairs=int(t["pairs"]), pair_steps=pair_steps, candidates=int(t["candidates"]),
         results=len(res),
         screen_ms=t["screen_ms"], refine_ms=t["refine_ms"], propagate_ms=t["propagate_ms"],
-        pair_steps_per_s=pair_steps / (t["screen_ms"] * 1e-3),
+        screened_pair_steps=screened, radial_filter_skipped=1.0 - screened / pair_steps,
+        every_tile=unfiltered,
+        # evaluated pair-steps per second of screen time (sorting and gathering included)
+        pair_steps_per_s=screened / (t["screen_ms"] * 1e-3),
+        # every pair-step resolved (evaluated or skipped) per second of screen time
+        pair_steps_resolved_per_s=pair_steps / (t["screen_ms"] * 1e-3),
         pairs_per_s_end_to_end=t["pairs"] / (t2 - t0),
         propagate_wall_s=t1 - t0, find_wall_s=t2 - t1,
         screen_roofline=dict(bound="valu",
-                             achieved=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12,
+                             achieved=9 * screened / (t["screen_ms"] * 1e-3) / 1e12,
                              peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                             frac=9 * pair_steps / (t["screen_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                             frac=9 * screened / (t["screen_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                             basis="9 FLOPs per evaluated pair-step",
                              hw_fp64=hw_rate("screen_pairs20k_3d_counters.md",
-                                             "hw_fp64_flop_per_pair_step", pair_steps,
+                                             "hw_fp64_flop_per_pair_step", screened,
                                              t["screen_ms"] * 1e-3, PEAK_FP64_TFLOPS)),
         propagate_roofline=dict(
             bound="valu", unit="TFLOP/s", peak=PEAK_FP64_TFLOPS,
@@ -350,7 +385,9 @@ def leg_pairs_shapes(ogp, c4):
                 res = ca.find_closest(threshold=10e3)
                 t = ca.timing()
                 rec[label] = dict(screen_ms=t["screen_ms"],
-                                  pair_steps_per_s=t["pair_steps"] / (t["screen_ms"] * 1e-3))
+                                  pair_steps_per_s=t["screened_pair_steps"] / (t["screen_ms"] * 1e-3),
+                                  radial_filter_skipped=1.0 - t["screened_pair_steps"]
+                                  / t["pair_steps"])
                 cands.append((np.asarray(res.a).tobytes(), np.asarray(res.b).tobytes(),
                               np.asarray(res.step).tobytes()))
             rec["tuned_shape_first_set"] = "narrow" if ctx.get_screen_shape(sizes[0]) == 2 else "square"

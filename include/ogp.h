@@ -348,6 +348,16 @@ int ogp_conj_results(ogp_context* ctx, uint32_t* obj_a, uint32_t* obj_b, int32_t
 int ogp_conj_find_noalloc(ogp_context* ctx, double threshold_m, int64_t capacity,
     uint32_t* obj_a, uint32_t* obj_b, int32_t* step, double* distance, double* time,
     double* pos_a, double* pos_b, int64_t* count);
+/* Radial filter of threshold screens (no counterpart in the reference, which screens every
+ * pair). With the filter, each set's objects are sorted by their smallest and largest sampled
+ * radius and every pair tile whose two radius ranges lie more than the threshold apart is
+ * skipped (for such pairs |rA - rB| >= ||rA| - |rB|| exceeds the threshold at every sample, so
+ * none is a candidate). mode 0: screen every tile; 1 (the default): filter when there are
+ * hundreds of pairs per object, where skipping outweighs the sort; 2: always filter. The
+ * candidates, their steps and distances and the refined results are the same in every mode;
+ * ogp_conj_screen and threshold_m <= 0 always screen every pair. */
+int ogp_set_radial_filter(ogp_context* ctx, int mode);
+
 /* Tile shape of the conjunction screen (no counterpart in the reference, which runs one
  * 256-thread block per pair range): 1 = square tiles of 64 x 64 pairs, 2 = narrow tiles of 4
  * objects of the lower set against 256 of the higher ones, which stream the higher set's
@@ -367,6 +377,9 @@ int ogp_conj_trajectory(ogp_context* ctx, double* out, int64_t* steps);
 typedef struct ogp_conj_timing {
     double propagate_ms, screen_ms, refine_ms;
     double pair_steps, pairs, candidates;
+    /* pair-steps the screen kernels evaluated: pair_steps less the pairs of the tiles the radial
+     * filter skipped (ogp_set_radial_filter) */
+    double screened_pair_steps;
 } ogp_conj_timing;
 int ogp_conj_last_timing(const ogp_context* ctx, ogp_conj_timing* t);
 

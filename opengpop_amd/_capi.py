@@ -48,7 +48,8 @@ SOURCE_SUN, SOURCE_MOON, SOURCE_DRAG = 1, 2, 4
 
 class OgpConjTiming(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("propagate_ms", "screen_ms", "refine_ms", "pair_steps", "pairs", "candidates")]
+                ("propagate_ms", "screen_ms", "refine_ms", "pair_steps", "pairs", "candidates",
+                 "screened_pair_steps")]
 
 
 class OgpAdjointTiming(ctypes.Structure):
@@ -102,6 +103,7 @@ def lib():
         "ogp_set_fixed_point_exit": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_set_lockstep": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_set_screen_shape": (ctypes.c_int, [_P, ctypes.c_int]),
+        "ogp_set_radial_filter": (ctypes.c_int, [_P, ctypes.c_int]),
         "ogp_get_screen_shape": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int)]),
         "ogp_get_lockstep": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_int)]),
         "ogp_set_derivative_memory": (ctypes.c_int, [_P, _I64]),

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -145,6 +146,10 @@ struct ogp_context {
     std::vector<uint32_t> cj_sets;
     std::vector<int64_t> cj_set_off;
     DevBuf cj_traj, cj_r, cj_v, cj_cnt, cj_a, cj_b, cj_k, cj_d, cj_out, cj_mind, cj_mink;
+    // radially sorted input of threshold screens (sort_for_screen)
+    DevBuf cj_sorted, cj_rpart, cj_rlo, cj_rhi, cj_slo, cj_shi, cj_ids;
+    int radial_filter = 1; // 0 off, 1 when it pays (radial_filter_pays), 2 always
+    double cj_screened = 0; // pairs the last screen evaluated
     std::vector<uint32_t> res_a, res_b;
     std::vector<int32_t> res_k;
     std::vector<double> res_out;
@@ -1487,6 +1492,16 @@ int ogp_set_screen_shape(ogp_context* ctx, int shape)
     return OGP_OK;
 }
 
+int ogp_set_radial_filter(ogp_context* ctx, int on)
+{
+    if (!ctx)
+        return fail(nullptr, OGP_ERR_ARGUMENT, "null context");
+    if (on < 0 || on > 2)
+        return fail(ctx, OGP_ERR_ARGUMENT, "radial filter must be 0 (off), 1 (automatic) or 2 (always)");
+    ctx->radial_filter = on;
+    return OGP_OK;
+}
+
 int ogp_get_screen_shape(const ogp_context* ctx, int64_t na, int* shape)
 {
     if (!ctx || !shape)
@@ -2182,6 +2197,105 @@ void tune_screen(ogp_context* ctx, int64_t a0, int64_t na, int64_t b0, int64_t n
     ctx->tuned_screen[ogp_context::screen_bucket(na)] = ms[1] < ms[0] ? kScreenNarrow : kScreenSquare;
 }
 
+// Radially sorted input of a threshold screen (ScreenSorted): every object's smallest and
+// largest sampled radius, the objects of each set ordered by the lower radius in 100 km bins
+// and by the upper radius within a bin (so that a 64-object tile spans a narrow range of both),
+// and the positions gathered in that order. slo / shi receive the bounds in slot order.
+ScreenSorted sort_for_screen(ogp_context* ctx, std::vector<double>& slo, std::vector<double>& shi)
+{
+    const int64_t n = ctx->cj_n, nsteps = ctx->cj_steps;
+    ctx->cj_rpart.reserve(sizeof(double) * 2 * kRadialChunks * n, &ctx->dev_bytes);
+    ctx->cj_rlo.reserve(sizeof(double) * n, &ctx->dev_bytes);
+    ctx->cj_rhi.reserve(sizeof(double) * n, &ctx->dev_bytes);
+    ctx->cj_slo.reserve(sizeof(double) * n, &ctx->dev_bytes);
+    ctx->cj_shi.reserve(sizeof(double) * n, &ctx->dev_bytes);
+    ctx->cj_ids.reserve(sizeof(uint32_t) * n, &ctx->dev_bytes);
+    ctx->cj_sorted.reserve(sizeof(double) * 3 * n * nsteps, &ctx->dev_bytes);
+    launch_radial_bounds(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps,
+        ctx->cj_rpart.as<double>(), ctx->cj_rlo.as<double>(), ctx->cj_rhi.as<double>());
+    ctx->check_launch();
+    std::vector<double> lo(n), hi(n);
+    HIPC(hipMemcpyAsync(lo.data(), ctx->cj_rlo.p, sizeof(double) * n, hipMemcpyDeviceToHost,
+        ctx->stream));
+    HIPC(hipMemcpyAsync(hi.data(), ctx->cj_rhi.p, sizeof(double) * n, hipMemcpyDeviceToHost,
+        ctx->stream));
+    ctx->sync();
+    std::vector<uint32_t> ids(n);
+    std::iota(ids.begin(), ids.end(), 0u);
+    std::vector<int64_t> bounds = ctx->cj_set_off;
+    bounds.push_back(n);
+    for (size_t s = 0; s + 1 < bounds.size(); ++s)
+        std::sort(ids.begin() + bounds[s], ids.begin() + bounds[s + 1], [&](uint32_t x, uint32_t y) {
+            const double bx = std::floor(lo[x] / 1e5), by = std::floor(lo[y] / 1e5);
+            if (bx != by)
+                return bx < by;
+            return hi[x] != hi[y] ? hi[x] < hi[y] : x < y;
+        });
+    slo.resize(n);
+    shi.resize(n);
+    for (int64_t s = 0; s < n; ++s) {
+        slo[s] = lo[ids[s]];
+        shi[s] = hi[ids[s]];
+    }
+    HIPC(hipMemcpyAsync(ctx->cj_ids.p, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice,
+        ctx->stream));
+    launch_gather_sorted(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps,
+        ctx->cj_ids.as<uint32_t>(), ctx->cj_rlo.as<double>(), ctx->cj_rhi.as<double>(),
+        ctx->cj_sorted.as<double>(), ctx->cj_slo.as<double>(), ctx->cj_shi.as<double>());
+    ctx->check_launch();
+    ctx->sync(); // ids leaves host memory
+    ScreenSorted srt;
+    srt.pos = ctx->cj_sorted.as<double>();
+    srt.rlo = ctx->cj_slo.as<double>();
+    srt.rhi = ctx->cj_shi.as<double>();
+    srt.ids = ctx->cj_ids.as<uint32_t>();
+    return srt;
+}
+
+// The pairs of rectangle A x B that the screen kernels of this shape evaluate on sorted input:
+// the tiles the kernels do not skip (the same tiles, bounds and comparison as the kernels).
+double screened_pairs(const std::vector<double>& slo, const std::vector<double>& shi, int64_t a0,
+    int64_t na, int64_t b0, int64_t nb, int shape, double threshold)
+{
+    const int64_t ta = shape == kScreenNarrow ? 4 : 64, tb = shape == kScreenNarrow ? 256 : 64;
+    auto range = [&](int64_t first, int64_t count, double& lo, double& hi) {
+        lo = std::numeric_limits<double>::infinity();
+        hi = -lo;
+        for (int64_t s = first; s < first + count; ++s) {
+            lo = std::fmin(lo, slo[s]);
+            hi = std::fmax(hi, shi[s]);
+        }
+    };
+    std::vector<double> blo, bhi;
+    for (int64_t j = 0; j < nb; j += tb) {
+        double lo, hi;
+        range(b0 + j, std::min(tb, nb - j), lo, hi);
+        blo.push_back(lo);
+        bhi.push_back(hi);
+    }
+    double pairs = 0;
+    for (int64_t i = 0; i < na; i += ta) {
+        const int64_t ca = std::min(ta, na - i);
+        double alo, ahi;
+        range(a0 + i, ca, alo, ahi);
+        for (size_t t = 0; t < blo.size(); ++t)
+            if (!(std::fmax(alo - bhi[t], blo[t] - ahi) > threshold + kRadialMargin))
+                pairs += (double)ca * (double)std::min(tb, nb - (int64_t)t * tb);
+    }
+    return pairs;
+}
+
+// Whether sorting for the radial filter pays: the sort reads the trajectory twice and writes
+// the positions once (about 100 B per object and step, plus a host sort), while the screen
+// costs about 0.3 ns per pair and step and the filter skipped 45% of the pairs of 20,000 LEO
+// objects. It pays with hundreds of pairs per object: 36 M pairs of 20,000 objects over 4,320
+// steps screen in 33 instead of 47 ms; one object against 19,999 took 6.0 ms sorted against
+// 1.3 ms unsorted.
+bool radial_filter_pays(int64_t pairs, int64_t n, int64_t nsteps)
+{
+    return pairs >= 256 * n && (double)pairs * (double)nsteps >= 2e10;
+}
+
 // Screens every set rectangle; all_pairs writes the per-pair minima, threshold collects
 // candidates. Returns the number of candidates.
 int64_t conj_screen(ogp_context* ctx, double threshold, bool all_pairs)
@@ -2195,6 +2309,13 @@ int64_t conj_screen(ogp_context* ctx, double threshold, bool all_pairs)
         ctx->cj_mink.reserve(sizeof(int32_t) * std::max<int64_t>(np, 1), &ctx->dev_bytes);
     }
     int64_t cap = std::max<int64_t>(1 << 16, ctx->cj_a.bytes / sizeof(uint32_t));
+    // threshold screens skip the tiles of radially separated objects (ogp_set_radial_filter)
+    ScreenSorted srt;
+    std::vector<double> slo, shi;
+    if (!all_pairs && threshold > 0.0
+        && (ctx->radial_filter == 2
+            || (ctx->radial_filter == 1 && radial_filter_pays(np, n, nsteps))))
+        srt = sort_for_screen(ctx, slo, shi);
     for (int attempt = 0; attempt < 2; ++attempt) {
         ctx->cj_cnt.reserve(sizeof(uint32_t), &ctx->dev_bytes);
         ctx->cj_a.reserve(sizeof(uint32_t) * cap, &ctx->dev_bytes);
@@ -2204,17 +2325,21 @@ int64_t conj_screen(ogp_context* ctx, double threshold, bool all_pairs)
         const unsigned long long zero = 0;
         HIPC(hipMemcpyAsync(ctx->cj_cnt.p, &zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream));
         int64_t pair0 = 0;
+        ctx->cj_screened = 0;
         for (size_t s = 0; s + 1 < off.size(); ++s) {
             const int64_t a0 = off[s], na = off[s + 1] - off[s];
             const int64_t b0 = off[s + 1], nb = n - off[s + 1];
             tune_screen(ctx, a0, na, b0, nb);
+            const int shape = ctx->screen_shape(na);
             launch_screen(ctx->stream, ctx->cj_traj.as<double>(), n, nsteps, a0, na, b0, nb, pair0,
                 all_pairs ? 0.0 : threshold, all_pairs ? ctx->cj_mind.as<double>() : nullptr,
                 all_pairs ? ctx->cj_mink.as<int32_t>() : nullptr,
                 ctx->cj_cnt.as<unsigned long long>(), cap,
                 ctx->cj_a.as<uint32_t>(), ctx->cj_b.as<uint32_t>(), ctx->cj_k.as<int32_t>(),
-                ctx->cj_d.as<double>(), ctx->screen_shape(na));
+                ctx->cj_d.as<double>(), shape, 0, srt);
             ctx->check_launch();
+            ctx->cj_screened += srt.pos ? screened_pairs(slo, shi, a0, na, b0, nb, shape, threshold)
+                                        : (double)na * (double)nb;
             pair0 += na * nb;
         }
         unsigned long long cnt = 0;
@@ -2333,6 +2458,7 @@ int ogp_conj_find(ogp_context* ctx, double threshold_m, int64_t* count)
         ctx->cj_timing.pairs = (double)np;
         ctx->cj_timing.pair_steps = (double)np * (double)ctx->cj_steps;
         ctx->cj_timing.candidates = (double)nc;
+        ctx->cj_timing.screened_pair_steps = ctx->cj_screened * (double)ctx->cj_steps;
         *count = nc;
     });
 }

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "conjunction.hpp"
@@ -51,15 +52,32 @@ __device__ __forceinline__ double min_f64(double a, double b)
 static_assert(TA == 64 && TB == 64 && THREADS == 256 && STAGE_ELEMS * 4 == STAGE * 3 * 2,
     "stage rows assume 64-object tiles and 4 waves");
 __device__ __forceinline__ double stage_load(const double* __restrict__ traj, int64_t N,
-    int64_t nsteps, int64_t k0, int row, int lane, int64_t ta0, int64_t tb0, int64_t alim,
-    int64_t blim)
+    int64_t sstride, int64_t nsteps, int64_t k0, int row, int lane, int64_t ta0, int64_t tb0,
+    int64_t alim, int64_t blim)
 {
     const int which = row / 24, r = row % 24;
     const int64_t k = k0 + r / 3;
     const int64_t obj = (which == 0 ? ta0 : tb0) + lane;
     if (k < nsteps && obj < (which == 0 ? alim : blim))
-        return traj[(k * 9 + r % 3) * N + obj];
+        return traj[k * sstride + (r % 3) * N + obj];
     return 0.0;
+}
+
+// Smallest and largest of the wave's values (every lane gets them).
+__device__ __forceinline__ void wave_min_max(double& lo, double& hi)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, m));
+        hi = fmax(hi, __shfl_xor(hi, m));
+    }
+}
+
+// Whether radius ranges [alo, ahi] and [blo, bhi] lie more than the threshold apart.
+__device__ __forceinline__ bool radially_apart(double alo, double ahi, double blo, double bhi,
+    double threshold)
+{
+    return fmax(alo - bhi, blo - ahi) > threshold + kRadialMargin;
 }
 }
 
@@ -92,10 +110,12 @@ template <int WIN>
 __global__ void __launch_bounds__(THREADS)
 __attribute__((amdgpu_waves_per_eu(OGP_SCREEN_WAVES_PER_EU)))
 screen_kernel(const double* __restrict__ traj,
-    int64_t N, int64_t nsteps, int64_t a0, int64_t na, int64_t b0, int64_t nb, int64_t pair0,
-    int64_t row_len, double threshold, double* __restrict__ min_dist, int32_t* __restrict__ min_step,
-    unsigned long long* __restrict__ cand_count, int64_t cand_cap, uint32_t* __restrict__ cand_a,
-    uint32_t* __restrict__ cand_b, int32_t* __restrict__ cand_k, double* __restrict__ cand_d)
+    int64_t N, int64_t sstride, int64_t nsteps, int64_t a0, int64_t na, int64_t b0, int64_t nb,
+    int64_t pair0, int64_t row_len, double threshold, double* __restrict__ min_dist,
+    int32_t* __restrict__ min_step, unsigned long long* __restrict__ cand_count, int64_t cand_cap,
+    uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b, int32_t* __restrict__ cand_k,
+    double* __restrict__ cand_d, const double* __restrict__ rlo, const double* __restrict__ rhi,
+    const uint32_t* __restrict__ ids)
 {
     constexpr int SPAN = STAGE * WIN; // samples per decision window
     __shared__ double sA[STAGE][3][TA];
@@ -106,6 +126,24 @@ screen_kernel(const double* __restrict__ traj,
     const int64_t ta0 = a0 + (int64_t)blockIdx.y * TA;
     const int64_t tb0 = b0 + (int64_t)blockIdx.x * TB;
     const int64_t alim = a0 + na, blim = b0 + nb;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int lane = tid % 64;
+    if (rlo) { // radially sorted input: the tile's radius ranges (wave 0: A, wave 1: B)
+        __shared__ double bnd[4];
+        if (wave < 2) {
+            const int64_t o = (wave == 0 ? ta0 : tb0) + lane;
+            const bool ok = o < (wave == 0 ? alim : blim);
+            double lo = ok ? rlo[o] : __builtin_inf(), hi = ok ? rhi[o] : -__builtin_inf();
+            wave_min_max(lo, hi);
+            if (lane == 0) {
+                bnd[2 * wave] = lo;
+                bnd[2 * wave + 1] = hi;
+            }
+        }
+        __syncthreads();
+        if (radially_apart(bnd[0], bnd[1], bnd[2], bnd[3], threshold))
+            return; // the whole block: no pair of the tile comes within the threshold
+    }
     int bcol[PB];
 #pragma unroll
     for (int j = 0; j < PB; ++j)
@@ -120,12 +158,10 @@ screen_kernel(const double* __restrict__ traj,
             mn[i][j] = __builtin_inf();
             sk[i][j] = -1;
         }
-    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
-    const int lane = tid % 64;
     double pre[STAGE_ELEMS];
 #pragma unroll
     for (int q = 0; q < STAGE_ELEMS; ++q)
-        pre[q] = stage_load(traj, N, nsteps, 0, 4 * q + wave, lane, ta0, tb0, alim, blim);
+        pre[q] = stage_load(traj, N, sstride, nsteps, 0, 4 * q + wave, lane, ta0, tb0, alim, blim);
     for (int64_t k0 = 0; k0 < nsteps; k0 += STAGE) {
         __syncthreads(); // the previous stage has been screened by every thread
 #pragma unroll
@@ -138,8 +174,8 @@ screen_kernel(const double* __restrict__ traj,
         if (k0 + STAGE < nsteps) { // flies while this stage is screened
 #pragma unroll
             for (int q = 0; q < STAGE_ELEMS; ++q)
-                pre[q] = stage_load(traj, N, nsteps, k0 + STAGE, 4 * q + wave, lane, ta0, tb0,
-                    alim, blim);
+                pre[q] = stage_load(traj, N, sstride, nsteps, k0 + STAGE, 4 * q + wave, lane, ta0,
+                    tb0, alim, blim);
         }
         if (k0 % SPAN == 0) {
 #pragma unroll
@@ -229,7 +265,7 @@ screen_kernel(const double* __restrict__ traj,
                 const int64_t kb = (int64_t)sk[i][j] * SPAN;
                 const int64_t ke = kb + SPAN < nsteps ? kb + SPAN : nsteps;
                 for (int64_t kk = kb; kk < ke; ++kk) {
-                    const double* p = traj + kk * 9 * N;
+                    const double* p = traj + kk * sstride;
                     const double dx = p[ia] - p[jb];
                     const double dy = p[N + ia] - p[N + jb];
                     const double dz = p[2 * N + ia] - p[2 * N + jb];
@@ -248,8 +284,8 @@ screen_kernel(const double* __restrict__ traj,
             if (cand) {
                 const unsigned long long slot = atomicAdd(cand_count, 1ull);
                 if (slot < (unsigned long long)cand_cap) {
-                    cand_a[slot] = (uint32_t)ia;
-                    cand_b[slot] = (uint32_t)jb;
+                    cand_a[slot] = ids ? ids[ia] : (uint32_t)ia;
+                    cand_b[slot] = ids ? ids[jb] : (uint32_t)jb;
                     cand_k[slot] = k;
                     cand_d[slot] = d;
                 }
@@ -267,11 +303,13 @@ screen_kernel(const double* __restrict__ traj,
 constexpr int NA = 4, NB_THREADS = 256;
 template <int WIN>
 __global__ void __launch_bounds__(NB_THREADS)
-screen_narrow_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps, int64_t a0,
-    int64_t na, int64_t b0, int64_t nb, int64_t pair0, int64_t row_len, double threshold,
-    double* __restrict__ min_dist, int32_t* __restrict__ min_step,
-    unsigned long long* __restrict__ cand_count, int64_t cand_cap, uint32_t* __restrict__ cand_a,
-    uint32_t* __restrict__ cand_b, int32_t* __restrict__ cand_k, double* __restrict__ cand_d)
+screen_narrow_kernel(const double* __restrict__ traj, int64_t N, int64_t sstride,
+    int64_t nsteps, int64_t a0, int64_t na, int64_t b0, int64_t nb, int64_t pair0,
+    int64_t row_len, double threshold, double* __restrict__ min_dist,
+    int32_t* __restrict__ min_step, unsigned long long* __restrict__ cand_count, int64_t cand_cap,
+    uint32_t* __restrict__ cand_a, uint32_t* __restrict__ cand_b, int32_t* __restrict__ cand_k,
+    double* __restrict__ cand_d, const double* __restrict__ rlo, const double* __restrict__ rhi,
+    const uint32_t* __restrict__ ids)
 {
     constexpr int SPAN = STAGE * WIN;
     __shared__ double sA[STAGE][3][NA];
@@ -281,12 +319,35 @@ screen_narrow_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps,
     const int64_t jb = b0 + (int64_t)blockIdx.x * NB_THREADS + tid;
     const bool bvalid = jb < blim;
     const int64_t jl = bvalid ? jb : b0; // lanes past set B read object b0 and write nothing
+    if (rlo) { // radially sorted input: the block's radius ranges (A: NA objects, B: its threads)
+        __shared__ double bnd[NB_THREADS / 64][2];
+        double lo = bvalid ? rlo[jb] : __builtin_inf(), hi = bvalid ? rhi[jb] : -__builtin_inf();
+        wave_min_max(lo, hi);
+        if (tid % 64 == 0) {
+            bnd[tid / 64][0] = lo;
+            bnd[tid / 64][1] = hi;
+        }
+        __syncthreads();
+        double alo = __builtin_inf(), ahi = -__builtin_inf();
+        for (int i = 0; i < NA; ++i)
+            if (ta0 + i < alim) {
+                alo = fmin(alo, rlo[ta0 + i]);
+                ahi = fmax(ahi, rhi[ta0 + i]);
+            }
+        double blo = bnd[0][0], bhi = bnd[0][1];
+        for (int w = 1; w < NB_THREADS / 64; ++w) {
+            blo = fmin(blo, bnd[w][0]);
+            bhi = fmax(bhi, bnd[w][1]);
+        }
+        if (radially_apart(alo, ahi, blo, bhi, threshold))
+            return; // the whole block: no pair of its objects comes within the threshold
+    }
     // A stage element e (< STAGE * 3 * NA): step e / (3 NA), component (e / NA) % 3, object e % NA
     auto a_load = [&](int64_t k0, int e) {
         const int64_t k = k0 + e / (3 * NA);
         const int c = (e / NA) % 3;
         const int64_t ia = ta0 + e % NA;
-        return k < nsteps && ia < alim ? traj[(k * 9 + c) * N + ia] : 0.0;
+        return k < nsteps && ia < alim ? traj[k * sstride + c * N + ia] : 0.0;
     };
     auto b_load = [&](int64_t k0, double* dst) {
 #pragma unroll
@@ -294,7 +355,7 @@ screen_narrow_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps,
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int64_t k = k0 + st;
-                dst[3 * st + c] = k < nsteps ? traj[(k * 9 + c) * N + jl] : 0.0;
+                dst[3 * st + c] = k < nsteps ? traj[k * sstride + c * N + jl] : 0.0;
             }
     };
     double mn[NA], wm[NA];
@@ -382,7 +443,7 @@ screen_narrow_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps,
             const int64_t kb = (int64_t)sk[i] * SPAN;
             const int64_t ke = kb + SPAN < nsteps ? kb + SPAN : nsteps;
             for (int64_t kk = kb; kk < ke; ++kk) {
-                const double* p = traj + kk * 9 * N;
+                const double* p = traj + kk * sstride;
                 const double dx = p[ia] - p[jb];
                 const double dy = p[N + ia] - p[N + jb];
                 const double dz = p[2 * N + ia] - p[2 * N + jb];
@@ -401,8 +462,8 @@ screen_narrow_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps,
         if (cand) {
             const unsigned long long slot = atomicAdd(cand_count, 1ull);
             if (slot < (unsigned long long)cand_cap) {
-                cand_a[slot] = (uint32_t)ia;
-                cand_b[slot] = (uint32_t)jb;
+                cand_a[slot] = ids ? ids[ia] : (uint32_t)ia;
+                cand_b[slot] = ids ? ids[jb] : (uint32_t)jb;
                 cand_k[slot] = k;
                 cand_d[slot] = d;
             }
@@ -413,34 +474,125 @@ screen_narrow_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps,
 void launch_screen(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, int64_t a0,
     int64_t na, int64_t b0, int64_t nb, int64_t pair0, double threshold, double* min_dist,
     int32_t* min_step, unsigned long long* cand_count, int64_t cand_cap, uint32_t* cand_a,
-    uint32_t* cand_b, int32_t* cand_k, double* cand_d, int shape, int64_t row_len)
+    uint32_t* cand_b, int32_t* cand_k, double* cand_d, int shape, int64_t row_len,
+    const ScreenSorted& sorted)
 {
     if (na <= 0 || nb <= 0 || nsteps <= 0)
         return;
     if (row_len <= 0)
         row_len = nb;
+    // the stored trajectory (9 N doubles per step, positions first) or the sorted positions
+    const bool srt = sorted.pos != nullptr && threshold > 0.0 && min_dist == nullptr;
+    const double* src = srt ? sorted.pos : traj;
+    const int64_t sstride = srt ? 3 * N : 9 * N;
+    const double* rlo = srt ? sorted.rlo : nullptr;
+    const double* rhi = srt ? sorted.rhi : nullptr;
+    const uint32_t* ids = srt ? sorted.ids : nullptr;
     // every pair written out: short windows; candidates only: long ones
     if (shape == kScreenNarrow) {
         const dim3 grid((unsigned)((nb + NB_THREADS - 1) / NB_THREADS), (unsigned)((na + NA - 1) / NA));
         if (min_dist)
-            hipLaunchKernelGGL(screen_narrow_kernel<1>, grid, dim3(NB_THREADS), 0, st, traj, N,
-                nsteps, a0, na, b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count,
-                cand_cap, cand_a, cand_b, cand_k, cand_d);
+            hipLaunchKernelGGL(screen_narrow_kernel<1>, grid, dim3(NB_THREADS), 0, st, src, N,
+                sstride, nsteps, a0, na, b0, nb, pair0, row_len, threshold, min_dist, min_step,
+                cand_count, cand_cap, cand_a, cand_b, cand_k, cand_d, rlo, rhi, ids);
         else
-            hipLaunchKernelGGL(screen_narrow_kernel<4>, grid, dim3(NB_THREADS), 0, st, traj, N,
-                nsteps, a0, na, b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count,
-                cand_cap, cand_a, cand_b, cand_k, cand_d);
+            hipLaunchKernelGGL(screen_narrow_kernel<4>, grid, dim3(NB_THREADS), 0, st, src, N,
+                sstride, nsteps, a0, na, b0, nb, pair0, row_len, threshold, min_dist, min_step,
+                cand_count, cand_cap, cand_a, cand_b, cand_k, cand_d, rlo, rhi, ids);
         return;
     }
     const dim3 grid((unsigned)((nb + TB - 1) / TB), (unsigned)((na + TA - 1) / TA));
     if (min_dist)
-        hipLaunchKernelGGL(screen_kernel<1>, grid, dim3(THREADS), 0, st, traj, N, nsteps, a0, na,
-            b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count, cand_cap, cand_a,
-            cand_b, cand_k, cand_d);
+        hipLaunchKernelGGL(screen_kernel<1>, grid, dim3(THREADS), 0, st, src, N, sstride, nsteps,
+            a0, na, b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count, cand_cap,
+            cand_a, cand_b, cand_k, cand_d, rlo, rhi, ids);
     else
-        hipLaunchKernelGGL(screen_kernel<4>, grid, dim3(THREADS), 0, st, traj, N, nsteps, a0, na,
-            b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count, cand_cap, cand_a,
-            cand_b, cand_k, cand_d);
+        hipLaunchKernelGGL(screen_kernel<4>, grid, dim3(THREADS), 0, st, src, N, sstride, nsteps,
+            a0, na, b0, nb, pair0, row_len, threshold, min_dist, min_step, cand_count, cand_cap,
+            cand_a, cand_b, cand_k, cand_d, rlo, rhi, ids);
+}
+
+// ------------------------------------------------------------------------------------------
+// Radially sorted input for threshold screens
+// ------------------------------------------------------------------------------------------
+// Squared radii of the samples of step chunk y (of kRadialChunks) for objects 64 x .. 64 x + 63:
+// the smallest into partial[y][i], the largest into partial[kRadialChunks + y][i]. A NaN sample
+// is passed over, as the screen passes over a NaN distance.
+__global__ void radial_partial_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps,
+    double* __restrict__ partial)
+{
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= N)
+        return;
+    const int64_t per = (nsteps + kRadialChunks - 1) / kRadialChunks;
+    const int64_t k0 = (int64_t)blockIdx.y * per;
+    const int64_t k1 = k0 + per < nsteps ? k0 + per : nsteps;
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    for (int64_t k = k0; k < k1; ++k) {
+        const double* p = traj + k * 9 * N;
+        const double x = p[i], y = p[N + i], z = p[2 * N + i];
+        const double r2 = __builtin_fma(z, z, __builtin_fma(y, y, x * x));
+        lo = fmin(lo, r2);
+        hi = fmax(hi, r2);
+    }
+    partial[(int64_t)blockIdx.y * N + i] = lo;
+    partial[(int64_t)(kRadialChunks + blockIdx.y) * N + i] = hi;
+}
+
+__global__ void radial_reduce_kernel(const double* __restrict__ partial, int64_t N,
+    double* __restrict__ rlo, double* __restrict__ rhi)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    for (int y = 0; y < kRadialChunks; ++y) {
+        lo = fmin(lo, partial[(int64_t)y * N + i]);
+        hi = fmax(hi, partial[(int64_t)(kRadialChunks + y) * N + i]);
+    }
+    rlo[i] = __builtin_sqrt(lo); // inf and -inf (no finite sample) leave every tile skippable
+    rhi[i] = hi >= 0.0 ? __builtin_sqrt(hi) : hi;
+}
+
+// Slot s of every step gets object ids[s]'s position; steps blockIdx.y, + gridDim.y, ...
+__global__ void gather_sorted_kernel(const double* __restrict__ traj, int64_t N, int64_t nsteps,
+    const uint32_t* __restrict__ ids, const double* __restrict__ rlo,
+    const double* __restrict__ rhi, double* __restrict__ pos, double* __restrict__ slo,
+    double* __restrict__ shi)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N)
+        return;
+    const int64_t o = ids[s];
+    if (blockIdx.y == 0) {
+        slo[s] = rlo[o];
+        shi[s] = rhi[o];
+    }
+    for (int64_t k = blockIdx.y; k < nsteps; k += gridDim.y)
+        for (int c = 0; c < 3; ++c)
+            pos[(k * 3 + c) * N + s] = traj[(k * 9 + c) * N + o];
+}
+
+void launch_radial_bounds(hipStream_t st, const double* traj, int64_t N, int64_t nsteps,
+    double* partial, double* rlo, double* rhi)
+{
+    if (N <= 0 || nsteps <= 0)
+        return;
+    hipLaunchKernelGGL(radial_partial_kernel, dim3((unsigned)((N + 63) / 64), kRadialChunks),
+        dim3(64), 0, st, traj, N, nsteps, partial);
+    hipLaunchKernelGGL(radial_reduce_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
+        partial, N, rlo, rhi);
+}
+
+void launch_gather_sorted(hipStream_t st, const double* traj, int64_t N, int64_t nsteps,
+    const uint32_t* ids, const double* rlo, const double* rhi, double* pos, double* slo,
+    double* shi)
+{
+    if (N <= 0 || nsteps <= 0)
+        return;
+    const unsigned gy = (unsigned)std::min<int64_t>(nsteps, 4096);
+    hipLaunchKernelGGL(gather_sorted_kernel, dim3((unsigned)((N + 255) / 256), gy), dim3(256), 0,
+        st, traj, N, nsteps, ids, rlo, rhi, pos, slo, shi);
 }
 
 // ------------------------------------------------------------------------------------------

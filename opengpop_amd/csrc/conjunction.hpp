@@ -13,11 +13,35 @@ namespace ogp {
 // shape: kScreenSquare (64 x 64 pair tiles) or kScreenNarrow (4 A objects x 256 B objects);
 // the results are the same bits.
 enum { kScreenSquare = 1, kScreenNarrow = 2 };
+// A threshold screen on radially sorted input: the positions of every object reordered within
+// its set (pos[step][3][N], 3 N doubles per step), each slot's smallest and largest sampled
+// radius, and the object index each slot holds. A tile whose A and B objects' radius ranges lie
+// more than the threshold (plus kRadialMargin) apart holds no pair closer than the threshold at
+// any sample (|rA - rB| >= ||rA| - |rB||) and is skipped; candidates are reported with the
+// object indices. Only with threshold > 0 and no per-pair minima.
+struct ScreenSorted {
+    const double* pos = nullptr; // null: screen the stored trajectory, every tile
+    const double* rlo = nullptr;
+    const double* rhi = nullptr;
+    const uint32_t* ids = nullptr;
+};
+// Far above the rounding of the radii and of the squared distances (~1e-9 m at 10^7 m).
+constexpr double kRadialMargin = 1e-3;
 void launch_screen(hipStream_t st, const double* traj, int64_t N, int64_t nsteps, int64_t a0,
     int64_t na, int64_t b0, int64_t nb, int64_t pair0, double threshold, double* min_dist,
     int32_t* min_step, unsigned long long* cand_count, int64_t cand_cap, uint32_t* cand_a,
     uint32_t* cand_b, int32_t* cand_k, double* cand_d, int shape = kScreenSquare,
-    int64_t row_len = 0);
+    int64_t row_len = 0, const ScreenSorted& sorted = ScreenSorted {});
+
+// The smallest and largest radius of each object over the samples [0, nsteps) of traj:
+// partial[2][chunks][N] is scratch for kRadialChunks step chunks.
+constexpr int kRadialChunks = 16;
+void launch_radial_bounds(hipStream_t st, const double* traj, int64_t N, int64_t nsteps,
+    double* partial, double* rlo, double* rhi);
+// pos[k][c][s] = traj[k][c][ids[s]] (positions) and the bounds in slot order.
+void launch_gather_sorted(hipStream_t st, const double* traj, int64_t N, int64_t nsteps,
+    const uint32_t* ids, const double* rlo, const double* rhi, double* pos, double* slo,
+    double* shi);
 
 // Refines n candidates (a, b, sampled minimum step k and distance d). out: distance, time
 // (t0 + seconds), position of a, position of b at the TCA; field_stride = 0: 8 doubles per

@@ -222,6 +222,12 @@ class Context:
         """Conjunction screen tiles: 0 tuned, 1 square (64 x 64), 2 narrow (4 x 256)."""
         check(self.h, lib().ogp_set_screen_shape(self.h, shape))
 
+    def set_radial_filter(self, mode=1):
+        """Threshold screens skip the pair tiles whose objects' sampled radius ranges lie more
+        than the threshold apart: 0 off, 1 when there are hundreds of pairs per object (the
+        default), 2 always. Results are the same in every mode."""
+        check(self.h, lib().ogp_set_radial_filter(self.h, int(mode)))
+
     def get_screen_shape(self, na):
         g = ctypes.c_int()
         check(self.h, lib().ogp_get_screen_shape(self.h, na, ctypes.byref(g)))
@@ -476,7 +482,7 @@ class ClosestApproachData:
     def __init__(self, a, b, k, d, t, pa, pb):
         self.a, self.b, self.step, self.distance, self.time = a, b, k, d, t
         self.pos_a, self.pos_b = pa, pb
-        self._index = {(int(x), int(y)): q for q, (x, y) in enumerate(zip(a, b))}
+        self._index = None  # (a, b) -> record, built on the first get()
 
     def __len__(self):
         return len(self.a)
@@ -485,6 +491,8 @@ class ClosestApproachData:
         if i == j:
             return dict(distance=0.0, time=0.0, pos_a=np.zeros(3), pos_b=np.zeros(3))
         swap = i > j
+        if self._index is None:
+            self._index = {(int(x), int(y)): q for q, (x, y) in enumerate(zip(self.a, self.b))}
         q = self._index.get((min(i, j), max(i, j)))
         if q is None:
             return dict(distance=-1.0, time=-1.0, pos_a=-np.ones(3), pos_b=-np.ones(3))

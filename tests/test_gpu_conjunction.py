@@ -224,6 +224,52 @@ def test_screen_tile_shapes_bit_identical(sizes, po):
     assert np.array_equal(d1, ref_d) and np.array_equal(k1, ref_k)
 
 
+@pytest.mark.parametrize("sizes", [(1, 1199), (3, 597, 600), (70, 1130), (600, 600)])
+@pytest.mark.parametrize("threshold", [20e3, 500e3])
+def test_radial_filter_same_candidates(sizes, threshold, po):
+    # threshold screens with the radial filter skip the tiles of radially separated objects:
+    # the candidates, steps and distances equal the oracle's per-pair screen of the same
+    # trajectory below the threshold, and every record equals the unfiltered search's
+    print(f"[radial filter] sets {sizes} threshold {threshold}", flush=True)
+    n = sum(sizes)
+    r, v = scenarios.leo(n, seed=53)
+    j = sizes[0]  # near neighbours of object 0 at the start of set 1
+    r[j:j + 3] = r[0] + [[50.0, 0, 0], [0, 8e3, 0], [0, 0, 15e3]]
+    v[j:j + 3] = v[0]
+    sets = np.repeat(np.arange(len(sizes), dtype=np.uint32), sizes)
+    out, tim = {}, {}
+    with ogp.Context(degree=20) as c:
+        ca = ogp.ClosestApproach(c, 60.0)
+        ca.initialize(r, v, sets)
+        ca.propagate(77)
+        traj = ca.trajectory()
+        for mode in (2, 0, 1):  # always, never, automatic
+            c.set_radial_filter(mode)
+            out[mode] = ca.find_closest(threshold=threshold)
+            tim[mode] = ca.timing()
+    ref_d, ref_k = po.Oracle(degree=2).screen(traj, sets)
+    pa, pb = [], []
+    for s in range(len(sizes)):
+        for i in range(sum(sizes[:s]), sum(sizes[:s + 1])):
+            for j in range(sum(sizes[:s + 1]), n):
+                pa.append(i)
+                pb.append(j)
+    sel = ref_d < threshold
+    f = out[2]
+    assert len(f) == int(sel.sum()) > 0
+    assert np.array_equal(f.a, np.asarray(pa)[sel]) and np.array_equal(f.b, np.asarray(pb)[sel])
+    assert np.array_equal(f.step, ref_k[sel])
+    for a in ("a", "b", "step", "distance", "time", "pos_a", "pos_b"):
+        assert np.array_equal(getattr(f, a), getattr(out[0], a)), a
+    assert tim[0]["screened_pair_steps"] == tim[0]["pair_steps"]
+    # automatic: too few pairs per object here for the sort to pay, so every tile is screened
+    assert tim[1]["screened_pair_steps"] == tim[1]["pair_steps"]
+    for a in ("a", "b", "step", "distance", "time", "pos_a", "pos_b"):
+        assert np.array_equal(getattr(out[1], a), getattr(out[0], a)), a
+    if threshold == 20e3:  # some tiles lie radially apart (few where a 64-row tile spans a set)
+        assert tim[2]["screened_pair_steps"] < tim[2]["pair_steps"]
+
+
 def test_screen_tuner_records_choice():
     # a one-object set against 2,047: the tuner times both shapes once and keeps one; the
     # results do not depend on it
