@@ -43,11 +43,16 @@ template <class Get> __device__ __forceinline__ void gj8_sums_of(Get&& get, GJ8S
     }
 }
 
-// The ring in memory: a_j at slot ring_slot(j), hist[(slot * 3 + c) * N + i].
+// The ring in memory: a_j at slot ring_slot(j), hist[(slot * 3 + c) * N + i]. The slots of
+// a_{k-9} .. a_{k-1} follow ring_slot(k - 9) cyclically (one 64-bit remainder per step).
 __device__ __forceinline__ void gj8_sums(const double* hist, int64_t N, int64_t i, int64_t k,
     GJ8Sums& q)
 {
-    gj8_sums_of([&](int j, int c) { return hist[(ring_slot(k - 9 + j) * 3 + c) * N + i]; }, q);
+    const int s0 = ring_slot(k - 9);
+    gj8_sums_of([&](int j, int c) {
+        const int s = s0 + j >= 9 ? s0 + j - 9 : s0 + j;
+        return hist[(s * 3 + c) * N + i];
+    }, q);
 }
 
 // Predictor: S_{k} = S_{k-1} + s_{k-1} + a_{k-1}/2, predicted v and r; sprev keeps s_{k-1}.

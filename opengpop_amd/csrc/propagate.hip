@@ -326,10 +326,12 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
 // G object groups share a workgroup in lockstep (G x W waves, one exchange area per group): the
 // barriers are the workgroup's, so the waves summing the same harmonic part of different groups
 // read the same table lines at about the same time. The results do not depend on G.
+// SRC: gravity with acceleration sources (f.src.mask != 0), an instance of its own so that the
+// gravity-only kernel's registers are allocated for the gravity-only path.
 #ifndef OGP_CHUNK_WAVES_PER_EU
 #define OGP_CHUNK_WAVES_PER_EU 4 // resident waves per SIMD the register allocation aims for
 #endif
-template <int KIND, int W, int G>
+template <int KIND, int W, int G, bool SRC = false>
 __global__ void __launch_bounds__(64 * W * G)
 __attribute__((amdgpu_waves_per_eu(OGP_CHUNK_WAVES_PER_EU)))
 gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
@@ -437,7 +439,7 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
             };
             double ri[3], gi[3];
             to_itrs(M, r, ri);
-            if (f.src.mask == 0) {
+            if constexpr (!SRC) {
                 double base[3], t[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
@@ -592,6 +594,22 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
 #undef L_
 }
 
+template <int K, int Wv, int G>
+void launch_chunk_kernel(bool src, dim3 grid, hipStream_t st, const ForceCtx& f,
+    const PropState& ps, int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr,
+    double* traj, int64_t traj_first, int64_t traj_steps, int fp, unsigned long long* evals)
+{
+    if constexpr (K == kGravity || K == kGravitySplit) {
+        if (src) {
+            hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, G, true>), grid, dim3(64 * Wv * G), 0, st,
+                f, ps, obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp, evals);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, G, false>), grid, dim3(64 * Wv * G), 0, st, f,
+        ps, obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp, evals);
+}
+
 void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
     int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
     int64_t traj_first, int64_t traj_steps, int waves, int lockstep, bool fixed_point_exit,
@@ -604,12 +622,13 @@ void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& 
     const int64_t ngroups = (nobj + 63) / 64;
     const dim3 grid((unsigned)((ngroups + G_ - 1) / G_));
     const int fp_ = fixed_point_exit ? 1 : 0;
+    const bool src_ = f.src.mask != 0; // gravity with sources: the SRC instances
 #define L_(K, Wv)                                                                                  \
-    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, 1>), grid, dim3(64 * (Wv)), 0, st, f, ps, obj0,   \
-        nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp_, evals)
+    launch_chunk_kernel<K, Wv, 1>(src_, grid, st, f, ps, obj0, nobj, k0, k1, ncorr, traj,         \
+        traj_first, traj_steps, fp_, evals)
 #define L2_(K, Wv)                                                                                 \
-    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, 2>), grid, dim3(64 * 2 * (Wv)), 0, st, f, ps,      \
-        obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp_, evals)
+    launch_chunk_kernel<K, Wv, 2>(src_, grid, st, f, ps, obj0, nobj, k0, k1, ncorr, traj,         \
+        traj_first, traj_steps, fp_, evals)
     if (G_ == 2) {
         const bool split_ = kind == kGravitySplit;
         if (W_ == 4) {
