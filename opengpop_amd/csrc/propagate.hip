@@ -343,6 +343,13 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     __shared__ double pos[W > 1 ? G * kPointLen * 64 : 1]; // the leads' geometry
     __shared__ int64_t group_end[G];
     __shared__ int more; // one group: the lead's control word for its helpers
+    // gravity without sources, 4 or 8 waves: what the one correction after a step's evaluations
+    // needs (the corrector rows T1, T2, a_{k-1}, s_{k-1} and S), parked per lane while the lead
+    // evaluates, so that the evaluations have the registers instead of spilling them (lane-
+    // private: written and read by one wave). 7.5 KB per group: with 1 or 2 waves it would cost
+    // resident workgroups, so those shapes keep the values in registers.
+    constexpr bool kPark = (KIND == kGravity || KIND == kGravitySplit) && W >= 4;
+    __shared__ double park[kPark ? G * 15 * 64 : 1];
     const int wg = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
     const int g = wg / W, w = wg % W; // object group of the workgroup, wave within the group
     double* gred = red + (W > 1 ? g * ExchangeArea<W>::size : 0);
@@ -445,6 +452,17 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
                 for (int c = 0; c < 3; ++c)
                     t[c] = q.T2[c] + S[c];
                 to_itrs(M, t, base);
+                double* pk = park + (kPark ? g * 15 * 64 + threadIdx.x % 64 : 0);
+                if constexpr (kPark) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        pk[c * 64] = q.T1[c];
+                        pk[(3 + c) * 64] = q.T2[c];
+                        pk[(6 + c) * 64] = q.alast[c];
+                        pk[(9 + c) * 64] = sprev[c];
+                        pk[(12 + c) * 64] = S[c];
+                    }
+                }
                 for (int it = 0; it < ncorr; ++it) {
                     gravity(ri, gi);
                     ++nev;
@@ -461,7 +479,21 @@ gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
                         break;
                 }
                 to_gcrs(M, gi, anew);
-                gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+                if constexpr (kPark) {
+                    GJ8Sums qc;
+                    double sp[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        qc.T1[c] = pk[c * 64];
+                        qc.T2[c] = pk[(3 + c) * 64];
+                        qc.alast[c] = pk[(6 + c) * 64];
+                        sp[c] = pk[(9 + c) * 64];
+                        S[c] = pk[(12 + c) * 64];
+                    }
+                    gj8_correct(qc, anew, sp, S, h, hh, s, r, v);
+                } else {
+                    gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+                }
             } else {
                 // With drag the velocity settles an iteration after the position: with the
                 // fixed-point exit, an evaluation whose ITRS point repeats the last one (in every
