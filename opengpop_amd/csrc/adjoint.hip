@@ -260,6 +260,9 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
 {
     __shared__ double red[W > 1 ? kParts * 4 * 64 : 1];
     __shared__ double pos[W > 1 ? kPointLen * 64 : 1];
+    // the one correction's inputs, parked per lane during a step's evaluations (as in
+    // gj8_chunk_kernel: the evaluations get the registers instead of spilling them)
+    __shared__ double park[15 * 64];
     __shared__ int more; // the lead's control word for its helpers
     // Every barrier sits in workgroup-uniform control flow: the loops run to the group's last
     // step (the same in every wave, which hold the same objects); a lane past its span, or past
@@ -323,6 +326,14 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
         for (int c = 0; c < 3; ++c)
             t[c] = q.T2[c] + S[c];
         to_itrs(M, t, base);
+        double* pk = park + threadIdx.x % 64;
+        for (int c = 0; c < 3; ++c) {
+            pk[c * 64] = q.T1[c];
+            pk[(3 + c) * 64] = q.T2[c];
+            pk[(6 + c) * 64] = q.alast[c];
+            pk[(9 + c) * 64] = sprev[c];
+            pk[(12 + c) * 64] = S[c];
+        }
         for (int it = 0; it < ncorr; ++it) {
             announce(1);
             if constexpr (W > 1)
@@ -345,7 +356,18 @@ gj8_record_kernel(ForceCtx f, PropState ps, int64_t k0,
             for (int c = 0; c < 3; ++c)
                 Gbuf[((k - k0) * 6 + c) * n + i] = rl[c];
         }
-        gj8_correct(q, anew, sprev, S, h, hh, s, r, v);
+        {
+            GJ8Sums qc;
+            double sp[3];
+            for (int c = 0; c < 3; ++c) {
+                qc.T1[c] = pk[c * 64];
+                qc.T2[c] = pk[(3 + c) * 64];
+                qc.alast[c] = pk[(6 + c) * 64];
+                sp[c] = pk[(9 + c) * 64];
+                S[c] = pk[(12 + c) * 64];
+            }
+            gj8_correct(qc, anew, sp, S, h, hh, s, r, v);
+        }
         if (!act)
             continue;
         const int wslot = ring_slot(k);
