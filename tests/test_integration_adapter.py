@@ -1,4 +1,5 @@
-"""The reference-side C++ adapter of INTEGRATION.md (tests/integration/ogp_adapter.hpp) builds
+"""The reference-side C++ adapter (tests/integration/ogp_adapter.hpp: the classes a maintainer
+of the reference would put in front of the C ABI) builds
 against include/ogp.h with a plain host compiler and links libogp.so (CPU), and its driver
 (tests/integration/adapter_test.cpp) passes on the GPU: stepwise GpuPropagator on the caller's
 device buffers equal to the bulk propagation, the IForceModel, GpuStartup and Predictor forms,
