@@ -559,13 +559,13 @@ struct ogp_context {
         // every launch is bracketed by events on its own stream, for its own duration
         std::vector<std::pair<std::unique_ptr<Event>, std::unique_ptr<Event>>> spans;
         auto launch = [&](hipStream_t st, size_t q, int64_t first, int64_t active, int64_t k0,
-                          int shape) {
+                          int shape, int64_t class_active) {
             const int64_t k1 = std::min<int64_t>(k0 + chunk, groups[q].max_steps + 1);
             spans.emplace_back(std::make_unique<Event>(), std::make_unique<Event>());
             HIPC(hipEventRecord(spans.back().first->e, st));
             launch_chunk(st, kind, fctx[q], ps, first, active, k0, k1, cfg.n_corrections, traj, 1,
                 traj_steps, shape_waves(shape), shape_groups(shape), fp_exit,
-                d_evals.as<unsigned long long>());
+                d_evals.as<unsigned long long>(), class_active);
             check_launch();
             HIPC(hipEventRecord(spans.back().second->e, st));
             launches += 1;
@@ -635,7 +635,7 @@ struct ogp_context {
                             continue;
                         const int64_t a = active_in(l.first, l.count, k0[q]);
                         if (a > 0) {
-                            steps = launch(l.st, q, l.first, a, k0[q], shape);
+                            steps = launch(l.st, q, l.first, a, k0[q], shape, active);
                             if (out)
                                 out->emplace_back(spans.size() - 1, (double)a * (double)steps);
                         }

@@ -88,10 +88,14 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
 // that stores the step, bit for bit the point just evaluated (every further evaluation would
 // repeat the last one, so the results are those of all ncorr evaluations). evals (optional)
 // accumulates, over the steps, the evaluations of each step times the objects that store it.
+// concurrent_objects: the objects whose chunks run on the device at the same time as this one
+// (this launch's and those of the other slices of its step-size class; 0 = this launch's
+// alone); it chooses the register budget of the 8-wave gravity instances.
 void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
     int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
     int64_t traj_first, int64_t traj_steps, int waves, int lockstep = 1,
-    bool fixed_point_exit = false, unsigned long long* evals = nullptr);
+    bool fixed_point_exit = false, unsigned long long* evals = nullptr,
+    int64_t concurrent_objects = 0);
 
 // The kParts balanced contiguous column ranges of the harmonic sums (fills kParts + 1 bounds,
 // parts[0] = 0, parts[kParts] = L + 1, non-decreasing).

@@ -328,12 +328,14 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
 // read the same table lines at about the same time. The results do not depend on G.
 // SRC: gravity with acceleration sources (f.src.mask != 0), an instance of its own so that the
 // gravity-only kernel's registers are allocated for the gravity-only path.
-#ifndef OGP_CHUNK_WAVES_PER_EU
-#define OGP_CHUNK_WAVES_PER_EU 4 // resident waves per SIMD the register allocation aims for
-#endif
-template <int KIND, int W, int G, bool SRC = false>
+// WPE: resident waves per SIMD the register allocation aims for. 4 (128 VGPRs) by default; the
+// 8-wave gravity shapes also come with 6 (80 VGPRs: three workgroups per CU instead of two, a
+// few spills), which the launcher takes when the launch has more than two object groups per CU
+// (100,000 objects at degree 70: 581 against 635 ms per propagation; 20,000 at degree 20, about
+// one group per CU: 3% slower, so it keeps 4).
+template <int KIND, int W, int G, bool SRC = false, int WPE = 4>
 __global__ void __launch_bounds__(64 * W * G)
-__attribute__((amdgpu_waves_per_eu(OGP_CHUNK_WAVES_PER_EU)))
+__attribute__((amdgpu_waves_per_eu(WPE)))
 gj8_chunk_kernel(ForceCtx f0, PropState ps, int64_t obj0,
     int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj, int64_t traj_first,
     int64_t traj_steps, int fixed_point_exit, unsigned long long* evals)
@@ -626,26 +628,57 @@ void launch_startup(hipStream_t st, int kind, const ForceCtx& f, int rk8_variant
 #undef L_
 }
 
+// Compute units of the current device (queried once; one device per process).
+int device_cus()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess
+            && hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            cus = n > 0 ? n : 1;
+        else
+            cus = 1;
+    }
+    return cus;
+}
+
+template <int K, int Wv, int G, bool S>
+void launch_chunk_instance(bool occ6, dim3 grid, hipStream_t st, const ForceCtx& f,
+    const PropState& ps, int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr,
+    double* traj, int64_t traj_first, int64_t traj_steps, int fp, unsigned long long* evals)
+{
+    if constexpr ((K == kGravity || K == kGravitySplit) && Wv == 8 && G == 1) {
+        if (occ6) {
+            hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, G, S, 6>), grid, dim3(64 * Wv * G), 0, st,
+                f, ps, obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp, evals);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, G, S, 4>), grid, dim3(64 * Wv * G), 0, st, f, ps,
+        obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp, evals);
+}
+
 template <int K, int Wv, int G>
-void launch_chunk_kernel(bool src, dim3 grid, hipStream_t st, const ForceCtx& f,
+void launch_chunk_kernel(bool src, bool occ6, dim3 grid, hipStream_t st, const ForceCtx& f,
     const PropState& ps, int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr,
     double* traj, int64_t traj_first, int64_t traj_steps, int fp, unsigned long long* evals)
 {
     if constexpr (K == kGravity || K == kGravitySplit) {
         if (src) {
-            hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, G, true>), grid, dim3(64 * Wv * G), 0, st,
-                f, ps, obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp, evals);
+            launch_chunk_instance<K, Wv, G, true>(occ6, grid, st, f, ps, obj0, nobj, k0, k1,
+                ncorr, traj, traj_first, traj_steps, fp, evals);
             return;
         }
     }
-    hipLaunchKernelGGL((gj8_chunk_kernel<K, Wv, G, false>), grid, dim3(64 * Wv * G), 0, st, f,
-        ps, obj0, nobj, k0, k1, ncorr, traj, traj_first, traj_steps, fp, evals);
+    launch_chunk_instance<K, Wv, G, false>(occ6, grid, st, f, ps, obj0, nobj, k0, k1, ncorr,
+        traj, traj_first, traj_steps, fp, evals);
 }
 
 void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& ps,
     int64_t obj0, int64_t nobj, int64_t k0, int64_t k1, int ncorr, double* traj,
     int64_t traj_first, int64_t traj_steps, int waves, int lockstep, bool fixed_point_exit,
-    unsigned long long* evals)
+    unsigned long long* evals, int64_t concurrent_objects)
 {
     if (nobj <= 0 || k1 <= k0)
         return;
@@ -655,11 +688,15 @@ void launch_chunk(hipStream_t st, int kind, const ForceCtx& f, const PropState& 
     const dim3 grid((unsigned)((ngroups + G_ - 1) / G_));
     const int fp_ = fixed_point_exit ? 1 : 0;
     const bool src_ = f.src.mask != 0; // gravity with sources: the SRC instances
+    // more than two 8-wave groups per CU at once (this launch's and those running beside it):
+    // the 6-waves-per-SIMD register allocation, which lets a CU hold three of them
+    const int64_t resident = (std::max(nobj, concurrent_objects) + 63) / 64;
+    const bool occ6 = resident > 2 * (int64_t)device_cus();
 #define L_(K, Wv)                                                                                  \
-    launch_chunk_kernel<K, Wv, 1>(src_, grid, st, f, ps, obj0, nobj, k0, k1, ncorr, traj,         \
+    launch_chunk_kernel<K, Wv, 1>(src_, occ6, grid, st, f, ps, obj0, nobj, k0, k1, ncorr, traj,   \
         traj_first, traj_steps, fp_, evals)
 #define L2_(K, Wv)                                                                                 \
-    launch_chunk_kernel<K, Wv, 2>(src_, grid, st, f, ps, obj0, nobj, k0, k1, ncorr, traj,         \
+    launch_chunk_kernel<K, Wv, 2>(src_, occ6, grid, st, f, ps, obj0, nobj, k0, k1, ncorr, traj,   \
         traj_first, traj_steps, fp_, evals)
     if (G_ == 2) {
         const bool split_ = kind == kGravitySplit;

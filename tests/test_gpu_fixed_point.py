@@ -135,3 +135,21 @@ def test_fixed_point_exit_startup_bit_identical(force):
             out[on] = [res[j][k] for j in range(2) for k in sorted(res[j])]
     for x, y in zip(out[False], out[True]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_register_budgets_bit_identical(precision):
+    # launches of more than two 8-wave groups per CU take the kernel instance compiled for 6
+    # waves per SIMD (fewer registers, a few spills), smaller launches the one for 4: 40,000
+    # objects (625 groups) as one slice and as four slices of ~156 groups, and at W = 4
+    print(f"[register budgets] {precision}", flush=True)
+    d = scenarios.mixed(40_000, seed=57)
+    steps = 5 + (np.arange(40_000) * 7) % 36
+    out = []
+    with ogp.Context(degree=20, precision=precision) as c:
+        for waves, slices in ((8, 1), (8, 4), (4, 1)):
+            c.set_tuning(waves, 64)
+            c.set_slices(slices)
+            out.append(c.propagate(d["r"], d["v"], d["step"], steps))
+    for r, v in out[1:]:
+        assert np.array_equal(r, out[0][0]) and np.array_equal(v, out[0][1])
