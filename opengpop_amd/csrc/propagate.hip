@@ -330,9 +330,9 @@ startup_kernel(ForceCtx f0, int rk8_variant, PropState ps,
 // gravity-only kernel's registers are allocated for the gravity-only path.
 // WPE: resident waves per SIMD the register allocation aims for. 4 (128 VGPRs) by default; the
 // 8-wave gravity shapes also come with 6 (80 VGPRs: three workgroups per CU instead of two, a
-// few spills), which the launcher takes when the launch has more than two object groups per CU
-// (100,000 objects at degree 70: 581 against 635 ms per propagation; 20,000 at degree 20, about
-// one group per CU: 3% slower, so it keeps 4).
+// few spills), which the launcher takes when more than two object groups per CU run at once,
+// counted over the slices of a step-size class (100,000 objects at degree 70: 12.4 against 13.7
+// ms per launch; 20,000 at degree 20, about one group per CU: 3% slower, so it keeps 4).
 template <int KIND, int W, int G, bool SRC = false, int WPE = 4>
 __global__ void __launch_bounds__(64 * W * G)
 __attribute__((amdgpu_waves_per_eu(WPE)))
